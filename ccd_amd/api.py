@@ -1,0 +1,285 @@
+"""ctypes front end of libccdhip.so (the C ABI in include/ccdhip.h).
+
+The library must have been built (python -m ccd_amd.build); importing this
+module never falls back to any other implementation: a missing library raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .data import CcdProblem, FitSpec
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libccdhip.so")
+
+FP64, MIXED = 0, 1
+
+# every symbol include/ccdhip.h declares
+EXPORTS = [
+    "ccdh_last_error", "ccdh_device_info", "ccdh_data_create", "ccdh_data_destroy",
+    "ccdh_data_device_bytes", "ccdh_data_shape", "ccdh_fit_create", "ccdh_fit_clone",
+    "ccdh_fit_destroy", "ccdh_set_weights", "ccdh_fixed_terms_loglik",
+    "ccdh_fixed_terms_grad_hess", "ccdh_zero_xbeta", "ccdh_axpy_xbeta", "ccdh_compute_xbeta",
+    "ccdh_remaining_stats", "ccdh_numerator", "ccdh_grad_hess", "ccdh_update_xbeta",
+    "ccdh_loglik", "ccdh_gradient_objective", "ccdh_pred_loglik", "ccdh_get_xbeta",
+    "ccdh_fit_plain", "ccdh_batch_fit",
+]
+
+
+class CcdhError(RuntimeError):
+    pass
+
+
+class DataDesc(ctypes.Structure):
+    _fields_ = [
+        ("model", ctypes.c_int), ("precision", ctypes.c_int), ("K", ctypes.c_int),
+        ("J", ctypes.c_int), ("col_ptr", ctypes.c_void_p), ("rows", ctypes.c_void_p),
+        ("vals", ctypes.c_void_p), ("fmt", ctypes.c_void_p), ("y", ctypes.c_void_p),
+        ("offs", ctypes.c_void_p), ("strata", ctypes.c_void_p), ("has_offset", ctypes.c_int),
+    ]
+
+
+class FitSpecC(ctypes.Structure):
+    _fields_ = [
+        ("prior_type", ctypes.c_void_p), ("prior_var", ctypes.c_void_p),
+        ("fixed", ctypes.c_void_p), ("beta0", ctypes.c_void_p), ("weights", ctypes.c_void_p),
+        ("heldout", ctypes.c_void_p), ("tolerance", ctypes.c_double),
+        ("max_iterations", ctypes.c_int), ("convergence_type", ctypes.c_int),
+        ("initial_bound", ctypes.c_double), ("max_bound_count", ctypes.c_int),
+    ]
+
+
+class FitResult(ctypes.Structure):
+    _fields_ = [
+        ("log_likelihood", ctypes.c_double), ("log_prior", ctypes.c_double),
+        ("pred_log_likelihood", ctypes.c_double), ("iterations", ctypes.c_int),
+        ("return_flag", ctypes.c_int), ("bound_count", ctypes.c_int), ("_pad", ctypes.c_int),
+        ("seconds", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return dict(log_likelihood=self.log_likelihood, log_prior=self.log_prior,
+                    pred_log_likelihood=self.pred_log_likelihood, iterations=self.iterations,
+                    return_flag=self.return_flag, bound_count=self.bound_count,
+                    seconds=self.seconds)
+
+
+class BatchOptions(ctypes.Structure):
+    _fields_ = [("threads", ctypes.c_int), ("sweeps_per_sync", ctypes.c_int),
+                ("use_graph", ctypes.c_int), ("_pad", ctypes.c_int)]
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libccdhip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise CcdhError(f"{path} not found: build it with `python -m ccd_amd.build`")
+    lib = ctypes.CDLL(path)
+    vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    sig = {
+        "ccdh_last_error": (ctypes.c_char_p, []),
+        "ccdh_device_info": (ci, [vp, ctypes.c_size_t, vp, vp]),
+        "ccdh_data_create": (ci, [vp, vp]),
+        "ccdh_data_destroy": (ci, [vp]),
+        "ccdh_data_device_bytes": (ctypes.c_int64, [vp]),
+        "ccdh_data_shape": (ci, [vp, vp, vp, vp, vp]),
+        "ccdh_fit_create": (ci, [vp, vp]),
+        "ccdh_fit_clone": (ci, [vp, vp]),
+        "ccdh_fit_destroy": (ci, [vp]),
+        "ccdh_set_weights": (ci, [vp, vp, ci]),
+        "ccdh_fixed_terms_loglik": (ci, [vp, ci]),
+        "ccdh_fixed_terms_grad_hess": (ci, [vp, ci]),
+        "ccdh_zero_xbeta": (ci, [vp]),
+        "ccdh_axpy_xbeta": (ci, [vp, cd, ci]),
+        "ccdh_compute_xbeta": (ci, [vp, vp]),
+        "ccdh_remaining_stats": (ci, [vp, ci]),
+        "ccdh_numerator": (ci, [vp, ci, ci]),
+        "ccdh_grad_hess": (ci, [vp, ci, ci, vp, vp]),
+        "ccdh_update_xbeta": (ci, [vp, cd, ci, ci]),
+        "ccdh_loglik": (ci, [vp, ci, vp]),
+        "ccdh_gradient_objective": (ci, [vp, ci, vp]),
+        "ccdh_pred_loglik": (ci, [vp, vp, vp]),
+        "ccdh_get_xbeta": (ci, [vp, vp]),
+        "ccdh_fit_plain": (ci, [vp, vp, vp, vp]),
+        "ccdh_batch_fit": (ci, [vp, ci, vp, vp, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = load_library().ccdh_last_error().decode()
+        raise CcdhError(f"ccdhip error {rc}: {msg}")
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def device_info():
+    lib = load_library()
+    buf = ctypes.create_string_buffer(256)
+    cu, gb = ctypes.c_int(), ctypes.c_double()
+    _check(lib.ccdh_device_info(buf, 256, ctypes.byref(cu), ctypes.byref(gb)))
+    return buf.value.decode(), cu.value, gb.value
+
+
+class Data:
+    """A problem resident in GPU memory (ccdh_data_t)."""
+
+    def __init__(self, p: CcdProblem, precision=FP64):
+        lib = load_library()
+        self.problem = p
+        self.precision = precision
+        self._arrs = dict(
+            col_ptr=np.ascontiguousarray(p.col_ptr, np.int64),
+            rows=np.ascontiguousarray(p.rows, np.int32),
+            vals=np.ascontiguousarray(p.vals, np.float64),
+            fmt=np.ascontiguousarray(p.fmt, np.int8),
+            y=np.ascontiguousarray(p.y, np.float64),
+            offs=np.ascontiguousarray(p.offs, np.float64),
+            strata=np.ascontiguousarray(p.strata, np.int32),
+        )
+        a = self._arrs
+        desc = DataDesc(p.model, precision, p.K, p.J, _p(a["col_ptr"]), _p(a["rows"]),
+                        _p(a["vals"]), _p(a["fmt"]), _p(a["y"]), _p(a["offs"]), _p(a["strata"]),
+                        int(p.has_offset))
+        h = ctypes.c_void_p()
+        _check(lib.ccdh_data_create(ctypes.byref(desc), ctypes.byref(h)))
+        self.handle = h
+        self._arrs = None  # the library copied everything
+
+    @property
+    def device_bytes(self):
+        return load_library().ccdh_data_device_bytes(self.handle)
+
+    def close(self):
+        if self.handle:
+            load_library().ccdh_data_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _spec_struct(spec: FitSpec, keep, heldout=None):
+    arrs = dict(
+        pt=np.ascontiguousarray(spec.prior_type, np.int32),
+        pv=np.ascontiguousarray(spec.prior_var, np.float64),
+        fx=np.ascontiguousarray(spec.fixed, np.uint8),
+        b0=np.ascontiguousarray(spec.beta0, np.float64),
+        w=None if spec.weights is None else np.ascontiguousarray(spec.weights, np.float64),
+        ho=None if heldout is None else np.ascontiguousarray(heldout, np.float64),
+    )
+    keep.append(arrs)
+    return FitSpecC(_p(arrs["pt"]), _p(arrs["pv"]), _p(arrs["fx"]), _p(arrs["b0"]),
+                    _p(arrs["w"]), _p(arrs["ho"]), spec.tolerance, spec.max_iterations,
+                    spec.convergence_type, spec.initial_bound, spec.max_bound_count)
+
+
+def fit_plain(data: Data, spec: FitSpec, heldout=None):
+    """One fit with the host CCD driving per-coordinate kernels."""
+    lib = load_library()
+    keep = []
+    s = _spec_struct(spec, keep, heldout)
+    beta = np.zeros(data.problem.J)
+    res = FitResult()
+    _check(lib.ccdh_fit_plain(data.handle, ctypes.byref(s), _p(beta), ctypes.byref(res)))
+    return beta, res.as_dict()
+
+
+def fit_batch(data: Data, specs, heldouts=None, threads=0, sweeps_per_sync=0, use_graph=1):
+    """Many fits over one resident data set with device-resident sweeps."""
+    lib = load_library()
+    n = len(specs)
+    keep = []
+    arr = (FitSpecC * n)()
+    for i, sp in enumerate(specs):
+        arr[i] = _spec_struct(sp, keep, None if heldouts is None else heldouts[i])
+    betas = np.zeros((n, data.problem.J))
+    res = (FitResult * n)()
+    opt = BatchOptions(threads, sweeps_per_sync, use_graph, 0)
+    _check(lib.ccdh_batch_fit(data.handle, n, arr, _p(betas), res, ctypes.byref(opt)))
+    return betas, [r.as_dict() for r in res]
+
+
+class FitHandle:
+    """Direct access to the per-coordinate operations (AbstractModelSpecifics)."""
+
+    def __init__(self, data: Data):
+        self.lib = load_library()
+        self.data = data
+        h = ctypes.c_void_p()
+        _check(self.lib.ccdh_fit_create(data.handle, ctypes.byref(h)))
+        self.handle = h
+
+    def set_weights(self, w=None, use_cv=False):
+        wa = None if w is None else np.ascontiguousarray(w, np.float64)
+        _check(self.lib.ccdh_set_weights(self.handle, _p(wa), int(use_cv)))
+
+    def fixed_terms(self, use_cv=False):
+        _check(self.lib.ccdh_fixed_terms_loglik(self.handle, int(use_cv)))
+        _check(self.lib.ccdh_fixed_terms_grad_hess(self.handle, int(use_cv)))
+
+    def compute_xbeta(self, beta):
+        b = np.ascontiguousarray(beta, np.float64)
+        _check(self.lib.ccdh_compute_xbeta(self.handle, _p(b)))
+
+    def remaining_stats(self, use_w=False):
+        _check(self.lib.ccdh_remaining_stats(self.handle, int(use_w)))
+
+    def grad_hess(self, j, use_w=False):
+        _check(self.lib.ccdh_numerator(self.handle, j, int(use_w)))
+        g, h = ctypes.c_double(), ctypes.c_double()
+        _check(self.lib.ccdh_grad_hess(self.handle, j, int(use_w), ctypes.byref(g),
+                                       ctypes.byref(h)))
+        return g.value, h.value
+
+    def update(self, delta, j, use_w=False):
+        _check(self.lib.ccdh_update_xbeta(self.handle, float(delta), j, int(use_w)))
+
+    def loglik(self, use_cv=False):
+        v = ctypes.c_double()
+        _check(self.lib.ccdh_loglik(self.handle, int(use_cv), ctypes.byref(v)))
+        return v.value
+
+    def gradient_objective(self, use_cv=False):
+        v = ctypes.c_double()
+        _check(self.lib.ccdh_gradient_objective(self.handle, int(use_cv), ctypes.byref(v)))
+        return v.value
+
+    def pred_loglik(self, w):
+        wa = np.ascontiguousarray(w, np.float64)
+        v = ctypes.c_double()
+        _check(self.lib.ccdh_pred_loglik(self.handle, _p(wa), ctypes.byref(v)))
+        return v.value
+
+    def xbeta(self):
+        out = np.zeros(self.data.problem.K)
+        _check(self.lib.ccdh_get_xbeta(self.handle, _p(out)))
+        return out
+
+    def close(self):
+        if self.handle:
+            self.lib.ccdh_fit_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

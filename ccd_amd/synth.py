@@ -1,0 +1,166 @@
+"""Seeded synthetic problems with the shapes of the benchmark workloads.
+
+  lr_indicator ...... logistic, iid Bernoulli(p) indicator columns (10,000 x 1,000, p=0.01)
+  poisson_zipf ...... Poisson with log-exposure offset, Zipf(s) column frequencies
+                      (1,000,000 x 100,000, ~40 nnz/row)
+  cox_zipf .......... Breslow Cox, integer-day event times with ties, uniform censoring
+                      (2,000,000 x 50,000, 30 nnz/row)
+  clr_strata ........ conditional logistic, strata of 1 case + U{1..10} controls
+                      (200,000 strata x 20,000 columns, 25 nnz/row)
+
+All generators are vectorised numpy and deterministic in their seed.
+"""
+import numpy as np
+
+from .data import (COX, CONDITIONAL_LOGISTIC, INDICATOR, LOGISTIC, POISSON, CcdProblem,
+                   INTERCEPT, DENSE)
+
+DEFAULT_SEED = 20261203
+
+
+def _csc_from_pairs(rows, cols, K, P):
+    """Sorted, de-duplicated (row, col) pairs -> col_ptr, rows (indicator columns)."""
+    key = cols.astype(np.int64) * K + rows.astype(np.int64)
+    key = np.unique(key)
+    c = (key // K).astype(np.int64)
+    r = (key % K).astype(np.int32)
+    col_ptr = np.zeros(P + 1, dtype=np.int64)
+    np.cumsum(np.bincount(c, minlength=P), out=col_ptr[1:])
+    return col_ptr, r
+
+
+def zipf_pairs(rng, K, P, per_row, s):
+    """Each row draws about `per_row` distinct columns with P(col i) ~ 1/(i+1)^s."""
+    w = 1.0 / np.arange(1, P + 1, dtype=np.float64) ** s
+    cdf = np.cumsum(w / w.sum())
+    cdf[-1] = 1.0
+    # heavy columns repeat within a row; oversample so the distinct count is ~per_row
+    draws = int(np.ceil(per_row * 1.25)) + 2
+    out_r, out_c = [], []
+    chunk = max(1, 4_000_000 // draws)
+    for r0 in range(0, K, chunk):
+        r1 = min(K, r0 + chunk)
+        n = r1 - r0
+        u = rng.random((n, draws))
+        c = np.searchsorted(cdf, u).astype(np.int64)
+        c.sort(axis=1)
+        dup = np.zeros_like(c, dtype=bool)
+        dup[:, 1:] = c[:, 1:] == c[:, :-1]
+        # keep the first `per_row` distinct columns of each row
+        keep = ~dup
+        rank = np.cumsum(keep, axis=1)
+        keep &= rank <= per_row
+        rr = np.broadcast_to(np.arange(r0, r1)[:, None], c.shape)
+        out_r.append(rr[keep].astype(np.int32))
+        out_c.append(c[keep])
+    return np.concatenate(out_r), np.concatenate(out_c)
+
+
+def _assemble(model, K, P, col_ptr, rows, y, offs=None, strata=None, offset=None,
+              intercept=False):
+    """Prepend optional offset / intercept columns to an indicator CSC."""
+    extra_fmt, extra_rows, extra_vals, extra_len = [], [], [], []
+    if offset is not None:
+        extra_fmt.append(DENSE)
+        extra_rows.append(np.arange(K, dtype=np.int32))
+        extra_vals.append(np.asarray(offset, np.float64))
+        extra_len.append(K)
+    if intercept:
+        extra_fmt.append(INTERCEPT)
+        extra_rows.append(np.arange(K, dtype=np.int32))
+        extra_vals.append(np.ones(K))
+        extra_len.append(K)
+    nnz = int(col_ptr[-1])
+    J = P + len(extra_fmt)
+    cp = np.zeros(J + 1, dtype=np.int64)
+    off = 0
+    for i, n in enumerate(extra_len):
+        off += n
+        cp[i + 1] = off
+    cp[len(extra_len):] = col_ptr + off
+    all_rows = np.concatenate(extra_rows + [rows]).astype(np.int32) if extra_rows else rows
+    vals = np.concatenate(extra_vals + [np.ones(nnz)]) if extra_vals else np.ones(nnz)
+    fmt = np.array(extra_fmt + [INDICATOR] * P, dtype=np.int8)
+    return CcdProblem(model, K, J, cp, all_rows, vals, fmt, np.asarray(y, np.float64),
+                      np.zeros(K) if offs is None else np.asarray(offs, np.float64),
+                      np.zeros(K, np.int32) if strata is None else np.asarray(strata, np.int32),
+                      has_offset=offset is not None)
+
+
+def _xbeta(col_ptr, rows, K, beta_cols, beta_vals):
+    eta = np.zeros(K)
+    for c, b in zip(beta_cols, beta_vals):
+        eta[rows[col_ptr[c]:col_ptr[c + 1]]] += b
+    return eta
+
+
+def lr_indicator(K=10_000, P=1_000, p=0.01, n_true=50, sd=0.5, intercept=-2.0,
+                 seed=DEFAULT_SEED):
+    rng = np.random.default_rng(seed)
+    mask = rng.random((K, P)) < p
+    r, c = np.nonzero(mask)
+    col_ptr, rows = _csc_from_pairs(r, c, K, P)
+    bc = rng.choice(P, n_true, replace=False)
+    bv = rng.normal(0.0, sd, n_true)
+    eta = _xbeta(col_ptr, rows, K, bc, bv) + intercept
+    y = (rng.random(K) < 1.0 / (1.0 + np.exp(-eta))).astype(np.float64)
+    prob = _assemble(LOGISTIC, K, P, col_ptr, rows, y, intercept=True)
+    return prob, dict(beta_cols=bc, beta_vals=bv)
+
+
+def poisson_zipf(K=1_000_000, P=100_000, per_row=40, s=1.1, n_true=500, sd=0.3,
+                 base=-7.0, seed=DEFAULT_SEED):
+    rng = np.random.default_rng(seed)
+    r, c = zipf_pairs(rng, K, P, per_row, s)
+    col_ptr, rows = _csc_from_pairs(r, c, K, P)
+    bc = rng.choice(P, n_true, replace=False)
+    bv = rng.normal(0.0, sd, n_true)
+    offset = np.log(rng.uniform(30.0, 3650.0, K))
+    eta = offset + _xbeta(col_ptr, rows, K, bc, bv) + base
+    y = rng.poisson(np.exp(eta)).astype(np.float64)
+    prob = _assemble(POISSON, K, P, col_ptr, rows, y, offset=offset, intercept=True)
+    return prob, dict(beta_cols=bc, beta_vals=bv)
+
+
+def cox_zipf(K=2_000_000, P=50_000, per_row=30, s=1.1, n_true=500, sd=0.3, rate=1e-3,
+             censor_max=3650.0, seed=DEFAULT_SEED):
+    rng = np.random.default_rng(seed)
+    r, c = zipf_pairs(rng, K, P, per_row, s)
+    col_ptr, rows = _csc_from_pairs(r, c, K, P)
+    bc = rng.choice(P, n_true, replace=False)
+    bv = rng.normal(0.0, sd, n_true)
+    eta = _xbeta(col_ptr, rows, K, bc, bv)
+    t_event = np.ceil(rng.exponential(1.0 / (np.exp(eta) * rate)))
+    t_cens = np.ceil(rng.uniform(0.0, censor_max, K))
+    y = (t_event <= t_cens).astype(np.float64)
+    time = np.where(y == 1, t_event, t_cens)
+    # sort rows by (stratum, -time, y, rowId) and renumber the covariate rows
+    order = np.lexsort((np.arange(K), y, -time))
+    inv = np.empty(K, dtype=np.int64)
+    inv[order] = np.arange(K)
+    new_rows = inv[rows]
+    cols = np.repeat(np.arange(P), np.diff(col_ptr))
+    col_ptr, rows = _csc_from_pairs(new_rows, cols, K, P)
+    prob = _assemble(COX, K, P, col_ptr, rows, y[order], offs=time[order])
+    return prob, dict(beta_cols=bc, beta_vals=bv)
+
+
+def clr_strata(n_strata=200_000, P=20_000, per_row=25, s=1.1, max_controls=10, n_true=200,
+               sd=0.5, seed=DEFAULT_SEED):
+    rng = np.random.default_rng(seed)
+    sizes = 1 + rng.integers(1, max_controls + 1, n_strata)
+    K = int(sizes.sum())
+    strata = np.repeat(np.arange(n_strata, dtype=np.int32), sizes)
+    r, c = zipf_pairs(rng, K, P, per_row, s)
+    col_ptr, rows = _csc_from_pairs(r, c, K, P)
+    bc = rng.choice(P, n_true, replace=False)
+    bv = rng.normal(0.0, sd, n_true)
+    eta = _xbeta(col_ptr, rows, K, bc, bv)
+    # one case per stratum, chosen with probability proportional to exp(eta)
+    g = rng.gumbel(size=K) + eta
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    best = np.maximum.reduceat(g, starts)
+    is_case = g == np.repeat(best, sizes)
+    y = is_case.astype(np.float64)
+    prob = _assemble(CONDITIONAL_LOGISTIC, K, P, col_ptr, rows, y, strata=strata)
+    return prob, dict(beta_cols=bc, beta_vals=bv, sizes=sizes)

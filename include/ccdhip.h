@@ -1,0 +1,192 @@
+/*
+ * ccdhip.h -- C ABI of the MI355X-native CCD engine (libccdhip.so).
+ *
+ * Plain pointers and sizes only; every function returns 0 on success or a
+ * negative CCDH_E* status, with a message available from ccdh_last_error().
+ * No C++ exception crosses this boundary.  Handles are independent: calls on
+ * different fit handles may come from different host threads; calls on one
+ * handle must be serialised by the caller (the reference's ModelSpecifics is
+ * single-threaded per instance in the same way).
+ *
+ * Reference interfaces each group replaces (Cyclops, src/cyclops/...):
+ *   data handle ........ ModelData<RealType> + CompressedDataMatrix (ModelData.h:178-779,
+ *                        CompressedDataMatrix.h:243-452)
+ *   fit handle ......... one ModelSpecifics<Model,RealType> instance
+ *                        (engine/AbstractModelSpecifics.h:36-229; factory
+ *                        engine/AbstractModelSpecifics.cpp:128-154, GPU slot :32-35)
+ *   per-coordinate ops . the AbstractModelSpecifics virtuals, one C function each
+ *   ccdh_fit_plain ..... CyclicCoordinateDescent::update -> findMode
+ *                        (CyclicCoordinateDescent.cpp:586-615, 909-1216) driving the
+ *                        per-coordinate ops from the host (the "plain" GPU engine)
+ *   ccdh_batch_fit ..... AbstractCrossValidationDriver::doCrossValidationStep's per-fold
+ *                        ccd.update() calls (drivers/AbstractCrossValidationDriver.cpp:131-248),
+ *                        run as device-resident sweeps for many fits at once
+ *   ccdh_server_* ...... no reference counterpart; a request loop over the batched engine
+ */
+#ifndef CCDHIP_H_
+#define CCDHIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define CCDH_OK 0
+#define CCDH_E_INVALID (-1)   /* bad argument / unsupported model-format combination */
+#define CCDH_E_DEVICE (-2)    /* HIP runtime error */
+#define CCDH_E_NOMEM (-3)     /* device allocation failed */
+#define CCDH_E_STATE (-4)     /* call out of order (e.g. gradient before statistics) */
+
+/* column formats: FormatType, CompressedDataMatrix.h:36-38 */
+#define CCDH_DENSE 0
+#define CCDH_SPARSE 1
+#define CCDH_INDICATOR 2
+#define CCDH_INTERCEPT 3
+
+/* models: ModelType, Types.h:146-159 (the values match) */
+#define CCDH_POISSON 2
+#define CCDH_LOGISTIC 3
+#define CCDH_CONDITIONAL_LOGISTIC 4
+#define CCDH_COX 8
+
+/* precision: PrecisionType, Types.h:168-173, plus the mixed path */
+#define CCDH_FP64 0
+#define CCDH_MIXED 1 /* FP32 xBeta / exp streams, FP64 reductions, scans and LL */
+
+/* priors: priors::PriorType, Types.h:101-106 */
+#define CCDH_PRIOR_NONE 0
+#define CCDH_PRIOR_LAPLACE 1
+#define CCDH_PRIOR_NORMAL 2
+
+/* convergence criteria: ConvergenceType, Types.h:110-116 (ZHANG_OLES not offered) */
+#define CCDH_CONV_GRADIENT 0
+#define CCDH_CONV_LANGE 1
+#define CCDH_CONV_MITTAL 2
+#define CCDH_CONV_ONE_STEP 3
+
+/* return flags: UpdateReturnFlags, Types.h:124-130 */
+#define CCDH_SUCCESS 0
+#define CCDH_FAIL 1
+#define CCDH_MAX_ITERATIONS 2
+#define CCDH_ILLCONDITIONED 3
+
+typedef struct ccdh_data* ccdh_data_t;
+typedef struct ccdh_fit* ccdh_fit_t;
+typedef struct ccdh_server* ccdh_server_t;
+
+/* Input data in column-major compressed form (host memory; copied to the GPU).
+ * DENSE/INTERCEPT columns span all K rows (rows[] = 0..K-1); INDICATOR and
+ * INTERCEPT columns carry no values.  Grouped (conditional logistic) rows must
+ * be sorted by stratum; Cox rows by (stratum, -time, y). */
+typedef struct {
+  int model;               /* CCDH_POISSON ... CCDH_COX */
+  int precision;           /* CCDH_FP64 or CCDH_MIXED */
+  int K;                   /* rows */
+  int J;                   /* columns */
+  const int64_t* col_ptr;  /* [J+1] */
+  const int32_t* rows;     /* [nnz] strictly increasing within a column */
+  const double* vals;      /* [nnz] or NULL when no SPARSE/DENSE column */
+  const int8_t* fmt;       /* [J] */
+  const double* y;         /* [K] outcome */
+  const double* offs;      /* [K] event time (Cox); may be NULL otherwise */
+  const int32_t* strata;   /* [K] stratum id (CLR, Cox); NULL = one stratum */
+  int has_offset;          /* column 0 is a DENSE offset with fixed coefficient 1 */
+} ccdh_data_desc;
+
+/* One fit's settings (CCDArguments / ModeFindingArguments, CcdInterface.h:70-146). */
+typedef struct {
+  const int32_t* prior_type;  /* [J] per-coordinate prior (NoPrior for offset/intercept) */
+  const double* prior_var;    /* [J] prior variance (Laplace lambda = sqrt(2/var)) */
+  const uint8_t* fixed;       /* [J] fixBeta */
+  const double* beta0;        /* [J] starting coefficients (offset column 1) */
+  const double* weights;      /* [K] or NULL: NULL means useCrossValidation=false */
+  const double* heldout;      /* [K] or NULL: predictive log likelihood of these weights */
+  double tolerance;           /* default 1e-6 */
+  int max_iterations;         /* default 1000 */
+  int convergence_type;       /* default CCDH_CONV_GRADIENT */
+  double initial_bound;       /* default 2.0 */
+  int max_bound_count;        /* default 5 */
+} ccdh_fit_spec;
+
+typedef struct {
+  double log_likelihood;
+  double log_prior;
+  double pred_log_likelihood; /* 0 unless heldout weights were given */
+  int iterations;
+  int return_flag;
+  int bound_count;
+  int _pad;
+  double seconds;             /* wall time of the fit on the host clock */
+} ccdh_fit_result;
+
+/* ---- library ---- */
+const char* ccdh_last_error(void);
+int ccdh_device_info(char* name, size_t len, int* cu_count, double* hbm_gb);
+
+/* ---- data (ModelData) ---- */
+int ccdh_data_create(const ccdh_data_desc* desc, ccdh_data_t* out);
+int ccdh_data_destroy(ccdh_data_t data);
+/* device bytes held by the data handle (X, outcome, index structures) */
+int64_t ccdh_data_device_bytes(ccdh_data_t data);
+/* model, rows, columns and offset flag of a data handle */
+int ccdh_data_shape(ccdh_data_t data, int* model, int* K, int* J, int* has_offset);
+
+/* ---- per-coordinate engine (AbstractModelSpecifics virtuals) ---- */
+/* ModelSpecifics ctor + initialize(N,K,J,...) (ModelSpecifics.hpp:1899-1952) */
+int ccdh_fit_create(ccdh_data_t data, ccdh_fit_t* out);
+/* clone() (AbstractModelSpecifics.h:103): new state over the same resident data */
+int ccdh_fit_clone(ccdh_fit_t fit, ccdh_fit_t* out);
+int ccdh_fit_destroy(ccdh_fit_t fit);
+/* setWeights(double*, double*, bool) (AbstractModelSpecifics.h:65) */
+int ccdh_set_weights(ccdh_fit_t fit, const double* weights, int use_cv);
+/* computeFixedTermsInLogLikelihood(bool) (:83) */
+int ccdh_fixed_terms_loglik(ccdh_fit_t fit, int use_cv);
+/* computeFixedTermsInGradientAndHessian(bool) (:85), i.e. XjY */
+int ccdh_fixed_terms_grad_hess(ccdh_fit_t fit, int use_cv);
+/* zeroXBeta() / axpyXBeta(double, int) / computeXBeta(double*, bool) (:79,120,122) */
+int ccdh_zero_xbeta(ccdh_fit_t fit);
+int ccdh_axpy_xbeta(ccdh_fit_t fit, double beta, int j);
+int ccdh_compute_xbeta(ccdh_fit_t fit, const double* beta);
+/* computeRemainingStatistics(bool) (:81) */
+int ccdh_remaining_stats(ccdh_fit_t fit, int use_weights);
+/* computeNumeratorForGradient(int, bool) (:72) */
+int ccdh_numerator(ccdh_fit_t fit, int j, int use_weights);
+/* computeGradientAndHessian(int, double*, double*, bool) (:67-68) */
+int ccdh_grad_hess(ccdh_fit_t fit, int j, int use_weights, double* gradient, double* hessian);
+/* updateXBeta(double, int, bool) (:77) */
+int ccdh_update_xbeta(ccdh_fit_t fit, double delta, int j, int use_weights);
+/* getLogLikelihood(bool) (:87) */
+int ccdh_loglik(ccdh_fit_t fit, int use_cv, double* out);
+/* getGradientObjective(bool) (:93) */
+int ccdh_gradient_objective(ccdh_fit_t fit, int use_cv, double* out);
+/* getPredictiveLogLikelihood(double*) (:89) */
+int ccdh_pred_loglik(ccdh_fit_t fit, const double* weights, double* out);
+/* getXBeta() (:114): copies K values */
+int ccdh_get_xbeta(ccdh_fit_t fit, double* out);
+
+/* ---- whole fits ---- */
+/* Plain engine: the host CCD (findMode) calling the per-coordinate ops. */
+int ccdh_fit_plain(ccdh_data_t data, const ccdh_fit_spec* spec, double* beta_out,
+                   ccdh_fit_result* result);
+
+/* Batched engine options. tile_fits / threads = 0 picks tuned defaults. */
+typedef struct {
+  int threads;           /* workgroup size (64..1024) */
+  int sweeps_per_sync;   /* sweeps launched between host convergence checks */
+  int use_graph;         /* capture the sweep launches in a hipGraph and replay it */
+  int _pad;
+} ccdh_batch_options;
+
+/* Batched engine: n_fits fits over one data handle, device-resident sweeps.
+ * betas_out is [n_fits * J]; results [n_fits]. */
+int ccdh_batch_fit(ccdh_data_t data, int n_fits, const ccdh_fit_spec* specs, double* betas_out,
+                   ccdh_fit_result* results, const ccdh_batch_options* options);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CCDHIP_H_ */
