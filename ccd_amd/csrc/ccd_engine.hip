@@ -532,6 +532,7 @@ DevX ccdh_data::view() const {
   x.valsF = valsF;
   x.fmt = fmt;
   x.y = y;
+  x.strata = strata;
   x.segPtr = segPtr;
   x.segOff = segOff;
   x.segKey = segKey;

@@ -1,0 +1,97 @@
+"""GPU parity of the batched (device-resident sweep) engine.
+
+Every fit of a batch matches the CPU oracle run alone to 1e-8 relative
+(coefficients, log likelihood, predictive log likelihood) with the same
+iteration count and return flag; a fit's coefficients are bit-identical alone
+and inside a batch of other fits; graph replay and direct launches agree.
+"""
+import numpy as np
+import pytest
+
+from ccd_amd import api
+from ccd_amd.data import CONDITIONAL_LOGISTIC, PRIOR_LAPLACE, PRIOR_NORMAL, default_spec
+from oracle import oracle as orc
+from tests import fixtures
+from tests.test_gpu_plain import SYNTH, check_fit
+
+pytestmark = pytest.mark.gpu
+
+
+def cv_specs(p, prior, var, maxit, n_fold=4, grid=(0.05, 0.5)):
+    by_pid = p.model == CONDITIONAL_LOGISTIC
+    ids = p.strata if by_pid else np.zeros(p.K, np.int32)
+    specs, held = [], []
+    for v in grid:
+        for fold in range(n_fold):
+            w = orc.cv_weights(ids, n_fold, 17, fold, by_pid=by_pid)
+            specs.append(default_spec(p, prior=prior, variance=v * var, weights=w,
+                                      max_iterations=maxit))
+            held.append(1.0 - w)
+    return specs, held
+
+
+@pytest.mark.parametrize("name", sorted(SYNTH))
+def test_batch_cv_fits_match_oracle(name):
+    p, prior, var, maxit = SYNTH[name]()
+    specs, held = cv_specs(p, prior, var, maxit)
+    d = api.Data(p)
+    bg, ig = api.fit_batch(d, specs, heldouts=held)
+    for i, sp in enumerate(specs):
+        bc, ic = orc.fit(p, sp, heldout=held[i])
+        check_fit(bg[i], ig[i], bc, ic)
+        np.testing.assert_allclose(ig[i]["pred_log_likelihood"], ic["pred_log_likelihood"],
+                                   rtol=1e-8)
+
+
+@pytest.mark.parametrize("name", fixtures.names())
+def test_batch_fixtures_match_oracle(name):
+    p, extra = fixtures.load(name)
+    specs = [default_spec(p), default_spec(p, prior=PRIOR_LAPLACE, variance=1.0),
+             default_spec(p, prior=PRIOR_NORMAL, variance=0.5)]
+    if "weights" in extra:
+        for s in specs:
+            s.weights = extra["weights"]
+    bg, ig = api.fit_batch(api.Data(p), specs)
+    for i, sp in enumerate(specs):
+        bc, ic = orc.fit(p, sp)
+        check_fit(bg[i], ig[i], bc, ic)
+
+
+@pytest.mark.parametrize("name", sorted(SYNTH))
+def test_batch_same_alone_and_in_the_mix(name):
+    p, prior, var, maxit = SYNTH[name]()
+    specs, _ = cv_specs(p, prior, var, maxit, n_fold=3, grid=(0.1, 1.0, 10.0))
+    d = api.Data(p)
+    b_mix, i_mix = api.fit_batch(d, specs)
+    for i in (0, 4, len(specs) - 1):
+        b_one, i_one = api.fit_batch(d, [specs[i]])
+        assert np.array_equal(b_one[0], b_mix[i])
+        assert i_one[0]["iterations"] == i_mix[i]["iterations"]
+        assert i_one[0]["log_likelihood"] == i_mix[i]["log_likelihood"]
+
+
+def test_batch_graph_and_direct_launch_agree():
+    p, prior, var, maxit = SYNTH["clr"]()
+    specs, _ = cv_specs(p, prior, var, maxit, n_fold=2, grid=(1.0,))
+    d = api.Data(p)
+    b1, _ = api.fit_batch(d, specs, use_graph=1, sweeps_per_sync=3)
+    b2, _ = api.fit_batch(d, specs, use_graph=0, sweeps_per_sync=1)
+    assert np.array_equal(b1, b2)
+
+
+@pytest.mark.parametrize("threads", [64, 512, 1024])
+def test_batch_workgroup_sizes(threads):
+    p, prior, var, maxit = SYNTH["lr"]()
+    spec = default_spec(p, prior=prior, variance=var)
+    bc, ic = orc.fit(p, spec)
+    bg, ig = api.fit_batch(api.Data(p), [spec], threads=threads)
+    check_fit(bg[0], ig[0], bc, ic)
+
+
+@pytest.mark.parametrize("name", sorted(SYNTH))
+def test_batch_mixed_precision(name):
+    p, prior, var, maxit = SYNTH[name]()
+    spec = default_spec(p, prior=prior, variance=var, max_iterations=maxit)
+    bc, ic = orc.fit(p, spec)
+    bg, ig = api.fit_batch(api.Data(p, precision=api.MIXED), [spec])
+    check_fit(bg[0], ig[0], bc, ic, rtol=2e-4, atol=2e-5, ll_rtol=1e-6, same_path=False)
