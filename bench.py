@@ -1,0 +1,185 @@
+"""Benchmark: CCD fits/sec, batched (device-resident sweeps) vs plain (per-coordinate)
+GPU engine, on the batched cross-validation workload.
+
+Workload (one "step"): conditional logistic regression, 200,000 strata of 1 case +
+U{1..10} controls (~1.3M rows), 20,000 Zipf(1.1) indicator covariates at 25 nnz per
+row; 10 folds (by stratum) x 16 Laplace variances log-spaced 1e-4..1 = 160 fits,
+cold start each, FP64, convergence 1e-6 ("gradient" criterion).  Synthetic data,
+seeded.  The timed region holds exactly `--steps` batches with the data already
+resident in HBM.
+
+Reported beside it: the plain engine (host CCD with one gradient/Hessian launch,
+one 16-byte readback and one update launch per coordinate) on a sample of the same
+fits, the CPU oracle on a sample (1 core), the largest coefficient difference of
+the batch against the CPU oracle on those fits, per-fit latency p50/p99 inside
+the batch, and the sweep kernel's achieved algorithmic bandwidth.
+
+  python bench.py [--steps 1] [--warmup 0] [--scale 1.0] [--plain-fits 2] [--cpu-fits 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "CCD fits/sec batched vs plain GPU at 1e-6 coef tol; p50/p99 fit latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(stats, nnz_free, precision):
+    """Bytes the batched CLR sweeps must move (indicator columns, v = 0):
+    per fit  gather n_j*s_r + 2*s_g per stratum run, update 3*s_r per nonzero +
+    2*s_g per run, convergence pass 2*s_r per row; index terms (row + stratum,
+    2*i per nonzero) counted once per sweep round for the whole batch."""
+    i, s_g = 4, 8
+    s_r = 8 if precision == 0 else 4
+    state = (stats["gather_nnz"] * s_r + stats["gather_runs"] * 2 * s_g
+             + stats["update_nnz"] * 3 * s_r + stats["update_runs"] * 2 * s_g
+             + stats["objective_rows"] * 2 * s_r)
+    u = stats["update_nnz"] / max(1, stats["gather_nnz"])
+    index = stats["sweep_rounds"] * nnz_free * 2 * i * (1.0 + u)
+    return state + index
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--scale", type=float, default=1.0, help="fraction of strata and covariates")
+    ap.add_argument("--plain-fits", type=int, default=2)
+    ap.add_argument("--cpu-fits", type=int, default=1)
+    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--sweeps-per-sync", type=int, default=0)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--folds", type=int, default=10)
+    ap.add_argument("--grid", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "independent fits would shard across GPUs by fit, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    from ccd_amd import api, synth
+    from ccd_amd.data import CONDITIONAL_LOGISTIC, PRIOR_LAPLACE, default_spec
+
+    name, n_cu, hbm = api.device_info()
+    log("device", name, n_cu, "CUs", f"{hbm:.0f} GB")
+    n_strata = max(100, int(200_000 * args.scale))
+    P = max(50, int(20_000 * args.scale))
+    t = time.perf_counter()
+    prob, _ = synth.clr_strata(n_strata=n_strata, P=P, per_row=25, s=1.1, max_controls=10,
+                               n_true=200, sd=0.5)
+    log(f"data K={prob.K} J={prob.J} nnz={prob.nnz} generated in {time.perf_counter() - t:.1f}s")
+    precision = api.FP64 if args.precision == "fp64" else api.MIXED
+    data = api.Data(prob, precision=precision)
+    log(f"resident: {data.device_bytes / 1e9:.2f} GB")
+
+    folds, grid = args.folds, args.grid
+    specs = []
+    for g in range(grid):
+        var = api.cv_grid_point(1e-4, 1.0, grid, g)
+        for f in range(folds):
+            w = api.cv_fold_weights(prob.strata, folds, 123, f, by_pid=True)
+            specs.append(default_spec(prob, prior=PRIOR_LAPLACE, variance=var, weights=w))
+    n_fits = len(specs)
+    opt = dict(threads=args.threads, sweeps_per_sync=args.sweeps_per_sync, use_graph=1)
+
+    # load the code objects and touch the allocator (untimed)
+    mini = [default_spec(prob, prior=PRIOR_LAPLACE, variance=1.0, max_iterations=1)]
+    api.fit_batch(data, mini, **opt)
+    for _ in range(args.warmup):
+        api.fit_batch(data, specs, **opt)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        betas, infos = api.fit_batch(data, specs, **opt)
+    elapsed = time.perf_counter() - t0
+    stats = api.batch_last_stats()
+    value = n_fits * args.steps / elapsed
+    lat = np.array([r["seconds"] for r in infos])
+    iters = np.array([r["iterations"] for r in infos])
+    flags = np.array([r["return_flag"] for r in infos])
+    log(f"batched: {n_fits} fits in {elapsed / args.steps:.2f}s/step -> {value:.2f} fits/s; "
+        f"iterations min/median/max {iters.min()}/{int(np.median(iters))}/{iters.max()}; "
+        f"flags {np.bincount(flags, minlength=4).tolist()}")
+
+    nnz_free = int(prob.nnz)
+    alg = algorithmic_bytes(stats, nnz_free, precision)
+    per_launch_ms = stats["kernel_ms"] / max(1, stats["launches"])
+    achieved = alg / max(1, stats["launches"]) / (per_launch_ms * 1e-3) / 1e9
+    log(f"sweep kernel: {stats['launches']} launches, {per_launch_ms:.2f} ms avg, "
+        f"{alg / 1e9:.2f} GB algorithmic -> {achieved:.1f} GB/s")
+
+    # plain engine and CPU oracle on a spread of the same fits
+    sample = np.linspace(0, n_fits - 1, max(args.plain_fits, args.cpu_fits, 1)).astype(int)
+    plain_t, plain_err = [], 0.0
+    for idx in sample[: args.plain_fits]:
+        t = time.perf_counter()
+        bp, ip = api.fit_plain(data, specs[idx])
+        plain_t.append(time.perf_counter() - t)
+        d = np.abs(bp - betas[idx]) / np.maximum(np.abs(betas[idx]), 1e-12)
+        plain_err = max(plain_err, float(np.max(np.where(np.abs(betas[idx]) > 0, d, np.abs(bp)))))
+        log(f"plain fit {idx}: {plain_t[-1]:.2f}s, {ip['iterations']} iterations "
+            f"(batched {infos[idx]['iterations']})")
+    plain_fps = len(plain_t) / sum(plain_t) if plain_t else None
+
+    cpu = None
+    coef_err = None
+    if args.cpu_fits > 0:
+        from oracle import oracle as orc
+        cpu_t, errs = [], []
+        for idx in sample[: args.cpu_fits]:
+            t = time.perf_counter()
+            bc, ic = orc.fit(prob, specs[idx])
+            cpu_t.append(time.perf_counter() - t)
+            scale = max(1.0, float(np.max(np.abs(bc))))
+            errs.append(float(np.max(np.abs(betas[idx] - bc)) / scale))
+            log(f"cpu oracle fit {idx}: {cpu_t[-1]:.2f}s, {ic['iterations']} iterations")
+        coef_err = max(errs)
+        cpu = {"value": len(cpu_t) / sum(cpu_t), "unit": "fits/s", "cores": 1, "kind": "port",
+               "sample": f"{len(cpu_t)} of the {n_fits} CV fits (indices {sample[:args.cpu_fits].tolist()}), "
+                         "single-threaded FP64 oracle"}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "fits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == api.FP64 else "f32/f64",
+        "data": "synthetic",
+        "config": {"workload": "batched CV, conditional logistic", "strata": n_strata,
+                   "rows": prob.K, "covariates": prob.J, "nnz": int(prob.nnz),
+                   "folds": folds, "variances": grid, "fits": n_fits,
+                   "prior": "Laplace var 1e-4..1", "tolerance": 1e-6},
+        "plain_gpu_fits_per_sec": plain_fps,
+        "speedup_vs_plain_gpu": (value / plain_fps) if plain_fps else None,
+        "p50_fit_latency_s": float(np.percentile(lat, 50)),
+        "p99_fit_latency_s": float(np.percentile(lat, 99)),
+        "max_rel_coef_err_vs_cpu_oracle": coef_err,
+        "max_rel_coef_diff_plain_vs_batched": plain_err,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_batch_sweep", "avg_launch_ms": per_launch_ms,
+                     "launches": stats["launches"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

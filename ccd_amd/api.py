@@ -23,7 +23,8 @@ EXPORTS = [
     "ccdh_fixed_terms_grad_hess", "ccdh_zero_xbeta", "ccdh_axpy_xbeta", "ccdh_compute_xbeta",
     "ccdh_remaining_stats", "ccdh_numerator", "ccdh_grad_hess", "ccdh_update_xbeta",
     "ccdh_loglik", "ccdh_gradient_objective", "ccdh_pred_loglik", "ccdh_get_xbeta",
-    "ccdh_fit_plain", "ccdh_batch_fit",
+    "ccdh_fit_plain", "ccdh_batch_fit", "ccdh_batch_last_stats", "ccdh_cv_fold_weights",
+    "ccdh_cv_grid_point",
 ]
 
 
@@ -63,6 +64,16 @@ class FitResult(ctypes.Structure):
                     pred_log_likelihood=self.pred_log_likelihood, iterations=self.iterations,
                     return_flag=self.return_flag, bound_count=self.bound_count,
                     seconds=self.seconds)
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "launches", "sweeps", "sweep_rounds", "gather_nnz", "gather_runs", "update_nnz",
+        "update_runs", "scan_pids", "objective_rows")] + [
+        ("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 class BatchOptions(ctypes.Structure):
@@ -108,6 +119,9 @@ def load_library(path=LIB_PATH):
         "ccdh_get_xbeta": (ci, [vp, vp]),
         "ccdh_fit_plain": (ci, [vp, vp, vp, vp]),
         "ccdh_batch_fit": (ci, [vp, ci, vp, vp, vp, vp]),
+        "ccdh_batch_last_stats": (ci, [vp]),
+        "ccdh_cv_fold_weights": (ci, [vp, ci, ci, ci, ctypes.c_long, ci, vp]),
+        "ccdh_cv_grid_point": (cd, [cd, cd, ci, ci]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -215,6 +229,25 @@ def fit_batch(data: Data, specs, heldouts=None, threads=0, sweeps_per_sync=0, us
     opt = BatchOptions(threads, sweeps_per_sync, use_graph, 0)
     _check(lib.ccdh_batch_fit(data.handle, n, arr, _p(betas), res, ctypes.byref(opt)))
     return betas, [r.as_dict() for r in res]
+
+
+def batch_last_stats():
+    st = BatchStats()
+    _check(load_library().ccdh_batch_last_stats(ctypes.byref(st)))
+    return st.as_dict()
+
+
+def cv_fold_weights(ids, n_fold, seed, task, by_pid=True):
+    """Fold weights of CV task `task` (fold = task % n_fold), 0 for held-out units."""
+    ids = np.ascontiguousarray(ids, np.int32)
+    out = np.zeros(len(ids))
+    _check(load_library().ccdh_cv_fold_weights(_p(ids), len(ids), int(by_pid), n_fold, seed,
+                                               task, _p(out)))
+    return out
+
+
+def cv_grid_point(lower, upper, size, step):
+    return load_library().ccdh_cv_grid_point(lower, upper, size, step)
 
 
 class FitHandle:

@@ -60,6 +60,8 @@ struct FitStatus {
   int flag;
   int done;
   int N;  // groups of this fit (Cox: tie groups under its weights)
+  // work counters (thread 0), read by the host for the bandwidth figure
+  long long gatherNnz, gatherRuns, updateNnz, updateRuns, scanPids, objRows;
 };
 
 template <typename R>
@@ -533,6 +535,11 @@ __global__ void k_batch_sweep(BatchArgs<R> a, const double* fixedLL, int sweeps)
         blockSum<2>(v);
       }
       if (threadIdx.x == 0) {
+        FitStatus& cs = a.st[f];
+        const long long runs = MODEL == M_CLR ? (long long)(d.segPtr[j + 1] - d.segPtr[j]) : 0;
+        cs.gatherNnz += nj;
+        cs.gatherRuns += runs;
+        if (MODEL == M_COX && nj > 0) cs.scanPids += N;
         double g = nj > 0 ? v[0] - xjy[j] : 0.0;
         double h = nj > 0 ? v[1] : 0.0;
         if (h < 0.0) {
@@ -541,7 +548,11 @@ __global__ void k_batch_sweep(BatchArgs<R> a, const double* fixedLL, int sweeps)
         }
         double delta = priorDelta(ptype[j], pvar[j], g, h, beta[j]);
         delta = applyBounds(delta, &bound[j]);
-        if (delta != 0.0) beta[j] += delta;
+        if (delta != 0.0) {
+          beta[j] += delta;
+          cs.updateNnz += nj;
+          cs.updateRuns += runs;
+        }
         sDelta = delta;
       }
       __syncthreads();
@@ -562,6 +573,7 @@ __global__ void k_batch_sweep(BatchArgs<R> a, const double* fixedLL, int sweeps)
     if (threadIdx.x == 0) {
       FitStatus& s = a.st[f];
       s.iteration += 1;
+      s.objRows += (c.convergenceType == 0 ? a.K : (c.convergenceType == 3 ? 0 : a.K + N));
       double conv;
       bool ill = false;
       if (obj != obj) {
@@ -622,6 +634,8 @@ __global__ void k_batch_finish(BatchArgs<R> a, const double* fixedLL) {
 // ================================================================ host side
 namespace {
 
+thread_local ccdh_batch_stats g_stats;
+
 size_t roundUp(size_t n, size_t m) { return (n + m - 1) / m * m; }
 
 template <typename T>
@@ -641,8 +655,11 @@ struct Owned {
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   FitStatus* hst = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
   ~Owned() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (void* p : ptrs) (void)hipFree(p);
@@ -806,15 +823,33 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
     HIP_TRY(hipGraphInstantiate(&o.exec, o.graph, nullptr, nullptr, 0));
   }
   std::vector<double> doneAt(F, 0.0);
+  std::vector<int> lastIter(F, 0);
+  std::memset(&g_stats, 0, sizeof(g_stats));
+  HIP_TRY(hipEventCreate(&o.ev0));
+  HIP_TRY(hipEventCreate(&o.ev1));
   int guard = 0;
   for (;;) {
+    HIP_TRY(hipEventRecord(o.ev0, s));
     if (useGraph) {
       HIP_TRY(hipGraphLaunch(o.exec, s));
     } else {
       rc = enqueueSweeps();
       if (rc) return rc;
     }
+    HIP_TRY(hipEventRecord(o.ev1, s));
     HIP_TRY(hipStreamSynchronize(s));
+    {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, o.ev0, o.ev1));
+      g_stats.kernel_ms += ms;
+      g_stats.launches += 1;
+      int rounds = 0;
+      for (int f = 0; f < F; ++f) {
+        rounds = std::max(rounds, o.hst[f].iteration - lastIter[f]);
+        lastIter[f] = o.hst[f].iteration;
+      }
+      g_stats.sweep_rounds += rounds;
+    }
     const double now = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     bool allDone = true, retry = false;
     for (int f = 0; f < F; ++f) {
@@ -835,6 +870,7 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
         HIP_TRY(hipMemcpyAsync(dctl + f, &ctl[f], sizeof(FitCtl), hipMemcpyHostToDevice, s));
         HIP_TRY(hipStreamSynchronize(s));
         doneAt[f] = 0.0;
+        lastIter[f] = 0;
         retry = true;
         active[0] = f;
         HIP_TRY(hipMemcpyAsync(dactive, active.data(), sizeof(int32_t), hipMemcpyHostToDevice, s));
@@ -860,6 +896,16 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
                            hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   const double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int f = 0; f < F; ++f) {
+    g_stats.sweeps += fin[f].iteration;
+    g_stats.gather_nnz += fin[f].gatherNnz;
+    g_stats.gather_runs += fin[f].gatherRuns;
+    g_stats.update_nnz += fin[f].updateNnz;
+    g_stats.update_runs += fin[f].updateRuns;
+    g_stats.scan_pids += fin[f].scanPids;
+    g_stats.objective_rows += fin[f].objRows;
+  }
+  g_stats.wall_ms = total * 1e3;
   for (int f = 0; f < F; ++f) {
     ccdh_fit_result& r = results[f];
     r.log_likelihood = fin[f].logLik;
@@ -899,6 +945,12 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
 }
 
 }  // namespace
+
+extern "C" int ccdh_batch_last_stats(ccdh_batch_stats* out) {
+  if (!out) return setError(CCDH_E_INVALID, "null argument");
+  *out = g_stats;
+  return 0;
+}
 
 extern "C" int ccdh_batch_fit(ccdh_data_t data, int n_fits, const ccdh_fit_spec* specs,
                               double* betas_out, ccdh_fit_result* results,

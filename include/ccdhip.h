@@ -185,6 +185,33 @@ typedef struct {
 int ccdh_batch_fit(ccdh_data_t data, int n_fits, const ccdh_fit_spec* specs, double* betas_out,
                    ccdh_fit_result* results, const ccdh_batch_options* options);
 
+/* Counters of the last ccdh_batch_fit on this host thread, for the roofline:
+ * kernel time of the sweep launches (HIP events on the batch stream) and the
+ * work they did, summed over fits. */
+typedef struct {
+  int64_t launches;        /* sweep launches (graph replays or direct) */
+  int64_t sweeps;          /* completed CCD cycles, summed over fits */
+  int64_t sweep_rounds;    /* max over fits of cycles per launch, summed over launches */
+  int64_t gather_nnz;      /* column nonzeros read by gradient/Hessian gathers */
+  int64_t gather_runs;     /* stratum runs (CLR) / touched pids (Cox) of those gathers */
+  int64_t update_nnz;      /* column nonzeros updated (coordinates with delta != 0) */
+  int64_t update_runs;
+  int64_t scan_pids;       /* pids swept by the Cox risk-set scans */
+  int64_t objective_rows;  /* rows read by the per-sweep convergence objective */
+  double kernel_ms;        /* summed HIP-event time of the sweep launches */
+  double wall_ms;          /* host wall time of the whole batch */
+} ccdh_batch_stats;
+int ccdh_batch_last_stats(ccdh_batch_stats* out);
+
+/* ---- cross-validation batch definition ---- */
+/* CrossValidationSelector::getWeights after the permutations of `task`
+ * (drivers/CrossValidationSelector.cpp:21-116,138-143): ids are the exchangeable
+ * unit of each row (stratum for by_pid, ignored otherwise). */
+int ccdh_cv_fold_weights(const int32_t* ids, int K, int by_pid, int n_fold, long seed, int task,
+                         double* weights_out);
+/* GridSearchCrossValidationDriver::computeGridPoint (GridSearchCrossValidationDriver.cpp:38-48) */
+double ccdh_cv_grid_point(double lower, double upper, int size, int step);
+
 #ifdef __cplusplus
 }
 #endif
