@@ -230,6 +230,212 @@ __device__ void ghCox(const BatchArgs<R>& a, int j, int fmt, const R* oeb, const
   }
 }
 
+// ------------------------------------------------ chunked gathers / updates
+// The column is processed in chunks of blockDim*IT nonzeros.  Every lane first
+// issues all of its index loads (row, stratum key: contiguous), then all of its
+// state loads (exp(xBeta), denominators, event weights), and only then
+// computes, so a chunk costs two memory latencies whatever IT is.  Per-stratum
+// sums of the grouped model come from a block-level segmented scan whose carry
+// crosses chunk boundaries; the lane at a run's last nonzero owns the stratum.
+constexpr int kIT = 4;
+
+template <int MODEL, typename R>
+__device__ void ghIndepC(const BatchArgs<R>& a, int j, int fmt, const R* oeb, const double* kw,
+                         int useW, double (&v)[2]) {
+  const DevX& d = a.d;
+  const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
+  const bool ind = isIndicator(fmt);
+  const int64_t chunk = (int64_t)blockDim.x * kIT;
+  for (int64_t c0 = b; c0 < e; c0 += chunk) {
+    const int64_t base = c0 + (int64_t)threadIdx.x * kIT;
+    int row[kIT];
+    R x[kIT], o[kIT];
+    double w[kIT];
+#pragma unroll
+    for (int it = 0; it < kIT; ++it) {
+      const int64_t p = base + it;
+      row[it] = p < e ? colRow(d, fmt, p, b) : -1;
+      x[it] = p < e ? colValue<R>(d, fmt, p) : (R)0;
+    }
+#pragma unroll
+    for (int it = 0; it < kIT; ++it) {
+      o[it] = row[it] >= 0 ? oeb[row[it]] : (R)0;
+      w[it] = (useW && row[it] >= 0) ? kw[row[it]] : 1.0;
+    }
+#pragma unroll
+    for (int it = 0; it < kIT; ++it) {
+      if (row[it] < 0) continue;
+      const R n1 = o[it] * x[it];
+      if (MODEL == M_LOGISTIC) {
+        const R den = (R)1 + o[it];
+        const R t = n1 / den;
+        const R hh = ind ? t * ((R)1 - t) : ((n1 * x[it]) / den - t * t);
+        v[0] += useW ? w[it] * (double)t : (double)t;
+        v[1] += useW ? w[it] * (double)hh : (double)hh;
+      } else {
+        const R n2 = ind ? n1 : n1 * x[it];
+        v[0] += useW ? w[it] * (double)n1 : (double)n1;
+        v[1] += useW ? w[it] * (double)n2 : (double)n2;
+      }
+    }
+  }
+}
+
+template <typename R>
+__device__ void ghClrC(const BatchArgs<R>& a, int j, int fmt, const R* oeb, const double* denom,
+                       const double* nw, double (&v)[2]) {
+  const DevX& d = a.d;
+  const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
+  const bool ind = isIndicator(fmt);
+  const int64_t chunk = (int64_t)blockDim.x * kIT;
+  Seg<2> carry;
+  carry.f = 0;
+  carry.v[0] = carry.v[1] = 0.0;
+  for (int64_t c0 = b; c0 < e; c0 += chunk) {
+    const int64_t base = c0 + (int64_t)threadIdx.x * kIT;
+    int row[kIT], key[kIT + 2];
+    R x[kIT], o[kIT];
+    double den[kIT], wn[kIT];
+    // keys of base-1 .. base+kIT (run starts and ends)
+#pragma unroll
+    for (int it = 0; it < kIT + 2; ++it) {
+      const int64_t p = base - 1 + it;
+      key[it] = (p >= b && p < e) ? d.nzKey[p] : -1;
+    }
+#pragma unroll
+    for (int it = 0; it < kIT; ++it) {
+      const int64_t p = base + it;
+      row[it] = p < e ? colRow(d, fmt, p, b) : -1;
+      x[it] = p < e ? colValue<R>(d, fmt, p) : (R)0;
+    }
+#pragma unroll
+    for (int it = 0; it < kIT; ++it) {
+      const bool last = row[it] >= 0 && key[it + 2] != key[it + 1];
+      o[it] = row[it] >= 0 ? oeb[row[it]] : (R)0;
+      den[it] = last ? denom[key[it + 1]] : 1.0;
+      wn[it] = last ? nw[key[it + 1]] : 0.0;
+    }
+    Seg<2> items[kIT];
+    Seg<2> agg;
+    agg.f = 0;
+    agg.v[0] = agg.v[1] = 0.0;
+#pragma unroll
+    for (int it = 0; it < kIT; ++it) {
+      Seg<2> q;
+      q.f = (row[it] >= 0 && key[it + 1] != key[it]) ? 1 : 0;
+      const R n1 = o[it] * x[it];
+      q.v[0] = row[it] >= 0 ? (double)n1 : 0.0;
+      q.v[1] = (row[it] >= 0 && !ind) ? (double)(n1 * x[it]) : 0.0;
+      items[it] = q;
+      agg = segCombine(agg, q);
+    }
+    Seg<2> total;
+    Seg<2> run = segCombine(carry, blockSegScanExcl<2>(agg, total));
+#pragma unroll
+    for (int it = 0; it < kIT; ++it) {
+      run = segCombine(run, items[it]);
+      if (row[it] >= 0 && key[it + 2] != key[it + 1]) {
+        const double t = run.v[0] / den[it];
+        const double g = wn[it] * t;
+        v[0] += g;
+        v[1] += ind ? g * (1.0 - t) : wn[it] * (run.v[1] / den[it] - t * t);
+      }
+    }
+    carry = segCombine(carry, total);
+  }
+}
+
+template <typename R>
+__device__ void updIndepC(const BatchArgs<R>& a, int j, int fmt, R* xb, R* oeb, double delta) {
+  const DevX& d = a.d;
+  const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
+  const R rd = (R)delta;
+  const int64_t chunk = (int64_t)blockDim.x * kIT;
+  for (int64_t c0 = b; c0 < e; c0 += chunk) {
+    const int64_t base = c0 + (int64_t)threadIdx.x * kIT;
+    int row[kIT];
+    R x[kIT], xo[kIT];
+#pragma unroll
+    for (int it = 0; it < kIT; ++it) {
+      const int64_t p = base + it;
+      row[it] = p < e ? colRow(d, fmt, p, b) : -1;
+      x[it] = p < e ? colValue<R>(d, fmt, p) : (R)0;
+    }
+#pragma unroll
+    for (int it = 0; it < kIT; ++it) xo[it] = row[it] >= 0 ? xb[row[it]] : (R)0;
+#pragma unroll
+    for (int it = 0; it < kIT; ++it) {
+      if (row[it] < 0) continue;
+      const R nx = xo[it] + rd * x[it];
+      xb[row[it]] = nx;
+      oeb[row[it]] = rexp(nx);
+    }
+  }
+}
+
+template <typename R>
+__device__ void updClrC(const BatchArgs<R>& a, int j, int fmt, R* xb, R* oeb, double* denom,
+                        double delta) {
+  const DevX& d = a.d;
+  const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
+  const R rd = (R)delta;
+  const int64_t chunk = (int64_t)blockDim.x * kIT;
+  Seg<1> carry;
+  carry.f = 0;
+  carry.v[0] = 0.0;
+  for (int64_t c0 = b; c0 < e; c0 += chunk) {
+    const int64_t base = c0 + (int64_t)threadIdx.x * kIT;
+    int row[kIT], key[kIT + 2];
+    R x[kIT], xo[kIT], oo[kIT];
+    double den[kIT];
+#pragma unroll
+    for (int it = 0; it < kIT + 2; ++it) {
+      const int64_t p = base - 1 + it;
+      key[it] = (p >= b && p < e) ? d.nzKey[p] : -1;
+    }
+#pragma unroll
+    for (int it = 0; it < kIT; ++it) {
+      const int64_t p = base + it;
+      row[it] = p < e ? colRow(d, fmt, p, b) : -1;
+      x[it] = p < e ? colValue<R>(d, fmt, p) : (R)0;
+    }
+#pragma unroll
+    for (int it = 0; it < kIT; ++it) {
+      const bool last = row[it] >= 0 && key[it + 2] != key[it + 1];
+      xo[it] = row[it] >= 0 ? xb[row[it]] : (R)0;
+      oo[it] = row[it] >= 0 ? oeb[row[it]] : (R)0;
+      den[it] = last ? denom[key[it + 1]] : 0.0;
+    }
+    Seg<1> items[kIT];
+    Seg<1> agg;
+    agg.f = 0;
+    agg.v[0] = 0.0;
+#pragma unroll
+    for (int it = 0; it < kIT; ++it) {
+      Seg<1> q;
+      q.f = (row[it] >= 0 && key[it + 1] != key[it]) ? 1 : 0;
+      q.v[0] = 0.0;
+      if (row[it] >= 0) {
+        const R nx = xo[it] + rd * x[it];
+        const R no = rexp(nx);
+        xb[row[it]] = nx;
+        oeb[row[it]] = no;
+        q.v[0] = (double)no - (double)oo[it];
+      }
+      items[it] = q;
+      agg = segCombine(agg, q);
+    }
+    Seg<1> total;
+    Seg<1> run = segCombine(carry, blockSegScanExcl<1>(agg, total));
+#pragma unroll
+    for (int it = 0; it < kIT; ++it) {
+      run = segCombine(run, items[it]);
+      if (row[it] >= 0 && key[it + 2] != key[it + 1]) denom[key[it + 1]] = den[it] + run.v[0];
+    }
+    carry = segCombine(carry, total);
+  }
+}
+
 // ------------------------------------------------------------ update pieces
 template <typename R>
 __device__ void updIndep(const BatchArgs<R>& a, int j, int fmt, R* xb, R* oeb, double delta) {
@@ -527,9 +733,9 @@ __global__ void k_batch_sweep(BatchArgs<R> a, const double* fixedLL, int sweeps)
       double v[2] = {0.0, 0.0};
       if (nj > 0) {
         if (MODEL == M_LOGISTIC || MODEL == M_POISSON)
-          ghIndep<MODEL, R>(a, j, fmt, oeb, kw, c.useW, v);
+          ghIndepC<MODEL, R>(a, j, fmt, oeb, kw, c.useW, v);
         else if (MODEL == M_CLR)
-          ghClr<R>(a, j, fmt, oeb, denom, nw, v);
+          ghClrC<R>(a, j, fmt, oeb, denom, nw, v);
         else
           ghCox<R>(a, j, fmt, oeb, kw, c.useW, pid, N, denom, numer, numer2, reset, nw, v);
         blockSum<2>(v);
@@ -559,9 +765,9 @@ __global__ void k_batch_sweep(BatchArgs<R> a, const double* fixedLL, int sweeps)
       const double delta = sDelta;
       if (delta != 0.0) {
         if (MODEL == M_LOGISTIC || MODEL == M_POISSON)
-          updIndep<R>(a, j, fmt, xb, oeb, delta);
+          updIndepC<R>(a, j, fmt, xb, oeb, delta);
         else if (MODEL == M_CLR)
-          updClr<R>(a, j, fmt, xb, oeb, denom, delta);
+          updClrC<R>(a, j, fmt, xb, oeb, denom, delta);
         else
           updCox<R>(a, j, fmt, xb, oeb, denom, kw, c.useW, pid, N, delta);
       }

@@ -36,6 +36,7 @@ struct DevX {
   const int32_t* segOff;    // [nseg] run start, relative to colPtr[j]
   const int32_t* segKey;    // [nseg] stratum of the run
   const int32_t* stratStart;// [nStrata+1] first row of each stratum
+  const int32_t* nzKey;     // [nnz] stratum of each nonzero's row (grouped models)
 };
 
 template <typename R>

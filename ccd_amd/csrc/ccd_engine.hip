@@ -537,6 +537,7 @@ DevX ccdh_data::view() const {
   x.segOff = segOff;
   x.segKey = segKey;
   x.stratStart = stratStart;
+  x.nzKey = nzKey;
   return x;
 }
 
@@ -671,6 +672,23 @@ int dataCreate(const ccdh_data_desc* desc, ccdh_data** out) {
       }
       segPtr[j + 1] = (int64_t)segOff.size();
     }
+    // stratum of every nonzero, stored beside the row index for one-hop gathers
+    {
+      std::vector<int32_t> key((size_t)nnz);
+      for (int j = 0; j < J; ++j) {
+        const int64_t b = desc->col_ptr[j], e = desc->col_ptr[j + 1];
+        const int f = desc->fmt[j];
+        for (int64_t p = b; p < e; ++p) {
+          const int k = (f == CCDH_DENSE || f == CCDH_INTERCEPT) ? (int)(p - b) : desc->rows[p];
+          key[p] = d->hStrata[k];
+        }
+      }
+      rc = devUpload(&d->nzKey, key.data(), key.size(), &acc);
+      if (rc) {
+        dataDestroy(d);
+        return rc;
+      }
+    }
     std::vector<int32_t> ss(nStrata + 1, 0);
     for (int k = 0; k < K; ++k) ss[d->hStrata[k] + 1]++;
     for (int s = 0; s < nStrata; ++s) ss[s + 1] += ss[s];
@@ -692,7 +710,7 @@ int dataCreate(const ccdh_data_desc* desc, ccdh_data** out) {
 void dataDestroy(ccdh_data* d) {
   if (!d) return;
   void* ptrs[] = {d->colPtr, d->rows, d->vals, d->valsF, d->fmt, d->y, d->strata,
-                  d->segPtr, d->segOff, d->segKey, d->stratStart};
+                  d->segPtr, d->segOff, d->segKey, d->stratStart, d->nzKey};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete d;
