@@ -436,6 +436,233 @@ __device__ void updClrC(const BatchArgs<R>& a, int j, int fmt, R* xb, R* oeb, do
   }
 }
 
+// One full coordinate cycle of a fit (independent rows or grouped): columns
+// that fit in one pass of the workgroup (blockDim * kIT nonzeros, the vast
+// majority of a Zipf-shaped design) keep row, key, x, xBeta, exp(xBeta) and the
+// stratum denominators in registers from the gather to the update, so the
+// update issues no loads; the next column's indices are loaded while the
+// current reduction runs; every lane takes the (deterministic) prior step, so
+// no broadcast barrier is needed.  Longer columns take the chunked passes.
+template <int IT, typename R>
+struct ColIdx {
+  int row[IT];
+  int key[IT + 2];
+  R x[IT];
+};
+
+template <int MODEL, int IT, typename R>
+__device__ __forceinline__ void loadColIdx(const DevX& d, int j, ColIdx<IT, R>& c) {
+  const int fmt = d.fmt[j];
+  const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
+  const int64_t base = b + (int64_t)threadIdx.x * IT;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int64_t p = base + it;
+    c.row[it] = p < e ? colRow(d, fmt, p, b) : -1;
+    c.x[it] = p < e ? colValue<R>(d, fmt, p) : (R)0;
+  }
+  if (MODEL == M_CLR) {
+#pragma unroll
+    for (int it = 0; it < IT + 2; ++it) {
+      const int64_t p = base - 1 + it;
+      c.key[it] = (p >= b && p < e) ? d.nzKey[p] : -1;
+    }
+  }
+}
+
+template <int MODEL, typename R>
+__device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R* oeb,
+                         const double* kw, double* denom, const double* nw, double* beta,
+                         double* bound, const double* xjy, const double* pvar,
+                         const int8_t* ptype, const uint8_t* fixed) {
+  constexpr int IT = kIT;
+  const DevX& d = a.d;
+  const int64_t CH = (int64_t)blockDim.x * IT;
+  const int J = a.J;
+  const int useW = c.useW;
+  long long cG = 0, cGR = 0, cU = 0, cUR = 0;
+  int j = 0;
+  while (j < J && fixed[j]) ++j;
+  ColIdx<IT, R> cur, nxt;
+  if (j < J && d.colPtr[j + 1] - d.colPtr[j] <= CH) loadColIdx<MODEL, IT, R>(d, j, cur);
+  while (j < J) {
+    int jn = j + 1;
+    while (jn < J && fixed[jn]) ++jn;
+    const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
+    const int64_t nj = e - b;
+    const int fmt = d.fmt[j];
+    const bool ind = isIndicator(fmt);
+    const double bj = beta[j], pv = pvar[j], xy = xjy[j];
+    double bnd = bound[j];
+    const int pt = ptype[j];
+    const long long runs = MODEL == M_CLR ? (long long)(d.segPtr[j + 1] - d.segPtr[j]) : 0;
+    const bool nxtSmall = jn < J && d.colPtr[jn + 1] - d.colPtr[jn] <= CH;
+    double delta;
+    if (nj == 0) {
+      delta = applyBounds(priorDelta(pt, pv, 0.0, 0.0, bj), &bnd);
+      if (threadIdx.x == 0) bound[j] = bnd;
+      if (threadIdx.x == 0 && delta != 0.0) beta[j] = bj + delta;
+      if (nxtSmall) loadColIdx<MODEL, IT, R>(d, jn, cur);
+    } else if (nj <= CH) {
+      R o[IT], xo[IT];
+      double den[IT], wn[IT], w[IT];
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        const int r = cur.row[it];
+        const bool valid = r >= 0;
+        o[it] = valid ? oeb[r] : (R)0;
+        xo[it] = valid ? xb[r] : (R)0;
+        w[it] = (MODEL != M_CLR && useW && valid) ? kw[r] : 1.0;
+        const bool last = MODEL == M_CLR && valid && cur.key[it + 2] != cur.key[it + 1];
+        den[it] = last ? denom[cur.key[it + 1]] : 1.0;
+        wn[it] = last ? nw[cur.key[it + 1]] : 0.0;
+      }
+      if (nxtSmall) loadColIdx<MODEL, IT, R>(d, jn, nxt);
+      double v[2] = {0.0, 0.0};
+      if (MODEL == M_CLR) {
+        Seg<2> items[IT];
+        Seg<2> agg;
+        agg.f = 0;
+        agg.v[0] = agg.v[1] = 0.0;
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+          const bool valid = cur.row[it] >= 0;
+          const R n1 = o[it] * cur.x[it];
+          Seg<2> q;
+          q.f = (valid && cur.key[it + 1] != cur.key[it]) ? 1 : 0;
+          q.v[0] = valid ? (double)n1 : 0.0;
+          q.v[1] = (valid && !ind) ? (double)(n1 * cur.x[it]) : 0.0;
+          items[it] = q;
+          agg = segCombine(agg, q);
+        }
+        Seg<2> total;
+        Seg<2> run = blockSegScanExcl<2>(agg, total);
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+          run = segCombine(run, items[it]);
+          if (cur.row[it] >= 0 && cur.key[it + 2] != cur.key[it + 1]) {
+            const double t = run.v[0] / den[it];
+            const double g = wn[it] * t;
+            v[0] += g;
+            v[1] += ind ? g * (1.0 - t) : wn[it] * (run.v[1] / den[it] - t * t);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+          if (cur.row[it] < 0) continue;
+          const R n1 = o[it] * cur.x[it];
+          if (MODEL == M_LOGISTIC) {
+            const R dd = (R)1 + o[it];
+            const R t = n1 / dd;
+            const R hh = ind ? t * ((R)1 - t) : ((n1 * cur.x[it]) / dd - t * t);
+            v[0] += useW ? w[it] * (double)t : (double)t;
+            v[1] += useW ? w[it] * (double)hh : (double)hh;
+          } else {
+            const R n2 = ind ? n1 : n1 * cur.x[it];
+            v[0] += useW ? w[it] * (double)n1 : (double)n1;
+            v[1] += useW ? w[it] * (double)n2 : (double)n2;
+          }
+        }
+      }
+      blockSum<2>(v);
+      double g = v[0] - xy, h = v[1];
+      if (h < 0.0) {
+        g = 0.0;
+        h = 0.0;
+      }
+      delta = applyBounds(priorDelta(pt, pv, g, h, bj), &bnd);
+      if (threadIdx.x == 0) {
+        bound[j] = bnd;
+        if (delta != 0.0) beta[j] = bj + delta;
+      }
+      if (delta != 0.0) {
+        const R rd = (R)delta;
+        if (MODEL == M_CLR) {
+          Seg<1> items[IT];
+          Seg<1> agg;
+          agg.f = 0;
+          agg.v[0] = 0.0;
+#pragma unroll
+          for (int it = 0; it < IT; ++it) {
+            const int r = cur.row[it];
+            Seg<1> q;
+            q.f = (r >= 0 && cur.key[it + 1] != cur.key[it]) ? 1 : 0;
+            q.v[0] = 0.0;
+            if (r >= 0) {
+              const R nx = xo[it] + rd * cur.x[it];
+              const R no = rexp(nx);
+              xb[r] = nx;
+              oeb[r] = no;
+              q.v[0] = (double)no - (double)o[it];
+            }
+            items[it] = q;
+            agg = segCombine(agg, q);
+          }
+          Seg<1> total;
+          Seg<1> run = blockSegScanExcl<1>(agg, total);
+#pragma unroll
+          for (int it = 0; it < IT; ++it) {
+            run = segCombine(run, items[it]);
+            if (cur.row[it] >= 0 && cur.key[it + 2] != cur.key[it + 1])
+              denom[cur.key[it + 1]] = den[it] + run.v[0];
+          }
+        } else {
+#pragma unroll
+          for (int it = 0; it < IT; ++it) {
+            const int r = cur.row[it];
+            if (r < 0) continue;
+            const R nx = xo[it] + rd * cur.x[it];
+            xb[r] = nx;
+            oeb[r] = rexp(nx);
+          }
+        }
+        __syncthreads();
+      }
+      if (nxtSmall) cur = nxt;
+    } else {
+      double v[2] = {0.0, 0.0};
+      if (MODEL == M_CLR)
+        ghClrC<R>(a, j, fmt, oeb, denom, nw, v);
+      else
+        ghIndepC<MODEL, R>(a, j, fmt, oeb, kw, useW, v);
+      blockSum<2>(v);
+      double g = v[0] - xy, h = v[1];
+      if (h < 0.0) {
+        g = 0.0;
+        h = 0.0;
+      }
+      delta = applyBounds(priorDelta(pt, pv, g, h, bj), &bnd);
+      if (threadIdx.x == 0) {
+        bound[j] = bnd;
+        if (delta != 0.0) beta[j] = bj + delta;
+      }
+      if (delta != 0.0) {
+        if (MODEL == M_CLR)
+          updClrC<R>(a, j, fmt, xb, oeb, denom, delta);
+        else
+          updIndepC<R>(a, j, fmt, xb, oeb, delta);
+        __syncthreads();
+      }
+      if (nxtSmall) loadColIdx<MODEL, IT, R>(d, jn, cur);
+    }
+    cG += nj;
+    cGR += runs;
+    if (delta != 0.0) {
+      cU += nj;
+      cUR += runs;
+    }
+    j = jn;
+  }
+  if (threadIdx.x == 0) {
+    FitStatus& cs = a.st[f];
+    cs.gatherNnz += cG;
+    cs.gatherRuns += cGR;
+    cs.updateNnz += cU;
+    cs.updateRuns += cUR;
+  }
+}
+
 // ------------------------------------------------------------ update pieces
 template <typename R>
 __device__ void updIndep(const BatchArgs<R>& a, int j, int fmt, R* xb, R* oeb, double delta) {
@@ -726,6 +953,9 @@ __global__ void k_batch_sweep(BatchArgs<R> a, const double* fixedLL, int sweeps)
   const uint8_t* reset = MODEL == M_COX ? a.reset + f * a.strideN : nullptr;
   __shared__ double sDelta;
   for (int sw = 0; sw < sweeps; ++sw) {
+    if (MODEL != M_COX) {
+      cycleFit<MODEL, R>(a, f, c, xb, oeb, kw, denom, nw, beta, bound, xjy, pvar, ptype, fixed);
+    } else
     for (int j = 0; j < a.J; ++j) {
       if (fixed[j]) continue;
       const int64_t nj = d.colPtr[j + 1] - d.colPtr[j];
