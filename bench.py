@@ -118,6 +118,11 @@ def main():
     achieved = alg / max(1, stats["launches"]) / (per_launch_ms * 1e-3) / 1e9
     log(f"sweep kernel: {stats['launches']} launches, {per_launch_ms:.2f} ms avg, "
         f"{alg / 1e9:.2f} GB algorithmic -> {achieved:.1f} GB/s")
+    tick = 1e-8  # device wall clock: 100 MHz
+    log(f"per-fit phase time (summed over fits): single-pass columns "
+        f"{stats['ticks_single'] * tick:.1f}s, long columns {stats['ticks_long'] * tick:.1f}s "
+        f"({stats['long_columns']} visits), objective {stats['ticks_objective'] * tick:.1f}s; "
+        f"sweeps {stats['sweeps']}, gathered nnz {stats['gather_nnz']}, updated nnz {stats['update_nnz']}")
 
     # plain engine and CPU oracle on a spread of the same fits
     sample = np.linspace(0, n_fits - 1, max(args.plain_fits, args.cpu_fits, 1)).astype(int)

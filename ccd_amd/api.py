@@ -69,7 +69,8 @@ class FitResult(ctypes.Structure):
 class BatchStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "launches", "sweeps", "sweep_rounds", "gather_nnz", "gather_runs", "update_nnz",
-        "update_runs", "scan_pids", "objective_rows")] + [
+        "update_runs", "scan_pids", "objective_rows", "ticks_single", "ticks_long",
+        "ticks_objective", "long_columns")] + [
         ("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double)]
 
     def as_dict(self):

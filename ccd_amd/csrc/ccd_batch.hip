@@ -62,6 +62,8 @@ struct FitStatus {
   int N;  // groups of this fit (Cox: tie groups under its weights)
   // work counters (thread 0), read by the host for the bandwidth figure
   long long gatherNnz, gatherRuns, updateNnz, updateRuns, scanPids, objRows;
+  // wall-clock ticks (thread 0) spent per phase: single-pass columns, long columns, objective
+  long long tSmall, tLarge, tObj, nLarge;
 };
 
 template <typename R>
@@ -237,9 +239,10 @@ __device__ void ghCox(const BatchArgs<R>& a, int j, int fmt, const R* oeb, const
 // computes, so a chunk costs two memory latencies whatever IT is.  Per-stratum
 // sums of the grouped model come from a block-level segmented scan whose carry
 // crosses chunk boundaries; the lane at a run's last nonzero owns the stratum.
-constexpr int kIT = 4;
+constexpr int kIT = 4;    // nonzeros per lane, single-pass columns
+constexpr int kITL = 16;  // nonzeros per lane per pass, long columns
 
-template <int MODEL, typename R>
+template <int MODEL, typename R, int kIT = kIT>
 __device__ void ghIndepC(const BatchArgs<R>& a, int j, int fmt, const R* oeb, const double* kw,
                          int useW, double (&v)[2]) {
   const DevX& d = a.d;
@@ -281,7 +284,7 @@ __device__ void ghIndepC(const BatchArgs<R>& a, int j, int fmt, const R* oeb, co
   }
 }
 
-template <typename R>
+template <typename R, int kIT = kIT>
 __device__ void ghClrC(const BatchArgs<R>& a, int j, int fmt, const R* oeb, const double* denom,
                        const double* nw, double (&v)[2]) {
   const DevX& d = a.d;
@@ -345,7 +348,7 @@ __device__ void ghClrC(const BatchArgs<R>& a, int j, int fmt, const R* oeb, cons
   }
 }
 
-template <typename R>
+template <typename R, int kIT = kIT>
 __device__ void updIndepC(const BatchArgs<R>& a, int j, int fmt, R* xb, R* oeb, double delta) {
   const DevX& d = a.d;
   const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
@@ -373,7 +376,7 @@ __device__ void updIndepC(const BatchArgs<R>& a, int j, int fmt, R* xb, R* oeb, 
   }
 }
 
-template <typename R>
+template <typename R, int kIT = kIT>
 __device__ void updClrC(const BatchArgs<R>& a, int j, int fmt, R* xb, R* oeb, double* denom,
                         double delta) {
   const DevX& d = a.d;
@@ -480,7 +483,7 @@ __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R
   const int64_t CH = (int64_t)blockDim.x * IT;
   const int J = a.J;
   const int useW = c.useW;
-  long long cG = 0, cGR = 0, cU = 0, cUR = 0;
+  long long cG = 0, cGR = 0, cU = 0, cUR = 0, tS = 0, tL = 0, nL = 0;
   int j = 0;
   while (j < J && fixed[j]) ++j;
   ColIdx<IT, R> cur, nxt;
@@ -498,6 +501,7 @@ __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R
     const long long runs = MODEL == M_CLR ? (long long)(d.segPtr[j + 1] - d.segPtr[j]) : 0;
     const bool nxtSmall = jn < J && d.colPtr[jn + 1] - d.colPtr[jn] <= CH;
     double delta;
+    const long long t0 = wall_clock64();
     if (nj == 0) {
       delta = applyBounds(priorDelta(pt, pv, 0.0, 0.0, bj), &bnd);
       if (threadIdx.x == 0) bound[j] = bnd;
@@ -623,9 +627,9 @@ __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R
     } else {
       double v[2] = {0.0, 0.0};
       if (MODEL == M_CLR)
-        ghClrC<R>(a, j, fmt, oeb, denom, nw, v);
+        ghClrC<R, kITL>(a, j, fmt, oeb, denom, nw, v);
       else
-        ghIndepC<MODEL, R>(a, j, fmt, oeb, kw, useW, v);
+        ghIndepC<MODEL, R, kITL>(a, j, fmt, oeb, kw, useW, v);
       blockSum<2>(v);
       double g = v[0] - xy, h = v[1];
       if (h < 0.0) {
@@ -639,12 +643,19 @@ __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R
       }
       if (delta != 0.0) {
         if (MODEL == M_CLR)
-          updClrC<R>(a, j, fmt, xb, oeb, denom, delta);
+          updClrC<R, kITL>(a, j, fmt, xb, oeb, denom, delta);
         else
-          updIndepC<R>(a, j, fmt, xb, oeb, delta);
+          updIndepC<R, kITL>(a, j, fmt, xb, oeb, delta);
         __syncthreads();
       }
       if (nxtSmall) loadColIdx<MODEL, IT, R>(d, jn, cur);
+    }
+    const long long t1 = wall_clock64();
+    if (nj > CH) {
+      tL += t1 - t0;
+      nL += 1;
+    } else {
+      tS += t1 - t0;
     }
     cG += nj;
     cGR += runs;
@@ -656,6 +667,9 @@ __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R
   }
   if (threadIdx.x == 0) {
     FitStatus& cs = a.st[f];
+    cs.tSmall += tS;
+    cs.tLarge += tL;
+    cs.nLarge += nL;
     cs.gatherNnz += cG;
     cs.gatherRuns += cGR;
     cs.updateNnz += cU;
@@ -1004,7 +1018,9 @@ __global__ void k_batch_sweep(BatchArgs<R> a, const double* fixedLL, int sweeps)
       __syncthreads();
     }
     // performCheckConvergence
+    const long long tq0 = wall_clock64();
     const double obj = fitObjective<MODEL, R>(a, f, fixedLL[f]);
+    if (threadIdx.x == 0) a.st[f].tObj += wall_clock64() - tq0;
     __shared__ int sDone;
     if (threadIdx.x == 0) {
       FitStatus& s = a.st[f];
@@ -1340,6 +1356,10 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
     g_stats.update_runs += fin[f].updateRuns;
     g_stats.scan_pids += fin[f].scanPids;
     g_stats.objective_rows += fin[f].objRows;
+    g_stats.ticks_single += fin[f].tSmall;
+    g_stats.ticks_long += fin[f].tLarge;
+    g_stats.ticks_objective += fin[f].tObj;
+    g_stats.long_columns += fin[f].nLarge;
   }
   g_stats.wall_ms = total * 1e3;
   for (int f = 0; f < F; ++f) {

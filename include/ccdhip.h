@@ -198,6 +198,10 @@ typedef struct {
   int64_t update_runs;
   int64_t scan_pids;       /* pids swept by the Cox risk-set scans */
   int64_t objective_rows;  /* rows read by the per-sweep convergence objective */
+  int64_t ticks_single;    /* device wall-clock ticks (100 MHz), summed over fits: columns */
+  int64_t ticks_long;      /*   done in one pass / in several passes / convergence objective */
+  int64_t ticks_objective;
+  int64_t long_columns;    /* multi-pass column visits */
   double kernel_ms;        /* summed HIP-event time of the sweep launches */
   double wall_ms;          /* host wall time of the whole batch */
 } ccdh_batch_stats;
