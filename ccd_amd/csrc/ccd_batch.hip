@@ -943,8 +943,9 @@ __global__ void k_batch_init(BatchArgs<R> a, const double* fixedLL) {
 }
 
 // Up to `sweeps` complete CCD cycles plus the convergence check after each.
-template <int MODEL, typename R>
-__global__ void k_batch_sweep(BatchArgs<R> a, const double* fixedLL, int sweeps) {
+template <int MODEL, typename R, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_batch_sweep(BatchArgs<R> a, const double* fixedLL,
+                                                        int sweeps) {
   const int f = a.active[blockIdx.x];
   if (a.st[f].done) return;
   const DevX& d = a.d;
@@ -1258,7 +1259,12 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
 
   // sweep launches, optionally replayed from a captured graph
   auto enqueueSweeps = [&]() -> int {
-    k_batch_sweep<MODEL, R><<<F, threads, 0, s>>>(a, dfixedLL, sweepsPerSync);
+    if (threads <= 256)
+      k_batch_sweep<MODEL, R, 256><<<F, 256, 0, s>>>(a, dfixedLL, sweepsPerSync);
+    else if (threads <= 512)
+      k_batch_sweep<MODEL, R, 512><<<F, 512, 0, s>>>(a, dfixedLL, sweepsPerSync);
+    else
+      k_batch_sweep<MODEL, R, 1024><<<F, 1024, 0, s>>>(a, dfixedLL, sweepsPerSync);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(o.hst, dst, F * sizeof(FitStatus), hipMemcpyDeviceToHost, s));
     return 0;
