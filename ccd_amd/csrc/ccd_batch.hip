@@ -94,59 +94,7 @@ struct BatchArgs {
 };
 
 // ------------------------------------------------------------ gather pieces
-template <int MODEL, typename R>
-__device__ void ghIndep(const BatchArgs<R>& a, int j, int fmt, const R* oeb, const double* kw,
-                        int useW, double (&v)[2]) {
-  const DevX& d = a.d;
-  const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
-  const bool ind = isIndicator(fmt);
-  for (int64_t p = b + threadIdx.x; p < e; p += blockDim.x) {
-    const int k = colRow(d, fmt, p, b);
-    const R x = colValue<R>(d, fmt, p);
-    const R o = oeb[k];
-    const R n1 = o * x;
-    const double w = useW ? kw[k] : 1.0;
-    if (MODEL == M_LOGISTIC) {
-      const R den = (R)1 + o;
-      const R t = n1 / den;
-      const R hh = ind ? t * ((R)1 - t) : ((n1 * x) / den - t * t);
-      v[0] += useW ? w * (double)t : (double)t;
-      v[1] += useW ? w * (double)hh : (double)hh;
-    } else {
-      const R n2 = ind ? n1 : n1 * x;
-      v[0] += useW ? w * (double)n1 : (double)n1;
-      v[1] += useW ? w * (double)n2 : (double)n2;
-    }
-  }
-}
 
-template <typename R>
-__device__ void ghClr(const BatchArgs<R>& a, int j, int fmt, const R* oeb, const double* denom,
-                      const double* nw, double (&v)[2]) {
-  const DevX& d = a.d;
-  const int64_t b = d.colPtr[j];
-  const int nj = (int)(d.colPtr[j + 1] - b);
-  const int64_t s0 = d.segPtr[j], s1 = d.segPtr[j + 1];
-  const bool ind = isIndicator(fmt);
-  for (int64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
-    const int q0 = d.segOff[s];
-    const int q1 = (s + 1 < s1) ? d.segOff[s + 1] : nj;
-    double n1 = 0.0, n2 = 0.0;
-    for (int q = q0; q < q1; ++q) {
-      const int64_t p = b + q;
-      const int k = colRow(d, fmt, p, b);
-      const R x = colValue<R>(d, fmt, p);
-      const R o = oeb[k];
-      n1 += (double)(o * x);
-      if (!ind) n2 += (double)(o * x * x);
-    }
-    const int key = d.segKey[s];
-    const double t = n1 / denom[key];
-    const double g = nw[key] * t;
-    v[0] += g;
-    v[1] += ind ? g * (1.0 - t) : nw[key] * (n2 / denom[key] - t * t);
-  }
-}
 
 template <typename R>
 __device__ __forceinline__ bool coxHead(const DevX& d, int fmt, int64_t b, int64_t p,
@@ -232,345 +180,182 @@ __device__ void ghCox(const BatchArgs<R>& a, int j, int fmt, const R* oeb, const
   }
 }
 
-// ------------------------------------------------ chunked gathers / updates
-// The column is processed in chunks of blockDim*IT nonzeros.  Every lane first
-// issues all of its index loads (row, stratum key: contiguous), then all of its
-// state loads (exp(xBeta), denominators, event weights), and only then
-// computes, so a chunk costs two memory latencies whatever IT is.  Per-stratum
-// sums of the grouped model come from a block-level segmented scan whose carry
-// crosses chunk boundaries; the lane at a run's last nonzero owns the stratum.
-constexpr int kIT = 4;    // nonzeros per lane, single-pass columns
-constexpr int kITL = 16;  // nonzeros per lane per pass, long columns
-
-template <int MODEL, typename R, int kIT = kIT>
-__device__ void ghIndepC(const BatchArgs<R>& a, int j, int fmt, const R* oeb, const double* kw,
-                         int useW, double (&v)[2]) {
-  const DevX& d = a.d;
-  const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
-  const bool ind = isIndicator(fmt);
-  const int64_t chunk = (int64_t)blockDim.x * kIT;
-  for (int64_t c0 = b; c0 < e; c0 += chunk) {
-    const int64_t base = c0 + (int64_t)threadIdx.x * kIT;
-    int row[kIT];
-    R x[kIT], o[kIT];
-    double w[kIT];
-#pragma unroll
-    for (int it = 0; it < kIT; ++it) {
-      const int64_t p = base + it;
-      row[it] = p < e ? colRow(d, fmt, p, b) : -1;
-      x[it] = p < e ? colValue<R>(d, fmt, p) : (R)0;
-    }
-#pragma unroll
-    for (int it = 0; it < kIT; ++it) {
-      o[it] = row[it] >= 0 ? oeb[row[it]] : (R)0;
-      w[it] = (useW && row[it] >= 0) ? kw[row[it]] : 1.0;
-    }
-#pragma unroll
-    for (int it = 0; it < kIT; ++it) {
-      if (row[it] < 0) continue;
-      const R n1 = o[it] * x[it];
-      if (MODEL == M_LOGISTIC) {
-        const R den = (R)1 + o[it];
-        const R t = n1 / den;
-        const R hh = ind ? t * ((R)1 - t) : ((n1 * x[it]) / den - t * t);
-        v[0] += useW ? w[it] * (double)t : (double)t;
-        v[1] += useW ? w[it] * (double)hh : (double)hh;
-      } else {
-        const R n2 = ind ? n1 : n1 * x[it];
-        v[0] += useW ? w[it] * (double)n1 : (double)n1;
-        v[1] += useW ? w[it] * (double)n2 : (double)n2;
-      }
-    }
-  }
-}
-
-template <typename R, int kIT = kIT>
-__device__ void ghClrC(const BatchArgs<R>& a, int j, int fmt, const R* oeb, const double* denom,
-                       const double* nw, double (&v)[2]) {
-  const DevX& d = a.d;
-  const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
-  const bool ind = isIndicator(fmt);
-  const int64_t chunk = (int64_t)blockDim.x * kIT;
-  Seg<2> carry;
-  carry.f = 0;
-  carry.v[0] = carry.v[1] = 0.0;
-  for (int64_t c0 = b; c0 < e; c0 += chunk) {
-    const int64_t base = c0 + (int64_t)threadIdx.x * kIT;
-    int row[kIT], key[kIT + 2];
-    R x[kIT], o[kIT];
-    double den[kIT], wn[kIT];
-    // keys of base-1 .. base+kIT (run starts and ends)
-#pragma unroll
-    for (int it = 0; it < kIT + 2; ++it) {
-      const int64_t p = base - 1 + it;
-      key[it] = (p >= b && p < e) ? d.nzKey[p] : -1;
-    }
-#pragma unroll
-    for (int it = 0; it < kIT; ++it) {
-      const int64_t p = base + it;
-      row[it] = p < e ? colRow(d, fmt, p, b) : -1;
-      x[it] = p < e ? colValue<R>(d, fmt, p) : (R)0;
-    }
-#pragma unroll
-    for (int it = 0; it < kIT; ++it) {
-      const bool last = row[it] >= 0 && key[it + 2] != key[it + 1];
-      o[it] = row[it] >= 0 ? oeb[row[it]] : (R)0;
-      den[it] = last ? denom[key[it + 1]] : 1.0;
-      wn[it] = last ? nw[key[it + 1]] : 0.0;
-    }
-    Seg<2> items[kIT];
-    Seg<2> agg;
-    agg.f = 0;
-    agg.v[0] = agg.v[1] = 0.0;
-#pragma unroll
-    for (int it = 0; it < kIT; ++it) {
-      Seg<2> q;
-      q.f = (row[it] >= 0 && key[it + 1] != key[it]) ? 1 : 0;
-      const R n1 = o[it] * x[it];
-      q.v[0] = row[it] >= 0 ? (double)n1 : 0.0;
-      q.v[1] = (row[it] >= 0 && !ind) ? (double)(n1 * x[it]) : 0.0;
-      items[it] = q;
-      agg = segCombine(agg, q);
-    }
-    Seg<2> total;
-    Seg<2> run = segCombine(carry, blockSegScanExcl<2>(agg, total));
-#pragma unroll
-    for (int it = 0; it < kIT; ++it) {
-      run = segCombine(run, items[it]);
-      if (row[it] >= 0 && key[it + 2] != key[it + 1]) {
-        const double t = run.v[0] / den[it];
-        const double g = wn[it] * t;
-        v[0] += g;
-        v[1] += ind ? g * (1.0 - t) : wn[it] * (run.v[1] / den[it] - t * t);
-      }
-    }
-    carry = segCombine(carry, total);
-  }
-}
-
-template <typename R, int kIT = kIT>
-__device__ void updIndepC(const BatchArgs<R>& a, int j, int fmt, R* xb, R* oeb, double delta) {
-  const DevX& d = a.d;
-  const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
-  const R rd = (R)delta;
-  const int64_t chunk = (int64_t)blockDim.x * kIT;
-  for (int64_t c0 = b; c0 < e; c0 += chunk) {
-    const int64_t base = c0 + (int64_t)threadIdx.x * kIT;
-    int row[kIT];
-    R x[kIT], xo[kIT];
-#pragma unroll
-    for (int it = 0; it < kIT; ++it) {
-      const int64_t p = base + it;
-      row[it] = p < e ? colRow(d, fmt, p, b) : -1;
-      x[it] = p < e ? colValue<R>(d, fmt, p) : (R)0;
-    }
-#pragma unroll
-    for (int it = 0; it < kIT; ++it) xo[it] = row[it] >= 0 ? xb[row[it]] : (R)0;
-#pragma unroll
-    for (int it = 0; it < kIT; ++it) {
-      if (row[it] < 0) continue;
-      const R nx = xo[it] + rd * x[it];
-      xb[row[it]] = nx;
-      oeb[row[it]] = rexp(nx);
-    }
-  }
-}
-
-template <typename R, int kIT = kIT>
-__device__ void updClrC(const BatchArgs<R>& a, int j, int fmt, R* xb, R* oeb, double* denom,
-                        double delta) {
-  const DevX& d = a.d;
-  const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
-  const R rd = (R)delta;
-  const int64_t chunk = (int64_t)blockDim.x * kIT;
-  Seg<1> carry;
-  carry.f = 0;
-  carry.v[0] = 0.0;
-  for (int64_t c0 = b; c0 < e; c0 += chunk) {
-    const int64_t base = c0 + (int64_t)threadIdx.x * kIT;
-    int row[kIT], key[kIT + 2];
-    R x[kIT], xo[kIT], oo[kIT];
-    double den[kIT];
-#pragma unroll
-    for (int it = 0; it < kIT + 2; ++it) {
-      const int64_t p = base - 1 + it;
-      key[it] = (p >= b && p < e) ? d.nzKey[p] : -1;
-    }
-#pragma unroll
-    for (int it = 0; it < kIT; ++it) {
-      const int64_t p = base + it;
-      row[it] = p < e ? colRow(d, fmt, p, b) : -1;
-      x[it] = p < e ? colValue<R>(d, fmt, p) : (R)0;
-    }
-#pragma unroll
-    for (int it = 0; it < kIT; ++it) {
-      const bool last = row[it] >= 0 && key[it + 2] != key[it + 1];
-      xo[it] = row[it] >= 0 ? xb[row[it]] : (R)0;
-      oo[it] = row[it] >= 0 ? oeb[row[it]] : (R)0;
-      den[it] = last ? denom[key[it + 1]] : 0.0;
-    }
-    Seg<1> items[kIT];
-    Seg<1> agg;
-    agg.f = 0;
-    agg.v[0] = 0.0;
-#pragma unroll
-    for (int it = 0; it < kIT; ++it) {
-      Seg<1> q;
-      q.f = (row[it] >= 0 && key[it + 1] != key[it]) ? 1 : 0;
-      q.v[0] = 0.0;
-      if (row[it] >= 0) {
-        const R nx = xo[it] + rd * x[it];
-        const R no = rexp(nx);
-        xb[row[it]] = nx;
-        oeb[row[it]] = no;
-        q.v[0] = (double)no - (double)oo[it];
-      }
-      items[it] = q;
-      agg = segCombine(agg, q);
-    }
-    Seg<1> total;
-    Seg<1> run = segCombine(carry, blockSegScanExcl<1>(agg, total));
-#pragma unroll
-    for (int it = 0; it < kIT; ++it) {
-      run = segCombine(run, items[it]);
-      if (row[it] >= 0 && key[it + 2] != key[it + 1]) denom[key[it + 1]] = den[it] + run.v[0];
-    }
-    carry = segCombine(carry, total);
-  }
-}
-
-// One full coordinate cycle of a fit (independent rows or grouped): columns
-// that fit in one pass of the workgroup (blockDim * kIT nonzeros, the vast
-// majority of a Zipf-shaped design) keep row, key, x, xBeta, exp(xBeta) and the
-// stratum denominators in registers from the gather to the update, so the
-// update issues no loads; the next column's indices are loaded while the
-// current reduction runs; every lane takes the (deterministic) prior step, so
-// no broadcast barrier is needed.  Longer columns take the chunked passes.
+// ------------------------------------------------ striped run-owner passes
+// A column is walked in passes of blockDim*IT nonzeros with the striped
+// assignment p = pass start + it*blockDim + lane, so every index load of a wave
+// reads 64 consecutive entries.  Grouped models: the lane holding the last
+// nonzero of a run of equal stratum owns the run (runs are contiguous because
+// rows are sorted by stratum, and hold at most one stratum's rows); it sums the
+// run's numerators, applies the stratum's increment, and in the update moves
+// every row of the run and adds the denominator changes in row order, as the
+// reference's sequential loops do.  Each row belongs to one run, so no two
+// lanes touch the same row or stratum and the passes need no barrier.
 template <int IT, typename R>
-struct ColIdx {
+struct Lanes {
   int row[IT];
-  int key[IT + 2];
-  R x[IT];
+  int key[IT];
+  bool own[IT];    // valid nonzero that ends its run (always, for independent rows)
+  bool single[IT]; // run of one nonzero (always, for independent rows)
+  R x[IT], o[IT], xo[IT];
+  double den[IT], wn[IT], w[IT];
 };
 
-template <int MODEL, int IT, typename R>
-__device__ __forceinline__ void loadColIdx(const DevX& d, int j, ColIdx<IT, R>& c) {
-  const int fmt = d.fmt[j];
-  const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
-  const int64_t base = b + (int64_t)threadIdx.x * IT;
+// index + state loads for one pass; KEEP also loads xBeta for a register-resident update
+template <int MODEL, int IT, bool KEEP, typename R>
+__device__ __forceinline__ void passLoad(const DevX& d, int fmt, int64_t b, int64_t e, int64_t c0,
+                                         const R* oeb, const R* xb, const double* denom,
+                                         const double* nw, const double* kw, int useW,
+                                         Lanes<IT, R>& L) {
+  int kp[IT], kn[IT];
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
-    const int64_t p = base + it;
-    c.row[it] = p < e ? colRow(d, fmt, p, b) : -1;
-    c.x[it] = p < e ? colValue<R>(d, fmt, p) : (R)0;
+    const int64_t p = c0 + (int64_t)it * blockDim.x + threadIdx.x;
+    const bool valid = p < e;
+    L.row[it] = valid ? colRow(d, fmt, p, b) : -1;
+    L.x[it] = valid ? colValue<R>(d, fmt, p) : (R)0;
+    if (MODEL == M_CLR) {
+      L.key[it] = valid ? d.nzKey[p] : -1;
+      kp[it] = (valid && p > b) ? d.nzKey[p - 1] : -2;
+      kn[it] = (valid && p + 1 < e) ? d.nzKey[p + 1] : -3;
+    }
   }
-  if (MODEL == M_CLR) {
 #pragma unroll
-    for (int it = 0; it < IT + 2; ++it) {
-      const int64_t p = base - 1 + it;
-      c.key[it] = (p >= b && p < e) ? d.nzKey[p] : -1;
+  for (int it = 0; it < IT; ++it) {
+    const int r = L.row[it];
+    const bool valid = r >= 0;
+    L.own[it] = MODEL == M_CLR ? (valid && L.key[it] != kn[it]) : valid;
+    L.single[it] = MODEL == M_CLR ? (L.key[it] != kp[it]) : true;
+    L.o[it] = valid ? oeb[r] : (R)0;
+    L.xo[it] = (KEEP && valid) ? xb[r] : (R)0;
+    L.w[it] = (MODEL != M_CLR && useW && valid) ? kw[r] : 1.0;
+    L.den[it] = (MODEL == M_CLR && L.own[it]) ? denom[L.key[it]] : 1.0;
+    L.wn[it] = (MODEL == M_CLR && L.own[it]) ? nw[L.key[it]] : 0.0;
+  }
+}
+
+template <int MODEL, int IT, typename R>
+__device__ __forceinline__ void passGather(const DevX& d, int fmt, int64_t b, int64_t c0,
+                                           const R* oeb, const Lanes<IT, R>& L, int useW,
+                                           double (&v)[2]) {
+  const bool ind = isIndicator(fmt);
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    if (!L.own[it]) continue;
+    if (MODEL == M_CLR) {
+      double n1, n2;
+      if (L.single[it]) {
+        const R m = L.o[it] * L.x[it];
+        n1 = (double)m;
+        n2 = ind ? 0.0 : (double)(m * L.x[it]);
+      } else {
+        const int64_t p = c0 + (int64_t)it * blockDim.x + threadIdx.x;
+        int64_t q = p;
+        while (q > b && d.nzKey[q - 1] == L.key[it]) --q;
+        n1 = 0.0;
+        n2 = 0.0;
+        for (; q <= p; ++q) {
+          const R xq = q == p ? L.x[it] : colValue<R>(d, fmt, q);
+          const R oq = q == p ? L.o[it] : oeb[colRow(d, fmt, q, b)];
+          const R m = oq * xq;
+          n1 += (double)m;
+          if (!ind) n2 += (double)(m * xq);
+        }
+      }
+      const double t = n1 / L.den[it];
+      const double g = L.wn[it] * t;
+      v[0] += g;
+      v[1] += ind ? g * (1.0 - t) : L.wn[it] * (n2 / L.den[it] - t * t);
+    } else {
+      const R n1 = L.o[it] * L.x[it];
+      if (MODEL == M_LOGISTIC) {
+        const R dd = (R)1 + L.o[it];
+        const R t = n1 / dd;
+        const R hh = ind ? t * ((R)1 - t) : ((n1 * L.x[it]) / dd - t * t);
+        v[0] += useW ? L.w[it] * (double)t : (double)t;
+        v[1] += useW ? L.w[it] * (double)hh : (double)hh;
+      } else {
+        const R n2 = ind ? n1 : n1 * L.x[it];
+        v[0] += useW ? L.w[it] * (double)n1 : (double)n1;
+        v[1] += useW ? L.w[it] * (double)n2 : (double)n2;
+      }
     }
   }
 }
 
+// update of one pass; KEEP means xBeta was loaded by passLoad
+template <int MODEL, int IT, bool KEEP, typename R>
+__device__ __forceinline__ void passUpdate(const DevX& d, int fmt, int64_t b, int64_t c0, R* xb,
+                                           R* oeb, double* denom, const Lanes<IT, R>& L,
+                                           double delta) {
+  const R rd = (R)delta;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    if (!L.own[it]) continue;
+    if (MODEL == M_CLR && !L.single[it]) {
+      const int64_t p = c0 + (int64_t)it * blockDim.x + threadIdx.x;
+      int64_t q = p;
+      while (q > b && d.nzKey[q - 1] == L.key[it]) --q;
+      double den = L.den[it];
+      for (; q <= p; ++q) {
+        const int r = q == p ? L.row[it] : colRow(d, fmt, q, b);
+        const R xq = q == p ? L.x[it] : colValue<R>(d, fmt, q);
+        const R oo = q == p ? L.o[it] : oeb[r];
+        const R nx = ((KEEP && q == p) ? L.xo[it] : xb[r]) + rd * xq;
+        const R no = rexp(nx);
+        xb[r] = nx;
+        oeb[r] = no;
+        den += (double)no - (double)oo;
+      }
+      denom[L.key[it]] = den;
+    } else {
+      const int r = L.row[it];
+      const R nx = (KEEP ? L.xo[it] : xb[r]) + rd * L.x[it];
+      const R no = rexp(nx);
+      xb[r] = nx;
+      oeb[r] = no;
+      if (MODEL == M_CLR) denom[L.key[it]] = L.den[it] + ((double)no - (double)L.o[it]);
+    }
+  }
+}
+
+constexpr int kIT = 4;   // nonzeros per lane: columns up to blockDim*kIT stay in registers
+constexpr int kITL = 8;  // nonzeros per lane per pass over longer columns
+
+// One full coordinate cycle of a fit (independent rows or grouped).  Columns of
+// at most blockDim*kIT nonzeros (the vast majority of a Zipf-shaped design) keep
+// their indices, exp(xBeta), xBeta and stratum terms in registers from the
+// gather to the update.  Every lane takes the (deterministic) prior step from the
+// block-reduced gradient and Hessian, so no broadcast barrier is needed; a barrier
+// follows an update only.
 template <int MODEL, typename R>
 __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R* oeb,
                          const double* kw, double* denom, const double* nw, double* beta,
                          double* bound, const double* xjy, const double* pvar,
                          const int8_t* ptype, const uint8_t* fixed) {
-  constexpr int IT = kIT;
   const DevX& d = a.d;
-  const int64_t CH = (int64_t)blockDim.x * IT;
+  const int64_t CH = (int64_t)blockDim.x * kIT;
+  const int64_t CHL = (int64_t)blockDim.x * kITL;
   const int J = a.J;
   const int useW = c.useW;
   long long cG = 0, cGR = 0, cU = 0, cUR = 0, tS = 0, tL = 0, nL = 0;
-  int j = 0;
-  while (j < J && fixed[j]) ++j;
-  ColIdx<IT, R> cur, nxt;
-  if (j < J && d.colPtr[j + 1] - d.colPtr[j] <= CH) loadColIdx<MODEL, IT, R>(d, j, cur);
-  while (j < J) {
-    int jn = j + 1;
-    while (jn < J && fixed[jn]) ++jn;
+  for (int j = 0; j < J; ++j) {
+    if (fixed[j]) continue;
     const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
     const int64_t nj = e - b;
     const int fmt = d.fmt[j];
-    const bool ind = isIndicator(fmt);
     const double bj = beta[j], pv = pvar[j], xy = xjy[j];
     double bnd = bound[j];
     const int pt = ptype[j];
     const long long runs = MODEL == M_CLR ? (long long)(d.segPtr[j + 1] - d.segPtr[j]) : 0;
-    const bool nxtSmall = jn < J && d.colPtr[jn + 1] - d.colPtr[jn] <= CH;
-    double delta;
     const long long t0 = wall_clock64();
-    if (nj == 0) {
-      delta = applyBounds(priorDelta(pt, pv, 0.0, 0.0, bj), &bnd);
-      if (threadIdx.x == 0) bound[j] = bnd;
-      if (threadIdx.x == 0 && delta != 0.0) beta[j] = bj + delta;
-      if (nxtSmall) loadColIdx<MODEL, IT, R>(d, jn, cur);
-    } else if (nj <= CH) {
-      R o[IT], xo[IT];
-      double den[IT], wn[IT], w[IT];
-#pragma unroll
-      for (int it = 0; it < IT; ++it) {
-        const int r = cur.row[it];
-        const bool valid = r >= 0;
-        o[it] = valid ? oeb[r] : (R)0;
-        xo[it] = valid ? xb[r] : (R)0;
-        w[it] = (MODEL != M_CLR && useW && valid) ? kw[r] : 1.0;
-        const bool last = MODEL == M_CLR && valid && cur.key[it + 2] != cur.key[it + 1];
-        den[it] = last ? denom[cur.key[it + 1]] : 1.0;
-        wn[it] = last ? nw[cur.key[it + 1]] : 0.0;
-      }
-      if (nxtSmall) loadColIdx<MODEL, IT, R>(d, jn, nxt);
+    double delta;
+    if (nj <= CH) {
+      Lanes<kIT, R> L;
+      passLoad<MODEL, kIT, true, R>(d, fmt, b, e, b, oeb, xb, denom, nw, kw, useW, L);
       double v[2] = {0.0, 0.0};
-      if (MODEL == M_CLR) {
-        Seg<2> items[IT];
-        Seg<2> agg;
-        agg.f = 0;
-        agg.v[0] = agg.v[1] = 0.0;
-#pragma unroll
-        for (int it = 0; it < IT; ++it) {
-          const bool valid = cur.row[it] >= 0;
-          const R n1 = o[it] * cur.x[it];
-          Seg<2> q;
-          q.f = (valid && cur.key[it + 1] != cur.key[it]) ? 1 : 0;
-          q.v[0] = valid ? (double)n1 : 0.0;
-          q.v[1] = (valid && !ind) ? (double)(n1 * cur.x[it]) : 0.0;
-          items[it] = q;
-          agg = segCombine(agg, q);
-        }
-        Seg<2> total;
-        Seg<2> run = blockSegScanExcl<2>(agg, total);
-#pragma unroll
-        for (int it = 0; it < IT; ++it) {
-          run = segCombine(run, items[it]);
-          if (cur.row[it] >= 0 && cur.key[it + 2] != cur.key[it + 1]) {
-            const double t = run.v[0] / den[it];
-            const double g = wn[it] * t;
-            v[0] += g;
-            v[1] += ind ? g * (1.0 - t) : wn[it] * (run.v[1] / den[it] - t * t);
-          }
-        }
-      } else {
-#pragma unroll
-        for (int it = 0; it < IT; ++it) {
-          if (cur.row[it] < 0) continue;
-          const R n1 = o[it] * cur.x[it];
-          if (MODEL == M_LOGISTIC) {
-            const R dd = (R)1 + o[it];
-            const R t = n1 / dd;
-            const R hh = ind ? t * ((R)1 - t) : ((n1 * cur.x[it]) / dd - t * t);
-            v[0] += useW ? w[it] * (double)t : (double)t;
-            v[1] += useW ? w[it] * (double)hh : (double)hh;
-          } else {
-            const R n2 = ind ? n1 : n1 * cur.x[it];
-            v[0] += useW ? w[it] * (double)n1 : (double)n1;
-            v[1] += useW ? w[it] * (double)n2 : (double)n2;
-          }
-        }
-      }
+      passGather<MODEL, kIT, R>(d, fmt, b, b, oeb, L, useW, v);
       blockSum<2>(v);
-      double g = v[0] - xy, h = v[1];
+      double g = nj > 0 ? v[0] - xy : 0.0, h = nj > 0 ? v[1] : 0.0;
       if (h < 0.0) {
         g = 0.0;
         h = 0.0;
@@ -580,56 +365,17 @@ __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R
         bound[j] = bnd;
         if (delta != 0.0) beta[j] = bj + delta;
       }
-      if (delta != 0.0) {
-        const R rd = (R)delta;
-        if (MODEL == M_CLR) {
-          Seg<1> items[IT];
-          Seg<1> agg;
-          agg.f = 0;
-          agg.v[0] = 0.0;
-#pragma unroll
-          for (int it = 0; it < IT; ++it) {
-            const int r = cur.row[it];
-            Seg<1> q;
-            q.f = (r >= 0 && cur.key[it + 1] != cur.key[it]) ? 1 : 0;
-            q.v[0] = 0.0;
-            if (r >= 0) {
-              const R nx = xo[it] + rd * cur.x[it];
-              const R no = rexp(nx);
-              xb[r] = nx;
-              oeb[r] = no;
-              q.v[0] = (double)no - (double)o[it];
-            }
-            items[it] = q;
-            agg = segCombine(agg, q);
-          }
-          Seg<1> total;
-          Seg<1> run = blockSegScanExcl<1>(agg, total);
-#pragma unroll
-          for (int it = 0; it < IT; ++it) {
-            run = segCombine(run, items[it]);
-            if (cur.row[it] >= 0 && cur.key[it + 2] != cur.key[it + 1])
-              denom[cur.key[it + 1]] = den[it] + run.v[0];
-          }
-        } else {
-#pragma unroll
-          for (int it = 0; it < IT; ++it) {
-            const int r = cur.row[it];
-            if (r < 0) continue;
-            const R nx = xo[it] + rd * cur.x[it];
-            xb[r] = nx;
-            oeb[r] = rexp(nx);
-          }
-        }
+      if (delta != 0.0 && nj > 0) {
+        passUpdate<MODEL, kIT, true, R>(d, fmt, b, b, xb, oeb, denom, L, delta);
         __syncthreads();
       }
-      if (nxtSmall) cur = nxt;
     } else {
       double v[2] = {0.0, 0.0};
-      if (MODEL == M_CLR)
-        ghClrC<R, kITL>(a, j, fmt, oeb, denom, nw, v);
-      else
-        ghIndepC<MODEL, R, kITL>(a, j, fmt, oeb, kw, useW, v);
+      for (int64_t c0 = b; c0 < e; c0 += CHL) {
+        Lanes<kITL, R> L;
+        passLoad<MODEL, kITL, false, R>(d, fmt, b, e, c0, oeb, xb, denom, nw, kw, useW, L);
+        passGather<MODEL, kITL, R>(d, fmt, b, c0, oeb, L, useW, v);
+      }
       blockSum<2>(v);
       double g = v[0] - xy, h = v[1];
       if (h < 0.0) {
@@ -642,13 +388,13 @@ __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R
         if (delta != 0.0) beta[j] = bj + delta;
       }
       if (delta != 0.0) {
-        if (MODEL == M_CLR)
-          updClrC<R, kITL>(a, j, fmt, xb, oeb, denom, delta);
-        else
-          updIndepC<R, kITL>(a, j, fmt, xb, oeb, delta);
+        for (int64_t c0 = b; c0 < e; c0 += CHL) {
+          Lanes<kITL, R> L;
+          passLoad<MODEL, kITL, false, R>(d, fmt, b, e, c0, oeb, xb, denom, nw, kw, useW, L);
+          passUpdate<MODEL, kITL, false, R>(d, fmt, b, c0, xb, oeb, denom, L, delta);
+        }
         __syncthreads();
       }
-      if (nxtSmall) loadColIdx<MODEL, IT, R>(d, jn, cur);
     }
     const long long t1 = wall_clock64();
     if (nj > CH) {
@@ -663,7 +409,6 @@ __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R
       cU += nj;
       cUR += runs;
     }
-    j = jn;
   }
   if (threadIdx.x == 0) {
     FitStatus& cs = a.st[f];
@@ -678,44 +423,7 @@ __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R
 }
 
 // ------------------------------------------------------------ update pieces
-template <typename R>
-__device__ void updIndep(const BatchArgs<R>& a, int j, int fmt, R* xb, R* oeb, double delta) {
-  const DevX& d = a.d;
-  const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
-  const R rd = (R)delta;
-  for (int64_t p = b + threadIdx.x; p < e; p += blockDim.x) {
-    const int k = colRow(d, fmt, p, b);
-    const R nx = xb[k] + rd * colValue<R>(d, fmt, p);
-    xb[k] = nx;
-    oeb[k] = rexp(nx);
-  }
-}
 
-template <typename R>
-__device__ void updClr(const BatchArgs<R>& a, int j, int fmt, R* xb, R* oeb, double* denom,
-                       double delta) {
-  const DevX& d = a.d;
-  const int64_t b = d.colPtr[j];
-  const int nj = (int)(d.colPtr[j + 1] - b);
-  const int64_t s0 = d.segPtr[j], s1 = d.segPtr[j + 1];
-  const R rd = (R)delta;
-  for (int64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
-    const int q0 = d.segOff[s];
-    const int q1 = (s + 1 < s1) ? d.segOff[s + 1] : nj;
-    const int key = d.segKey[s];
-    double den = denom[key];
-    for (int q = q0; q < q1; ++q) {
-      const int64_t p = b + q;
-      const int k = colRow(d, fmt, p, b);
-      const R nx = xb[k] + rd * colValue<R>(d, fmt, p);
-      xb[k] = nx;
-      const R o = rexp(nx);
-      den += (double)o - (double)oeb[k];
-      oeb[k] = o;
-    }
-    denom[key] = den;
-  }
-}
 
 template <typename R>
 __device__ void updCox(const BatchArgs<R>& a, int j, int fmt, R* xb, R* oeb, double* denom,
@@ -977,12 +685,7 @@ __global__ void __launch_bounds__(BLOCK) k_batch_sweep(BatchArgs<R> a, const dou
       const int fmt = d.fmt[j];
       double v[2] = {0.0, 0.0};
       if (nj > 0) {
-        if (MODEL == M_LOGISTIC || MODEL == M_POISSON)
-          ghIndepC<MODEL, R>(a, j, fmt, oeb, kw, c.useW, v);
-        else if (MODEL == M_CLR)
-          ghClrC<R>(a, j, fmt, oeb, denom, nw, v);
-        else
-          ghCox<R>(a, j, fmt, oeb, kw, c.useW, pid, N, denom, numer, numer2, reset, nw, v);
+        ghCox<R>(a, j, fmt, oeb, kw, c.useW, pid, N, denom, numer, numer2, reset, nw, v);
         blockSum<2>(v);
       }
       if (threadIdx.x == 0) {
@@ -1009,12 +712,7 @@ __global__ void __launch_bounds__(BLOCK) k_batch_sweep(BatchArgs<R> a, const dou
       __syncthreads();
       const double delta = sDelta;
       if (delta != 0.0) {
-        if (MODEL == M_LOGISTIC || MODEL == M_POISSON)
-          updIndepC<R>(a, j, fmt, xb, oeb, delta);
-        else if (MODEL == M_CLR)
-          updClrC<R>(a, j, fmt, xb, oeb, denom, delta);
-        else
-          updCox<R>(a, j, fmt, xb, oeb, denom, kw, c.useW, pid, N, delta);
+        updCox<R>(a, j, fmt, xb, oeb, denom, kw, c.useW, pid, N, delta);
       }
       __syncthreads();
     }
@@ -1261,10 +959,8 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
   auto enqueueSweeps = [&]() -> int {
     if (threads <= 256)
       k_batch_sweep<MODEL, R, 256><<<F, 256, 0, s>>>(a, dfixedLL, sweepsPerSync);
-    else if (threads <= 512)
-      k_batch_sweep<MODEL, R, 512><<<F, 512, 0, s>>>(a, dfixedLL, sweepsPerSync);
     else
-      k_batch_sweep<MODEL, R, 1024><<<F, 1024, 0, s>>>(a, dfixedLL, sweepsPerSync);
+      k_batch_sweep<MODEL, R, 512><<<F, 512, 0, s>>>(a, dfixedLL, sweepsPerSync);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(o.hst, dst, F * sizeof(FitStatus), hipMemcpyDeviceToHost, s));
     return 0;
