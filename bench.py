@@ -63,6 +63,9 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--folds", type=int, default=10)
     ap.add_argument("--grid", type=int, default=16)
+    ap.add_argument("--limit", type=int, default=0, help="run only the last N fits (probe)")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run a device copy of this size (torch) for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -91,7 +94,18 @@ def main():
         for f in range(folds):
             w = api.cv_fold_weights(prob.strata, folds, 123, f, by_pid=True)
             specs.append(default_spec(prob, prior=PRIOR_LAPLACE, variance=var, weights=w))
+    if args.limit:
+        specs = specs[-args.limit:]
     n_fits = len(specs)
+    if args.copy_calibration_mb:
+        import torch
+        n = args.copy_calibration_mb * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {n * 8} bytes read + {n * 8} bytes written")
+        del src, dst
     opt = dict(threads=args.threads, sweeps_per_sync=args.sweeps_per_sync, use_graph=1)
 
     # load the code objects and touch the allocator (untimed)
@@ -123,6 +137,8 @@ def main():
         f"{stats['ticks_single'] * tick:.1f}s, long columns {stats['ticks_long'] * tick:.1f}s "
         f"({stats['long_columns']} visits), objective {stats['ticks_objective'] * tick:.1f}s; "
         f"sweeps {stats['sweeps']}, gathered nnz {stats['gather_nnz']}, updated nnz {stats['update_nnz']}")
+    mx = int(iters.max())
+    log(f"slowest fit: {mx} sweeps in {lat.max():.2f}s -> {lat.max() / mx * 1e3:.1f} ms/sweep")
 
     # plain engine and CPU oracle on a spread of the same fits
     sample = np.linspace(0, n_fits - 1, max(args.plain_fits, args.cpu_fits, 1)).astype(int)
