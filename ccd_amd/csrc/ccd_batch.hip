@@ -318,6 +318,87 @@ __device__ __forceinline__ void passUpdate(const DevX& d, int fmt, int64_t b, in
   }
 }
 
+// Run mode (grouped columns whose stratum runs average >= 1.5 rows, i.e. the
+// near-dense columns): lanes take the precomputed runs striped, read the run's
+// bounds and stratum (one coalesced hop), then all of the run's rows and their
+// exp(xBeta) together, so a run costs the same latency whatever its length.
+constexpr int kRL = 8;  // rows of a run loaded together
+
+template <typename R>
+__device__ __forceinline__ void runGather(const DevX& d, int fmt, int64_t b, int nj, int64_t s1,
+                                          int64_t sp, const R* oeb, const double* denom,
+                                          const double* nw, double (&v)[2]) {
+  const bool ind = isIndicator(fmt);
+  const int64_t s = sp + threadIdx.x;
+  if (s >= s1) return;
+  const int q0 = d.segOff[s];
+  const int q1 = (s + 1 < s1) ? d.segOff[s + 1] : nj;
+  const int key = d.segKey[s];
+  const double den = denom[key], wn = nw[key];
+  double n1 = 0.0, n2 = 0.0;
+  for (int q = q0; q < q1; q += kRL) {
+    int rw[kRL];
+    R xx[kRL], oo[kRL];
+#pragma unroll
+    for (int u = 0; u < kRL; ++u) {
+      const bool in = q + u < q1;
+      rw[u] = in ? colRow(d, fmt, b + q + u, b) : -1;
+      xx[u] = in ? colValue<R>(d, fmt, b + q + u) : (R)0;
+    }
+#pragma unroll
+    for (int u = 0; u < kRL; ++u) oo[u] = rw[u] >= 0 ? oeb[rw[u]] : (R)0;
+#pragma unroll
+    for (int u = 0; u < kRL; ++u) {
+      if (rw[u] < 0) continue;
+      const R m = oo[u] * xx[u];
+      n1 += (double)m;
+      if (!ind) n2 += (double)(m * xx[u]);
+    }
+  }
+  const double t = n1 / den;
+  const double g = wn * t;
+  v[0] += g;
+  v[1] += ind ? g * (1.0 - t) : wn * (n2 / den - t * t);
+}
+
+template <typename R>
+__device__ __forceinline__ void runUpdate(const DevX& d, int fmt, int64_t b, int nj, int64_t s1,
+                                          int64_t sp, R* xb, R* oeb, double* denom,
+                                          double delta) {
+  const int64_t s = sp + threadIdx.x;
+  if (s >= s1) return;
+  const R rd = (R)delta;
+  const int q0 = d.segOff[s];
+  const int q1 = (s + 1 < s1) ? d.segOff[s + 1] : nj;
+  const int key = d.segKey[s];
+  double den = denom[key];
+  for (int q = q0; q < q1; q += kRL) {
+    int rw[kRL];
+    R xx[kRL], xo[kRL], oo[kRL];
+#pragma unroll
+    for (int u = 0; u < kRL; ++u) {
+      const bool in = q + u < q1;
+      rw[u] = in ? colRow(d, fmt, b + q + u, b) : -1;
+      xx[u] = in ? colValue<R>(d, fmt, b + q + u) : (R)0;
+    }
+#pragma unroll
+    for (int u = 0; u < kRL; ++u) {
+      xo[u] = rw[u] >= 0 ? xb[rw[u]] : (R)0;
+      oo[u] = rw[u] >= 0 ? oeb[rw[u]] : (R)0;
+    }
+#pragma unroll
+    for (int u = 0; u < kRL; ++u) {
+      if (rw[u] < 0) continue;
+      const R nx = xo[u] + rd * xx[u];
+      const R no = rexp(nx);
+      xb[rw[u]] = nx;
+      oeb[rw[u]] = no;
+      den += (double)no - (double)oo[u];
+    }
+  }
+  denom[key] = den;
+}
+
 constexpr int kIT = 4;   // nonzeros per lane: columns up to blockDim*kIT stay in registers
 constexpr int kITL = 8;  // nonzeros per lane per pass over longer columns
 
@@ -349,7 +430,28 @@ __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R
     const long long runs = MODEL == M_CLR ? (long long)(d.segPtr[j + 1] - d.segPtr[j]) : 0;
     const long long t0 = wall_clock64();
     double delta;
-    if (nj <= CH) {
+    if (MODEL == M_CLR && d.colMode[j]) {
+      const int64_t s0 = d.segPtr[j], s1 = d.segPtr[j + 1];
+      double v[2] = {0.0, 0.0};
+      for (int64_t sp = s0; sp < s1; sp += blockDim.x)
+        runGather<R>(d, fmt, b, (int)nj, s1, sp, oeb, denom, nw, v);
+      blockSum<2>(v);
+      double g = v[0] - xy, h = v[1];
+      if (h < 0.0) {
+        g = 0.0;
+        h = 0.0;
+      }
+      delta = applyBounds(priorDelta(pt, pv, g, h, bj), &bnd);
+      if (threadIdx.x == 0) {
+        bound[j] = bnd;
+        if (delta != 0.0) beta[j] = bj + delta;
+      }
+      if (delta != 0.0) {
+        for (int64_t sp = s0; sp < s1; sp += blockDim.x)
+          runUpdate<R>(d, fmt, b, (int)nj, s1, sp, xb, oeb, denom, delta);
+        __syncthreads();
+      }
+    } else if (nj <= CH) {
       Lanes<kIT, R> L;
       passLoad<MODEL, kIT, true, R>(d, fmt, b, e, b, oeb, xb, denom, nw, kw, useW, L);
       double v[2] = {0.0, 0.0};

@@ -538,6 +538,7 @@ DevX ccdh_data::view() const {
   x.segKey = segKey;
   x.stratStart = stratStart;
   x.nzKey = nzKey;
+  x.colMode = colMode;
   return x;
 }
 
@@ -693,7 +694,14 @@ int dataCreate(const ccdh_data_desc* desc, ccdh_data** out) {
     for (int k = 0; k < K; ++k) ss[d->hStrata[k] + 1]++;
     for (int s = 0; s < nStrata; ++s) ss[s + 1] += ss[s];
     d->hSegPtr = segPtr;
-    rc = devUpload(&d->segPtr, segPtr.data(), J + 1, &acc);
+    std::vector<int8_t> mode(J, 0);
+    for (int j = 0; j < J; ++j) {
+      const int64_t n = desc->col_ptr[j + 1] - desc->col_ptr[j];
+      const int64_t r = segPtr[j + 1] - segPtr[j];
+      mode[j] = (r > 0 && 2 * n >= 3 * r) ? 1 : 0;
+    }
+    rc = devUpload(&d->colMode, mode.data(), J, &acc);
+    rc = rc ? rc : devUpload(&d->segPtr, segPtr.data(), J + 1, &acc);
     rc = rc ? rc : devUpload(&d->segOff, segOff.data(), segOff.size(), &acc);
     rc = rc ? rc : devUpload(&d->segKey, segKey.data(), segKey.size(), &acc);
     rc = rc ? rc : devUpload(&d->stratStart, ss.data(), ss.size(), &acc);
@@ -710,7 +718,7 @@ int dataCreate(const ccdh_data_desc* desc, ccdh_data** out) {
 void dataDestroy(ccdh_data* d) {
   if (!d) return;
   void* ptrs[] = {d->colPtr, d->rows, d->vals, d->valsF, d->fmt, d->y, d->strata,
-                  d->segPtr, d->segOff, d->segKey, d->stratStart, d->nzKey};
+                  d->segPtr, d->segOff, d->segKey, d->stratStart, d->nzKey, d->colMode};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete d;

@@ -27,6 +27,7 @@ struct ccdh_data {
   int32_t* segKey = nullptr;
   int32_t* stratStart = nullptr;
   int32_t* nzKey = nullptr;
+  int8_t* colMode = nullptr;
   // host copies of the small per-column / per-row vectors
   std::vector<int64_t> hColPtr, hSegPtr;
   std::vector<int8_t> hFmt;
