@@ -91,6 +91,14 @@ struct BatchArgs {
   const FitCtl* ctl;
   FitStatus* st;
   const int32_t* active;  // fit index per workgroup
+  // per-fit records of long columns (stride strideRec)
+  size_t strideRec;
+  int32_t* recRow;
+  int32_t* recKey;
+  uint8_t* recFlags;
+  R* recO;
+  R* recXo;
+  double* recDen;
 };
 
 // ------------------------------------------------------------ gather pieces
@@ -180,28 +188,42 @@ __device__ void ghCox(const BatchArgs<R>& a, int j, int fmt, const R* oeb, const
   }
 }
 
-// ------------------------------------------------ striped run-owner passes
+// ------------------------------------------------------------ column passes
 // A column is walked in passes of blockDim*IT nonzeros with the striped
-// assignment p = pass start + it*blockDim + lane, so every index load of a wave
-// reads 64 consecutive entries.  Grouped models: the lane holding the last
-// nonzero of a run of equal stratum owns the run (runs are contiguous because
-// rows are sorted by stratum, and hold at most one stratum's rows); it sums the
-// run's numerators, applies the stratum's increment, and in the update moves
-// every row of the run and adds the denominator changes in row order, as the
-// reference's sequential loops do.  Each row belongs to one run, so no two
-// lanes touch the same row or stratum and the passes need no barrier.
+// assignment p = pass start + it*blockDim + lane, so each index load of a wave
+// reads 64 consecutive entries.  Grouped models: runs of equal stratum are
+// contiguous in a column (rows are sorted by stratum); the lane at a run's last
+// nonzero owns the stratum.  Runs of one nonzero (almost all) are summed in
+// registers; columns with longer runs stage per-nonzero values in LDS, the owner
+// sums its run there in row order, and a run that continues into the next pass
+// is carried.  Each row belongs to one run, so lanes never share a row or a
+// stratum.  Columns of at most blockDim*kIT nonzeros keep indices, exp(xBeta),
+// xBeta and stratum terms in registers from gather to update; longer columns
+// write one record per nonzero in the gather pass and the update pass reads the
+// records back in one coalesced hop.
+constexpr int kIT = 4;
+constexpr int kITL = 8;
+constexpr int kPassMax = 512 * kITL;
+
+struct PassLds {
+  double v1[kPassMax];
+  double v2[kPassMax];
+  unsigned char start[kPassMax];
+  double c1, c2, n1, n2;
+  int cValid, nValid;
+};
+
 template <int IT, typename R>
 struct Lanes {
   int row[IT];
   int key[IT];
   bool own[IT];    // valid nonzero that ends its run (always, for independent rows)
-  bool single[IT]; // run of one nonzero (always, for independent rows)
+  bool start[IT];  // valid nonzero that starts its run (always, for independent rows)
   R x[IT], o[IT], xo[IT];
   double den[IT], wn[IT], w[IT];
 };
 
-// index + state loads for one pass; KEEP also loads xBeta for a register-resident update
-template <int MODEL, int IT, bool KEEP, typename R>
+template <int MODEL, int IT, typename R>
 __device__ __forceinline__ void passLoad(const DevX& d, int fmt, int64_t b, int64_t e, int64_t c0,
                                          const R* oeb, const R* xb, const double* denom,
                                          const double* nw, const double* kw, int useW,
@@ -224,48 +246,93 @@ __device__ __forceinline__ void passLoad(const DevX& d, int fmt, int64_t b, int6
     const int r = L.row[it];
     const bool valid = r >= 0;
     L.own[it] = MODEL == M_CLR ? (valid && L.key[it] != kn[it]) : valid;
-    L.single[it] = MODEL == M_CLR ? (L.key[it] != kp[it]) : true;
+    L.start[it] = MODEL == M_CLR ? (valid && L.key[it] != kp[it]) : valid;
     L.o[it] = valid ? oeb[r] : (R)0;
-    L.xo[it] = (KEEP && valid) ? xb[r] : (R)0;
+    L.xo[it] = valid ? xb[r] : (R)0;
     L.w[it] = (MODEL != M_CLR && useW && valid) ? kw[r] : 1.0;
     L.den[it] = (MODEL == M_CLR && L.own[it]) ? denom[L.key[it]] : 1.0;
     L.wn[it] = (MODEL == M_CLR && L.own[it]) ? nw[L.key[it]] : 0.0;
   }
 }
 
-template <int MODEL, int IT, typename R>
-__device__ __forceinline__ void passGather(const DevX& d, int fmt, int64_t b, int64_t c0,
-                                           const R* oeb, const Lanes<IT, R>& L, int useW,
-                                           double (&v)[2]) {
-  const bool ind = isIndicator(fmt);
+// Run totals of NV per-nonzero values for the owners of one pass (see above).
+// `staged` is uniform: false when every run of the column has one nonzero.
+template <int IT, int NV>
+__device__ __forceinline__ void runSums(PassLds& S, int nLocal, const bool (&own)[IT],
+                                        const bool (&start)[IT], double (&val)[IT][NV],
+                                        bool staged) {
+  if (!staged) return;  // every owner's run is its own nonzero
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
-    if (!L.own[it]) continue;
-    if (MODEL == M_CLR) {
-      double n1, n2;
-      if (L.single[it]) {
-        const R m = L.o[it] * L.x[it];
-        n1 = (double)m;
-        n2 = ind ? 0.0 : (double)(m * L.x[it]);
-      } else {
-        const int64_t p = c0 + (int64_t)it * blockDim.x + threadIdx.x;
-        int64_t q = p;
-        while (q > b && d.nzKey[q - 1] == L.key[it]) --q;
-        n1 = 0.0;
-        n2 = 0.0;
-        for (; q <= p; ++q) {
-          const R xq = q == p ? L.x[it] : colValue<R>(d, fmt, q);
-          const R oq = q == p ? L.o[it] : oeb[colRow(d, fmt, q, b)];
-          const R m = oq * xq;
-          n1 += (double)m;
-          if (!ind) n2 += (double)(m * xq);
-        }
-      }
-      const double t = n1 / L.den[it];
+    const int li = it * blockDim.x + threadIdx.x;
+    if (li < nLocal) {
+      S.v1[li] = val[it][0];
+      if (NV > 1) S.v2[li] = val[it][1];
+      S.start[li] = start[it] ? 1 : 0;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int li = it * blockDim.x + threadIdx.x;
+    const bool last = li == nLocal - 1;
+    if (li >= nLocal || !(own[it] || last)) continue;
+    if (own[it] && start[it]) {
+      if (last) S.nValid = 0;
+      continue;
+    }
+    int k = li;
+    while (k > 0 && !S.start[k]) --k;
+    const bool before = !S.start[k];  // run began in an earlier pass
+    double s1 = (before && S.cValid) ? S.c1 : 0.0;
+    double s2 = (before && S.cValid) ? S.c2 : 0.0;
+    for (int q = k; q <= li; ++q) {
+      s1 += S.v1[q];
+      if (NV > 1) s2 += S.v2[q];
+    }
+    if (own[it]) {
+      val[it][0] = s1;
+      if (NV > 1) val[it][1] = s2;
+      if (last) S.nValid = 0;
+    } else {  // last nonzero of the pass, run continues
+      S.n1 = s1;
+      S.n2 = s2;
+      S.nValid = 1;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    S.c1 = S.n1;
+    S.c2 = S.n2;
+    S.cValid = S.nValid;
+  }
+}
+
+template <int MODEL, int IT, typename R>
+__device__ __forceinline__ void passGather(int fmt, PassLds& S, int nLocal, bool staged,
+                                           const Lanes<IT, R>& L, int useW, double (&v)[2]) {
+  const bool ind = isIndicator(fmt);
+  if (MODEL == M_CLR) {
+    double val[IT][2];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const R m = L.o[it] * L.x[it];
+      val[it][0] = L.row[it] >= 0 ? (double)m : 0.0;
+      val[it][1] = (L.row[it] >= 0 && !ind) ? (double)(m * L.x[it]) : 0.0;
+    }
+    runSums<IT, 2>(S, nLocal, L.own, L.start, val, staged);
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      if (!L.own[it]) continue;
+      const double t = val[it][0] / L.den[it];
       const double g = L.wn[it] * t;
       v[0] += g;
-      v[1] += ind ? g * (1.0 - t) : L.wn[it] * (n2 / L.den[it] - t * t);
-    } else {
+      v[1] += ind ? g * (1.0 - t) : L.wn[it] * (val[it][1] / L.den[it] - t * t);
+    }
+  } else {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      if (!L.own[it]) continue;
       const R n1 = L.o[it] * L.x[it];
       if (MODEL == M_LOGISTIC) {
         const R dd = (R)1 + L.o[it];
@@ -282,138 +349,94 @@ __device__ __forceinline__ void passGather(const DevX& d, int fmt, int64_t b, in
   }
 }
 
-// update of one pass; KEEP means xBeta was loaded by passLoad
-template <int MODEL, int IT, bool KEEP, typename R>
-__device__ __forceinline__ void passUpdate(const DevX& d, int fmt, int64_t b, int64_t c0, R* xb,
-                                           R* oeb, double* denom, const Lanes<IT, R>& L,
-                                           double delta) {
+template <int MODEL, int IT, typename R>
+__device__ __forceinline__ void passUpdate(PassLds& S, int nLocal, bool staged, R* xb, R* oeb,
+                                           double* denom, const Lanes<IT, R>& L, double delta) {
   const R rd = (R)delta;
+  double val[IT][1];
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
-    if (!L.own[it]) continue;
-    if (MODEL == M_CLR && !L.single[it]) {
-      const int64_t p = c0 + (int64_t)it * blockDim.x + threadIdx.x;
-      int64_t q = p;
-      while (q > b && d.nzKey[q - 1] == L.key[it]) --q;
-      double den = L.den[it];
-      for (; q <= p; ++q) {
-        const int r = q == p ? L.row[it] : colRow(d, fmt, q, b);
-        const R xq = q == p ? L.x[it] : colValue<R>(d, fmt, q);
-        const R oo = q == p ? L.o[it] : oeb[r];
-        const R nx = ((KEEP && q == p) ? L.xo[it] : xb[r]) + rd * xq;
-        const R no = rexp(nx);
-        xb[r] = nx;
-        oeb[r] = no;
-        den += (double)no - (double)oo;
-      }
-      denom[L.key[it]] = den;
+    val[it][0] = 0.0;
+    const int r = L.row[it];
+    if (r < 0) continue;
+    const R nx = L.xo[it] + rd * L.x[it];
+    const R no = rexp(nx);
+    xb[r] = nx;
+    oeb[r] = no;
+    val[it][0] = (double)no - (double)L.o[it];
+  }
+  if (MODEL == M_CLR) {
+    runSums<IT, 1>(S, nLocal, L.own, L.start, val, staged);
+#pragma unroll
+    for (int it = 0; it < IT; ++it)
+      if (L.own[it]) denom[L.key[it]] = L.den[it] + val[it][0];
+  }
+}
+
+// gather-pass records of long columns, read back by the update pass
+template <typename R>
+struct Records {
+  int32_t* row;
+  int32_t* key;
+  uint8_t* flags;  // bit 0 own, bit 1 start
+  R* o;
+  R* xo;
+  double* den;
+};
+
+template <int MODEL, int IT, typename R>
+__device__ __forceinline__ void storeRecords(const Records<R>& rec, int64_t off,
+                                             const Lanes<IT, R>& L) {
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    if (L.row[it] < 0) continue;
+    const int64_t q = off + (int64_t)it * blockDim.x + threadIdx.x;
+    rec.row[q] = L.row[it];
+    rec.o[q] = L.o[it];
+    rec.xo[q] = L.xo[it];
+    if (MODEL == M_CLR) {
+      rec.key[q] = L.key[it];
+      rec.flags[q] = (L.own[it] ? 1 : 0) | (L.start[it] ? 2 : 0);
+      rec.den[q] = L.den[it];
+    }
+  }
+}
+
+template <int MODEL, int IT, typename R>
+__device__ __forceinline__ void loadRecords(const DevX& d, int fmt, int64_t e, int64_t c0,
+                                            int64_t off, const Records<R>& rec, Lanes<IT, R>& L) {
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int64_t p = c0 + (int64_t)it * blockDim.x + threadIdx.x;
+    const int64_t q = off + (int64_t)it * blockDim.x + threadIdx.x;
+    const bool valid = p < e;
+    L.row[it] = valid ? rec.row[q] : -1;
+    L.x[it] = valid ? colValue<R>(d, fmt, p) : (R)0;
+    L.o[it] = valid ? rec.o[q] : (R)0;
+    L.xo[it] = valid ? rec.xo[q] : (R)0;
+    if (MODEL == M_CLR) {
+      const int fl = valid ? rec.flags[q] : 0;
+      L.key[it] = valid ? rec.key[q] : -1;
+      L.own[it] = fl & 1;
+      L.start[it] = fl & 2;
+      L.den[it] = valid ? rec.den[q] : 1.0;
     } else {
-      const int r = L.row[it];
-      const R nx = (KEEP ? L.xo[it] : xb[r]) + rd * L.x[it];
-      const R no = rexp(nx);
-      xb[r] = nx;
-      oeb[r] = no;
-      if (MODEL == M_CLR) denom[L.key[it]] = L.den[it] + ((double)no - (double)L.o[it]);
+      L.own[it] = valid;
+      L.start[it] = valid;
     }
   }
 }
 
-// Run mode (grouped columns whose stratum runs average >= 1.5 rows, i.e. the
-// near-dense columns): lanes take the precomputed runs striped, read the run's
-// bounds and stratum (one coalesced hop), then all of the run's rows and their
-// exp(xBeta) together, so a run costs the same latency whatever its length.
-constexpr int kRL = 8;  // rows of a run loaded together
-
-template <typename R>
-__device__ __forceinline__ void runGather(const DevX& d, int fmt, int64_t b, int nj, int64_t s1,
-                                          int64_t sp, const R* oeb, const double* denom,
-                                          const double* nw, double (&v)[2]) {
-  const bool ind = isIndicator(fmt);
-  const int64_t s = sp + threadIdx.x;
-  if (s >= s1) return;
-  const int q0 = d.segOff[s];
-  const int q1 = (s + 1 < s1) ? d.segOff[s + 1] : nj;
-  const int key = d.segKey[s];
-  const double den = denom[key], wn = nw[key];
-  double n1 = 0.0, n2 = 0.0;
-  for (int q = q0; q < q1; q += kRL) {
-    int rw[kRL];
-    R xx[kRL], oo[kRL];
-#pragma unroll
-    for (int u = 0; u < kRL; ++u) {
-      const bool in = q + u < q1;
-      rw[u] = in ? colRow(d, fmt, b + q + u, b) : -1;
-      xx[u] = in ? colValue<R>(d, fmt, b + q + u) : (R)0;
-    }
-#pragma unroll
-    for (int u = 0; u < kRL; ++u) oo[u] = rw[u] >= 0 ? oeb[rw[u]] : (R)0;
-#pragma unroll
-    for (int u = 0; u < kRL; ++u) {
-      if (rw[u] < 0) continue;
-      const R m = oo[u] * xx[u];
-      n1 += (double)m;
-      if (!ind) n2 += (double)(m * xx[u]);
-    }
-  }
-  const double t = n1 / den;
-  const double g = wn * t;
-  v[0] += g;
-  v[1] += ind ? g * (1.0 - t) : wn * (n2 / den - t * t);
-}
-
-template <typename R>
-__device__ __forceinline__ void runUpdate(const DevX& d, int fmt, int64_t b, int nj, int64_t s1,
-                                          int64_t sp, R* xb, R* oeb, double* denom,
-                                          double delta) {
-  const int64_t s = sp + threadIdx.x;
-  if (s >= s1) return;
-  const R rd = (R)delta;
-  const int q0 = d.segOff[s];
-  const int q1 = (s + 1 < s1) ? d.segOff[s + 1] : nj;
-  const int key = d.segKey[s];
-  double den = denom[key];
-  for (int q = q0; q < q1; q += kRL) {
-    int rw[kRL];
-    R xx[kRL], xo[kRL], oo[kRL];
-#pragma unroll
-    for (int u = 0; u < kRL; ++u) {
-      const bool in = q + u < q1;
-      rw[u] = in ? colRow(d, fmt, b + q + u, b) : -1;
-      xx[u] = in ? colValue<R>(d, fmt, b + q + u) : (R)0;
-    }
-#pragma unroll
-    for (int u = 0; u < kRL; ++u) {
-      xo[u] = rw[u] >= 0 ? xb[rw[u]] : (R)0;
-      oo[u] = rw[u] >= 0 ? oeb[rw[u]] : (R)0;
-    }
-#pragma unroll
-    for (int u = 0; u < kRL; ++u) {
-      if (rw[u] < 0) continue;
-      const R nx = xo[u] + rd * xx[u];
-      const R no = rexp(nx);
-      xb[rw[u]] = nx;
-      oeb[rw[u]] = no;
-      den += (double)no - (double)oo[u];
-    }
-  }
-  denom[key] = den;
-}
-
-constexpr int kIT = 4;   // nonzeros per lane: columns up to blockDim*kIT stay in registers
-constexpr int kITL = 8;  // nonzeros per lane per pass over longer columns
-
-// One full coordinate cycle of a fit (independent rows or grouped).  Columns of
-// at most blockDim*kIT nonzeros (the vast majority of a Zipf-shaped design) keep
-// their indices, exp(xBeta), xBeta and stratum terms in registers from the
-// gather to the update.  Every lane takes the (deterministic) prior step from the
-// block-reduced gradient and Hessian, so no broadcast barrier is needed; a barrier
-// follows an update only.
+// One full coordinate cycle of a fit (independent rows or grouped).  Every lane
+// takes the (deterministic) prior step from the block-reduced gradient and
+// Hessian, so no broadcast barrier is needed; a barrier follows an update only.
 template <int MODEL, typename R>
 __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R* oeb,
                          const double* kw, double* denom, const double* nw, double* beta,
                          double* bound, const double* xjy, const double* pvar,
-                         const int8_t* ptype, const uint8_t* fixed) {
+                         const int8_t* ptype, const uint8_t* fixed, const Records<R>& rec) {
   const DevX& d = a.d;
+  __shared__ PassLds S;
   const int64_t CH = (int64_t)blockDim.x * kIT;
   const int64_t CHL = (int64_t)blockDim.x * kITL;
   const int J = a.J;
@@ -427,35 +450,16 @@ __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R
     const double bj = beta[j], pv = pvar[j], xy = xjy[j];
     double bnd = bound[j];
     const int pt = ptype[j];
-    const long long runs = MODEL == M_CLR ? (long long)(d.segPtr[j + 1] - d.segPtr[j]) : 0;
+    const long long runs = MODEL == M_CLR ? (long long)(d.segPtr[j + 1] - d.segPtr[j]) : nj;
+    const bool staged = MODEL == M_CLR && runs < nj;
+    if (threadIdx.x == 0) S.cValid = 0;
     const long long t0 = wall_clock64();
     double delta;
-    if (MODEL == M_CLR && d.colMode[j]) {
-      const int64_t s0 = d.segPtr[j], s1 = d.segPtr[j + 1];
-      double v[2] = {0.0, 0.0};
-      for (int64_t sp = s0; sp < s1; sp += blockDim.x)
-        runGather<R>(d, fmt, b, (int)nj, s1, sp, oeb, denom, nw, v);
-      blockSum<2>(v);
-      double g = v[0] - xy, h = v[1];
-      if (h < 0.0) {
-        g = 0.0;
-        h = 0.0;
-      }
-      delta = applyBounds(priorDelta(pt, pv, g, h, bj), &bnd);
-      if (threadIdx.x == 0) {
-        bound[j] = bnd;
-        if (delta != 0.0) beta[j] = bj + delta;
-      }
-      if (delta != 0.0) {
-        for (int64_t sp = s0; sp < s1; sp += blockDim.x)
-          runUpdate<R>(d, fmt, b, (int)nj, s1, sp, xb, oeb, denom, delta);
-        __syncthreads();
-      }
-    } else if (nj <= CH) {
+    if (nj <= CH) {
       Lanes<kIT, R> L;
-      passLoad<MODEL, kIT, true, R>(d, fmt, b, e, b, oeb, xb, denom, nw, kw, useW, L);
+      passLoad<MODEL, kIT, R>(d, fmt, b, e, b, oeb, xb, denom, nw, kw, useW, L);
       double v[2] = {0.0, 0.0};
-      passGather<MODEL, kIT, R>(d, fmt, b, b, oeb, L, useW, v);
+      passGather<MODEL, kIT, R>(fmt, S, (int)nj, staged, L, useW, v);
       blockSum<2>(v);
       double g = nj > 0 ? v[0] - xy : 0.0, h = nj > 0 ? v[1] : 0.0;
       if (h < 0.0) {
@@ -468,15 +472,16 @@ __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R
         if (delta != 0.0) beta[j] = bj + delta;
       }
       if (delta != 0.0 && nj > 0) {
-        passUpdate<MODEL, kIT, true, R>(d, fmt, b, b, xb, oeb, denom, L, delta);
+        passUpdate<MODEL, kIT, R>(S, (int)nj, staged, xb, oeb, denom, L, delta);
         __syncthreads();
       }
     } else {
       double v[2] = {0.0, 0.0};
       for (int64_t c0 = b; c0 < e; c0 += CHL) {
         Lanes<kITL, R> L;
-        passLoad<MODEL, kITL, false, R>(d, fmt, b, e, c0, oeb, xb, denom, nw, kw, useW, L);
-        passGather<MODEL, kITL, R>(d, fmt, b, c0, oeb, L, useW, v);
+        passLoad<MODEL, kITL, R>(d, fmt, b, e, c0, oeb, xb, denom, nw, kw, useW, L);
+        storeRecords<MODEL, kITL, R>(rec, c0 - b, L);
+        passGather<MODEL, kITL, R>(fmt, S, (int)min(CHL, e - c0), staged, L, useW, v);
       }
       blockSum<2>(v);
       double g = v[0] - xy, h = v[1];
@@ -488,12 +493,14 @@ __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R
       if (threadIdx.x == 0) {
         bound[j] = bnd;
         if (delta != 0.0) beta[j] = bj + delta;
+        S.cValid = 0;
       }
       if (delta != 0.0) {
+        __syncthreads();  // carry reset visible; records were written by these lanes
         for (int64_t c0 = b; c0 < e; c0 += CHL) {
           Lanes<kITL, R> L;
-          passLoad<MODEL, kITL, false, R>(d, fmt, b, e, c0, oeb, xb, denom, nw, kw, useW, L);
-          passUpdate<MODEL, kITL, false, R>(d, fmt, b, c0, xb, oeb, denom, L, delta);
+          loadRecords<MODEL, kITL, R>(d, fmt, e, c0, c0 - b, rec, L);
+          passUpdate<MODEL, kITL, R>(S, (int)min(CHL, e - c0), staged, xb, oeb, denom, L, delta);
         }
         __syncthreads();
       }
@@ -506,10 +513,10 @@ __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R
       tS += t1 - t0;
     }
     cG += nj;
-    cGR += runs;
+    cGR += MODEL == M_CLR ? runs : 0;
     if (delta != 0.0) {
       cU += nj;
-      cUR += runs;
+      cUR += MODEL == M_CLR ? runs : 0;
     }
   }
   if (threadIdx.x == 0) {
@@ -779,7 +786,15 @@ __global__ void __launch_bounds__(BLOCK) k_batch_sweep(BatchArgs<R> a, const dou
   __shared__ double sDelta;
   for (int sw = 0; sw < sweeps; ++sw) {
     if (MODEL != M_COX) {
-      cycleFit<MODEL, R>(a, f, c, xb, oeb, kw, denom, nw, beta, bound, xjy, pvar, ptype, fixed);
+      Records<R> rec;
+      rec.row = a.recRow + f * a.strideRec;
+      rec.key = a.recKey ? a.recKey + f * a.strideRec : nullptr;
+      rec.flags = a.recFlags ? a.recFlags + f * a.strideRec : nullptr;
+      rec.o = a.recO + f * a.strideRec;
+      rec.xo = a.recXo + f * a.strideRec;
+      rec.den = a.recDen ? a.recDen + f * a.strideRec : nullptr;
+      cycleFit<MODEL, R>(a, f, c, xb, oeb, kw, denom, nw, beta, bound, xjy, pvar, ptype, fixed,
+                         rec);
     } else
     for (int j = 0; j < a.J; ++j) {
       if (fixed[j]) continue;
@@ -979,6 +994,20 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
     rc = rc ? rc : allocZero(&a.pid, F * sK, s, ow);
     rc = rc ? rc : allocZero(&a.pidFirst, F * sN, s, ow);
     rc = rc ? rc : allocZero(&a.reset, F * sN, s, ow);
+  }
+  if (MODEL != M_COX) {
+    int64_t maxLen = 1;
+    for (int j = 0; j < J; ++j) maxLen = std::max(maxLen, d->hColPtr[j + 1] - d->hColPtr[j]);
+    const size_t sR = roundUp((size_t)maxLen, 64);
+    a.strideRec = sR;
+    rc = rc ? rc : allocZero(&a.recRow, F * sR, s, ow);
+    rc = rc ? rc : allocZero(&a.recO, F * sR, s, ow);
+    rc = rc ? rc : allocZero(&a.recXo, F * sR, s, ow);
+    if (MODEL == M_CLR) {
+      rc = rc ? rc : allocZero(&a.recKey, F * sR, s, ow);
+      rc = rc ? rc : allocZero(&a.recFlags, F * sR, s, ow);
+      rc = rc ? rc : allocZero(&a.recDen, F * sR, s, ow);
+    }
   }
   rc = rc ? rc : allocZero(&a.xjy, F * sJ, s, ow);
   rc = rc ? rc : allocZero(&a.beta, F * sJ, s, ow);
