@@ -64,6 +64,7 @@ def main():
     ap.add_argument("--folds", type=int, default=10)
     ap.add_argument("--grid", type=int, default=16)
     ap.add_argument("--limit", type=int, default=0, help="run only the last N fits (probe)")
+    ap.add_argument("--max-iter", type=int, default=1000, help="iteration cap (probe runs)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a device copy of this size (torch) for counter calibration")
     args = ap.parse_args()
@@ -93,7 +94,8 @@ def main():
         var = api.cv_grid_point(1e-4, 1.0, grid, g)
         for f in range(folds):
             w = api.cv_fold_weights(prob.strata, folds, 123, f, by_pid=True)
-            specs.append(default_spec(prob, prior=PRIOR_LAPLACE, variance=var, weights=w))
+            specs.append(default_spec(prob, prior=PRIOR_LAPLACE, variance=var, weights=w,
+                                      max_iterations=args.max_iter))
     if args.limit:
         specs = specs[-args.limit:]
     n_fits = len(specs)
