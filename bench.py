@@ -100,14 +100,21 @@ def main():
         specs = specs[-args.limit:]
     n_fits = len(specs)
     if args.copy_calibration_mb:
-        import torch
-        n = args.copy_calibration_mb * (1 << 20) // 8
-        src = torch.ones(n, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"copy calibration: {n * 8} bytes read + {n * 8} bytes written")
-        del src, dst
+        # a plain device-to-device copy of known size (HIP runtime blit kernel), used
+        # to scale the FETCH_SIZE / WRITE_SIZE counters
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
+        nbytes = args.copy_calibration_mb << 20
+        src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+        hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes))
+        hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes))
+        hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
+        hip.hipDeviceSynchronize()
+        hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3)  # device to device
+        hip.hipDeviceSynchronize()
+        hip.hipFree(src)
+        hip.hipFree(dst)
+        log(f"copy calibration: {nbytes} bytes read + {nbytes} bytes written")
     opt = dict(threads=args.threads, sweeps_per_sync=args.sweeps_per_sync, use_graph=1)
 
     # load the code objects and touch the allocator (untimed)
