@@ -149,6 +149,16 @@ def main():
     mx = int(iters.max())
     log(f"slowest fit: {mx} sweeps in {lat.max():.2f}s -> {lat.max() / mx * 1e3:.1f} ms/sweep")
 
+    # HBM traffic per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
+    # workload (profiles/), used when the workload matches
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_batch_sweep_traffic.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath))
+        if (t.get("strata") == n_strata and t.get("covariates") == P and t.get("fits") == n_fits
+                and precision == api.FP64):
+            traffic = t["traffic_bytes_per_launch"]
+
     # plain engine and CPU oracle on a spread of the same fits
     sample = np.linspace(0, n_fits - 1, max(args.plain_fits, args.cpu_fits, 1)).astype(int)
     plain_t, plain_err = [], 0.0
@@ -203,7 +213,7 @@ def main():
         "max_rel_coef_err_vs_cpu_oracle": coef_err,
         "max_rel_coef_diff_plain_vs_batched": plain_err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_batch_sweep", "avg_launch_ms": per_launch_ms,
                      "launches": stats["launches"]},
         "cpu_baseline": cpu,
