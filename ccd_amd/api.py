@@ -24,7 +24,9 @@ EXPORTS = [
     "ccdh_remaining_stats", "ccdh_numerator", "ccdh_grad_hess", "ccdh_update_xbeta",
     "ccdh_loglik", "ccdh_gradient_objective", "ccdh_pred_loglik", "ccdh_get_xbeta",
     "ccdh_fit_plain", "ccdh_batch_fit", "ccdh_batch_last_stats", "ccdh_cv_fold_weights",
-    "ccdh_cv_grid_point",
+    "ccdh_cv_grid_point", "ccdh_server_create", "ccdh_server_submit",
+    "ccdh_server_submit_resident", "ccdh_server_poll", "ccdh_server_wait",
+    "ccdh_server_get_stats", "ccdh_server_destroy",
 ]
 
 
@@ -82,6 +84,31 @@ class BatchOptions(ctypes.Structure):
                 ("use_graph", ctypes.c_int), ("_pad", ctypes.c_int)]
 
 
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("max_slots", ctypes.c_int),
+                ("threads", ctypes.c_int), ("_pad", ctypes.c_int),
+                ("quantum_ms", ctypes.c_double), ("hbm_budget_bytes", ctypes.c_int64)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("latency", ctypes.c_double), ("queued", ctypes.c_double),
+                ("prepare", ctypes.c_double), ("footprint", ctypes.c_int64),
+                ("launches", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "submitted", "completed", "failed", "in_flight", "queued", "launches")] + [
+        ("kernel_ms", ctypes.c_double)] + [(n, ctypes.c_int64) for n in (
+            "arena_bytes", "arena_used", "arena_peak")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -123,6 +150,13 @@ def load_library(path=LIB_PATH):
         "ccdh_batch_last_stats": (ci, [vp]),
         "ccdh_cv_fold_weights": (ci, [vp, ci, ci, ci, ctypes.c_long, ci, vp]),
         "ccdh_cv_grid_point": (cd, [cd, cd, ci, ci]),
+        "ccdh_server_create": (ci, [vp, vp]),
+        "ccdh_server_submit": (ci, [vp, vp, vp, vp]),
+        "ccdh_server_submit_resident": (ci, [vp, vp, vp, vp]),
+        "ccdh_server_poll": (ci, [vp, ctypes.c_int64]),
+        "ccdh_server_wait": (ci, [vp, ctypes.c_int64, cd, vp, vp, vp]),
+        "ccdh_server_get_stats": (ci, [vp, vp]),
+        "ccdh_server_destroy": (ci, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -150,6 +184,24 @@ def device_info():
     return buf.value.decode(), cu.value, gb.value
 
 
+def _desc(p: CcdProblem, precision):
+    """ccdh_data_desc over contiguous copies of the problem's arrays (kept in the
+    returned dict, which must outlive every use of the descriptor)."""
+    a = dict(
+        col_ptr=np.ascontiguousarray(p.col_ptr, np.int64),
+        rows=np.ascontiguousarray(p.rows, np.int32),
+        vals=np.ascontiguousarray(p.vals, np.float64),
+        fmt=np.ascontiguousarray(p.fmt, np.int8),
+        y=np.ascontiguousarray(p.y, np.float64),
+        offs=np.ascontiguousarray(p.offs, np.float64),
+        strata=np.ascontiguousarray(p.strata, np.int32),
+    )
+    desc = DataDesc(p.model, precision, p.K, p.J, _p(a["col_ptr"]), _p(a["rows"]),
+                    _p(a["vals"]), _p(a["fmt"]), _p(a["y"]), _p(a["offs"]), _p(a["strata"]),
+                    int(p.has_offset))
+    return desc, a
+
+
 class Data:
     """A problem resident in GPU memory (ccdh_data_t)."""
 
@@ -157,19 +209,7 @@ class Data:
         lib = load_library()
         self.problem = p
         self.precision = precision
-        self._arrs = dict(
-            col_ptr=np.ascontiguousarray(p.col_ptr, np.int64),
-            rows=np.ascontiguousarray(p.rows, np.int32),
-            vals=np.ascontiguousarray(p.vals, np.float64),
-            fmt=np.ascontiguousarray(p.fmt, np.int8),
-            y=np.ascontiguousarray(p.y, np.float64),
-            offs=np.ascontiguousarray(p.offs, np.float64),
-            strata=np.ascontiguousarray(p.strata, np.int32),
-        )
-        a = self._arrs
-        desc = DataDesc(p.model, precision, p.K, p.J, _p(a["col_ptr"]), _p(a["rows"]),
-                        _p(a["vals"]), _p(a["fmt"]), _p(a["y"]), _p(a["offs"]), _p(a["strata"]),
-                        int(p.has_offset))
+        desc, self._arrs = _desc(p, precision)
         h = ctypes.c_void_p()
         _check(lib.ccdh_data_create(ctypes.byref(desc), ctypes.byref(h)))
         self.handle = h
@@ -311,6 +351,83 @@ class FitHandle:
         if self.handle:
             self.lib.ccdh_fit_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching fit server (ccdh_server_*).
+
+    submit() queues a fit over a host problem (uploaded when admitted) or over a
+    resident Data; wait() returns (beta, result, info) or raises the request's
+    error.  The host arrays of a submitted problem are kept here until the
+    request is collected."""
+
+    def __init__(self, precision=FP64, max_slots=0, threads=0, quantum_ms=0.0,
+                 hbm_budget_bytes=0):
+        self.lib = load_library()
+        opt = ServerOptions(precision, max_slots, threads, 0, quantum_ms, hbm_budget_bytes)
+        h = ctypes.c_void_p()
+        _check(self.lib.ccdh_server_create(ctypes.byref(opt), ctypes.byref(h)))
+        self.handle = h
+        self.precision = precision
+        self._keep = {}
+
+    def submit(self, problem, spec: FitSpec, heldout=None):
+        keep = []
+        s = _spec_struct(spec, keep, heldout)
+        rid = ctypes.c_int64()
+        if isinstance(problem, Data):
+            _check(self.lib.ccdh_server_submit_resident(self.handle, problem.handle,
+                                                        ctypes.byref(s), ctypes.byref(rid)))
+            J = problem.problem.J
+        else:
+            desc, arrs = _desc(problem, self.precision)
+            keep.append(arrs)
+            _check(self.lib.ccdh_server_submit(self.handle, ctypes.byref(desc), ctypes.byref(s),
+                                               ctypes.byref(rid)))
+            J = problem.J
+        self._keep[rid.value] = (keep, J)
+        return rid.value
+
+    def poll(self, rid):
+        rc = self.lib.ccdh_server_poll(self.handle, rid)
+        if rc < 0:
+            _check(rc)
+        return rc == 1
+
+    def wait(self, rid, timeout_ms=-1.0):
+        keep, J = self._keep[rid]
+        beta = np.zeros(J)
+        res, info = FitResult(), RequestInfo()
+        rc = self.lib.ccdh_server_wait(self.handle, rid, float(timeout_ms), _p(beta),
+                                       ctypes.byref(res), ctypes.byref(info))
+        if rc == -4 and timeout_ms >= 0 and "timeout" in self.lib.ccdh_last_error().decode():
+            _check(rc)  # still in flight: keep its arrays
+        del self._keep[rid]
+        _check(rc)
+        return beta, res.as_dict(), info.as_dict()
+
+    def stats(self):
+        st = ServerStats()
+        _check(self.lib.ccdh_server_get_stats(self.handle, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self.handle:
+            self.lib.ccdh_server_destroy(self.handle)
+            self.handle = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
         try:

@@ -16,13 +16,14 @@ SOURCES = [
     "ccd_engine.hip",
     "ccd_batch.hip",
     "ccd_cv.cpp",
+    "ccd_serve.cpp",
     "engine/HipModelSpecifics.cpp",
     "engine/HostCyclicCoordinateDescent.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CCDH_ARCH", "gfx950")
-COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-pthread", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-function", "-Wno-unused-result",
           f"-I{os.path.join(ROOT, 'include')}"]
@@ -55,7 +56,7 @@ def build(verbose=False, jobs=8):
         if verbose and err.strip():
             print(err)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     return LIB
 

@@ -1,5 +1,5 @@
-// ccd_batch.hip -- device-resident batched CCD: many fits over one resident
-// data set, each fit's whole coordinate sweep run inside one workgroup.
+// ccd_batch.hip -- device-resident CCD over fit slots: many fits in flight, each
+// fit's whole coordinate sweep run inside one workgroup.
 //
 // Replaces, for batches of fits, the host loop of CyclicCoordinateDescent::findMode
 // (CyclicCoordinateDescent.cpp:909-1216) around the per-coordinate engine calls:
@@ -10,14 +10,16 @@
 // The batches are AbstractCrossValidationDriver's fold x grid-point fits
 // (drivers/AbstractCrossValidationDriver.cpp:131-248) or served requests.
 //
-// Design (MI355X): one workgroup owns one fit for a whole sweep; the column's
+// Design (MI355X): one workgroup owns one slot for a whole sweep; the column's
 // nonzeros (or stratum runs) are spread over the workgroup's lanes, the
-// gradient and Hessian are reduced in FP64 with wave64 shuffles + LDS, lane 0
-// takes the prior step, and the same lanes apply the update.  No grid-wide
-// barrier and no host round trip per coordinate.  Per-fit state is contiguous
-// per fit, so a workgroup streams its own rows; X (indices, values, runs) is
-// read by every workgroup and stays hot in L2/MALL.  Reductions are in a fixed
-// order, so a fit's result does not depend on what else is in the batch.
+// gradient and Hessian are reduced in FP64 with wave64 shuffles + LDS, every
+// lane takes the prior step, and the same lanes apply the update.  No
+// grid-wide barrier and no host round trip per coordinate.  A slot's state is
+// one contiguous block, so a workgroup streams its own rows; X (indices,
+// values, runs) is read by every workgroup over it and stays hot in L2/MALL.
+// Reductions are in a fixed order, so a fit's result does not depend on what
+// else is in the launch.  The sweep kernel dispatches on each slot's model, so
+// slots of different models and data sets share one launch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,10 +27,12 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "ccd_device.h"
 #include "ccd_host.h"
+#include "ccd_slot.h"
 
 using namespace ccdamd;
 
@@ -43,66 +47,7 @@ namespace {
 
 constexpr int kItems = 4;  // Cox pid items per lane per scan chunk
 
-struct FitCtl {
-  double tolerance;
-  double initialBound;
-  int maxIterations;
-  int convergenceType;
-  int useW;
-  int pad;
-};
-
-struct FitStatus {
-  double lastObj;
-  double logLik;
-  double predLogLik;
-  int iteration;
-  int flag;
-  int done;
-  int N;  // groups of this fit (Cox: tie groups under its weights)
-  // work counters (thread 0), read by the host for the bandwidth figure
-  long long gatherNnz, gatherRuns, updateNnz, updateRuns, scanPids, objRows;
-  // wall-clock ticks (thread 0) spent per phase: single-pass columns, long columns, objective
-  long long tSmall, tLarge, tObj, nLarge;
-};
-
-template <typename R>
-struct BatchArgs {
-  DevX d;
-  int model, K, J, nFits;
-  size_t strideK, strideN, strideJ;
-  R* xb;
-  R* oeb;
-  double* kw;
-  double* heldout;  // optional [nFits * K]
-  double* denom;
-  double* nw;
-  double* numer;
-  double* numer2;
-  double* xjy;
-  double* beta;
-  double* bound;
-  double* pvar;
-  int8_t* ptype;
-  uint8_t* fixed;
-  int32_t* pid;
-  int32_t* pidFirst;
-  uint8_t* reset;
-  const FitCtl* ctl;
-  FitStatus* st;
-  const int32_t* active;  // fit index per workgroup
-  // per-fit records of long columns (stride strideRec)
-  size_t strideRec;
-  int32_t* recRow;
-  int32_t* recKey;
-  uint8_t* recFlags;
-  R* recO;
-  R* recXo;
-  double* recDen;
-};
-
-// ------------------------------------------------------------ gather pieces
-
+// ------------------------------------------------------------ Cox pieces
 
 template <typename R>
 __device__ __forceinline__ bool coxHead(const DevX& d, int fmt, int64_t b, int64_t p,
@@ -120,10 +65,14 @@ __device__ __forceinline__ bool coxHead(const DevX& d, int fmt, int64_t b, int64
 // Cox inside one workgroup: numerators by run heads, then one pass over all
 // pids with a block-level segmented scan carrying across chunks.
 template <typename R>
-__device__ void ghCox(const BatchArgs<R>& a, int j, int fmt, const R* oeb, const double* kw,
-                      int useW, const int32_t* pid, int N, const double* denom, double* numer,
-                      double* numer2, const uint8_t* reset, const double* nw, double (&v)[2]) {
-  const DevX& d = a.d;
+__device__ __forceinline__ void ghCox(const Slot<R>& s, int j, int fmt, int useW, double (&v)[2]) {
+  const DevX& d = s.d;
+  const R* oeb = s.oeb;
+  const double* kw = s.kw;
+  const int32_t* pid = s.pid;
+  const int N = s.N;
+  double* numer = s.numer;
+  double* numer2 = s.numer2;
   const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
   const bool ind = isIndicator(fmt);
   for (int64_t p = b + threadIdx.x; p < e; p += blockDim.x) {
@@ -161,8 +110,8 @@ __device__ void ghCox(const BatchArgs<R>& a, int j, int fmt, const R* oeb, const
       x.f = 0;
       x.v[0] = x.v[1] = x.v[2] = 0.0;
       if (i < N) {
-        x.f = reset[i];
-        x.v[0] = denom[i];
+        x.f = s.reset[i];
+        x.v[0] = s.denom[i];
         x.v[1] = numer[i];
         x.v[2] = ind ? 0.0 : numer2[i];
       }
@@ -178,14 +127,106 @@ __device__ void ghCox(const BatchArgs<R>& a, int j, int fmt, const R* oeb, const
       run = segCombine(run, items[it]);
       const double accD = run.v[0];
       const double t = run.v[1] / accD;
-      const double g = nw[i] * t;
+      const double g = s.nw[i] * t;
       v[0] += g;
-      v[1] += ind ? g * (1.0 - t) : nw[i] * (run.v[2] / accD - t * t);
+      v[1] += ind ? g * (1.0 - t) : s.nw[i] * (run.v[2] / accD - t * t);
       if (items[it].v[1] != 0.0) numer[i] = 0.0;
       if (!ind && items[it].v[2] != 0.0) numer2[i] = 0.0;
     }
     carry = segCombine(carry, total);
   }
+}
+
+template <typename R>
+__device__ __forceinline__ void updCox(const Slot<R>& s, int j, int fmt, int useW, double delta) {
+  const DevX& d = s.d;
+  R* xb = s.xb;
+  R* oeb = s.oeb;
+  const double* kw = s.kw;
+  const int32_t* pid = s.pid;
+  const int N = s.N;
+  const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
+  const R rd = (R)delta;
+  for (int64_t p = b + threadIdx.x; p < e; p += blockDim.x) {
+    int pp;
+    if (!coxHead<R>(d, fmt, b, p, pid, N, &pp)) {
+      if (pp >= N) {
+        const int k = colRow(d, fmt, p, b);
+        const R nx = xb[k] + rd * colValue<R>(d, fmt, p);
+        xb[k] = nx;
+        oeb[k] = rexp(nx);
+      }
+      continue;
+    }
+    double den = s.denom[pp];
+    for (int64_t q = p; q < e; ++q) {
+      const int k = colRow(d, fmt, q, b);
+      const int pq = pid[k];
+      if (pq >= N) continue;
+      if (pq != pp) break;
+      const R nx = xb[k] + rd * colValue<R>(d, fmt, q);
+      xb[k] = nx;
+      const double w = useW ? kw[k] : 1.0;
+      const double oldE = useW ? w * (double)oeb[k] : (double)oeb[k];
+      const R o = rexp(nx);
+      oeb[k] = o;
+      const double newE = useW ? w * (double)o : (double)o;
+      den += newE - oldE;
+    }
+    s.denom[pp] = den;
+  }
+}
+
+// True when the launch's time quantum has run out (block-uniform).
+__device__ __forceinline__ bool quantumOver(long long deadline) {
+  __shared__ int sOver;
+  if (deadline <= 0) return false;
+  __syncthreads();
+  if (threadIdx.x == 0) sOver = wall_clock64() >= deadline;
+  __syncthreads();
+  return sOver != 0;
+}
+
+// One coordinate cycle of a Cox fit inside one workgroup, from coordinate j0.
+// Returns the coordinate to resume at when the quantum ran out, else J.
+template <typename R>
+__device__ __forceinline__ int cycleCox(const Slot<R>& s, FitStatus& cs, int j0, long long deadline) {
+  const DevX& d = s.d;
+  const int useW = s.ctl.useW;
+  __shared__ double sDelta;
+  for (int j = j0; j < s.J; ++j) {
+    if (j > j0 && (j & 3) == 0 && quantumOver(deadline)) return j;
+    if (s.fixed[j]) continue;
+    const int64_t nj = d.colPtr[j + 1] - d.colPtr[j];
+    const int fmt = d.fmt[j];
+    double v[2] = {0.0, 0.0};
+    if (nj > 0) {
+      ghCox<R>(s, j, fmt, useW, v);
+      blockSum<2>(v);
+    }
+    if (threadIdx.x == 0) {
+      cs.gatherNnz += nj;
+      if (nj > 0) cs.scanPids += s.N;
+      double g = nj > 0 ? v[0] - s.xjy[j] : 0.0;
+      double h = nj > 0 ? v[1] : 0.0;
+      if (h < 0.0) {
+        g = 0.0;
+        h = 0.0;
+      }
+      double delta = priorDelta(s.ptype[j], s.pvar[j], g, h, s.beta[j]);
+      delta = applyBounds(delta, &s.bound[j]);
+      if (delta != 0.0) {
+        s.beta[j] += delta;
+        cs.updateNnz += nj;
+      }
+      sDelta = delta;
+    }
+    __syncthreads();
+    const double delta = sDelta;
+    if (delta != 0.0) updCox<R>(s, j, fmt, useW, delta);
+    __syncthreads();
+  }
+  return s.J;
 }
 
 // ------------------------------------------------------------ column passes
@@ -203,12 +244,12 @@ __device__ void ghCox(const BatchArgs<R>& a, int j, int fmt, const R* oeb, const
 // records back in one coalesced hop.
 constexpr int kIT = 4;
 constexpr int kITL = 8;
-constexpr int kPassMax = 512 * kITL;
 
+template <int NMAX>
 struct PassLds {
-  double v1[kPassMax];
-  double v2[kPassMax];
-  unsigned char start[kPassMax];
+  double v1[NMAX];
+  double v2[NMAX];
+  unsigned char start[NMAX];
   double c1, c2, n1, n2;
   int cValid, nValid;
 };
@@ -224,10 +265,9 @@ struct Lanes {
 };
 
 template <int MODEL, int IT, typename R>
-__device__ __forceinline__ void passLoad(const DevX& d, int fmt, int64_t b, int64_t e, int64_t c0,
-                                         const R* oeb, const R* xb, const double* denom,
-                                         const double* nw, const double* kw, int useW,
-                                         Lanes<IT, R>& L) {
+__device__ __forceinline__ void passLoad(const Slot<R>& s, int fmt, int64_t b, int64_t e,
+                                         int64_t c0, int useW, Lanes<IT, R>& L) {
+  const DevX& d = s.d;
   int kp[IT], kn[IT];
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
@@ -247,18 +287,18 @@ __device__ __forceinline__ void passLoad(const DevX& d, int fmt, int64_t b, int6
     const bool valid = r >= 0;
     L.own[it] = MODEL == M_CLR ? (valid && L.key[it] != kn[it]) : valid;
     L.start[it] = MODEL == M_CLR ? (valid && L.key[it] != kp[it]) : valid;
-    L.o[it] = valid ? oeb[r] : (R)0;
-    L.xo[it] = valid ? xb[r] : (R)0;
-    L.w[it] = (MODEL != M_CLR && useW && valid) ? kw[r] : 1.0;
-    L.den[it] = (MODEL == M_CLR && L.own[it]) ? denom[L.key[it]] : 1.0;
-    L.wn[it] = (MODEL == M_CLR && L.own[it]) ? nw[L.key[it]] : 0.0;
+    L.o[it] = valid ? s.oeb[r] : (R)0;
+    L.xo[it] = valid ? s.xb[r] : (R)0;
+    L.w[it] = (MODEL != M_CLR && useW && valid) ? s.kw[r] : 1.0;
+    L.den[it] = (MODEL == M_CLR && L.own[it]) ? s.denom[L.key[it]] : 1.0;
+    L.wn[it] = (MODEL == M_CLR && L.own[it]) ? s.nw[L.key[it]] : 0.0;
   }
 }
 
 // Run totals of NV per-nonzero values for the owners of one pass (see above).
 // `staged` is uniform: false when every run of the column has one nonzero.
-template <int IT, int NV>
-__device__ __forceinline__ void runSums(PassLds& S, int nLocal, const bool (&own)[IT],
+template <int IT, int NV, class LDS>
+__device__ __forceinline__ void runSums(LDS& S, int nLocal, const bool (&own)[IT],
                                         const bool (&start)[IT], double (&val)[IT][NV],
                                         bool staged) {
   if (!staged) return;  // every owner's run is its own nonzero
@@ -267,7 +307,7 @@ __device__ __forceinline__ void runSums(PassLds& S, int nLocal, const bool (&own
     const int li = it * blockDim.x + threadIdx.x;
     if (li < nLocal) {
       S.v1[li] = val[it][0];
-      if (NV > 1) S.v2[li] = val[it][1];
+      if constexpr (NV > 1) S.v2[li] = val[it][1];
       S.start[li] = start[it] ? 1 : 0;
     }
   }
@@ -288,11 +328,11 @@ __device__ __forceinline__ void runSums(PassLds& S, int nLocal, const bool (&own
     double s2 = (before && S.cValid) ? S.c2 : 0.0;
     for (int q = k; q <= li; ++q) {
       s1 += S.v1[q];
-      if (NV > 1) s2 += S.v2[q];
+      if constexpr (NV > 1) s2 += S.v2[q];
     }
     if (own[it]) {
       val[it][0] = s1;
-      if (NV > 1) val[it][1] = s2;
+      if constexpr (NV > 1) val[it][1] = s2;
       if (last) S.nValid = 0;
     } else {  // last nonzero of the pass, run continues
       S.n1 = s1;
@@ -308,8 +348,8 @@ __device__ __forceinline__ void runSums(PassLds& S, int nLocal, const bool (&own
   }
 }
 
-template <int MODEL, int IT, typename R>
-__device__ __forceinline__ void passGather(int fmt, PassLds& S, int nLocal, bool staged,
+template <int MODEL, int IT, typename R, class LDS>
+__device__ __forceinline__ void passGather(int fmt, LDS& S, int nLocal, bool staged,
                                            const Lanes<IT, R>& L, int useW, double (&v)[2]) {
   const bool ind = isIndicator(fmt);
   if (MODEL == M_CLR) {
@@ -349,9 +389,9 @@ __device__ __forceinline__ void passGather(int fmt, PassLds& S, int nLocal, bool
   }
 }
 
-template <int MODEL, int IT, typename R>
-__device__ __forceinline__ void passUpdate(PassLds& S, int nLocal, bool staged, R* xb, R* oeb,
-                                           double* denom, const Lanes<IT, R>& L, double delta) {
+template <int MODEL, int IT, typename R, class LDS>
+__device__ __forceinline__ void passUpdate(const Slot<R>& s, LDS& S, int nLocal, bool staged,
+                                           const Lanes<IT, R>& L, double delta) {
   const R rd = (R)delta;
   double val[IT][1];
 #pragma unroll
@@ -361,28 +401,17 @@ __device__ __forceinline__ void passUpdate(PassLds& S, int nLocal, bool staged, 
     if (r < 0) continue;
     const R nx = L.xo[it] + rd * L.x[it];
     const R no = rexp(nx);
-    xb[r] = nx;
-    oeb[r] = no;
+    s.xb[r] = nx;
+    s.oeb[r] = no;
     val[it][0] = (double)no - (double)L.o[it];
   }
   if (MODEL == M_CLR) {
     runSums<IT, 1>(S, nLocal, L.own, L.start, val, staged);
 #pragma unroll
     for (int it = 0; it < IT; ++it)
-      if (L.own[it]) denom[L.key[it]] = L.den[it] + val[it][0];
+      if (L.own[it]) s.denom[L.key[it]] = L.den[it] + val[it][0];
   }
 }
-
-// gather-pass records of long columns, read back by the update pass
-template <typename R>
-struct Records {
-  int32_t* row;
-  int32_t* key;
-  uint8_t* flags;  // bit 0 own, bit 1 start
-  R* o;
-  R* xo;
-  double* den;
-};
 
 template <int MODEL, int IT, typename R>
 __device__ __forceinline__ void storeRecords(const Records<R>& rec, int64_t off,
@@ -430,26 +459,29 @@ __device__ __forceinline__ void loadRecords(const DevX& d, int fmt, int64_t e, i
 // One full coordinate cycle of a fit (independent rows or grouped).  Every lane
 // takes the (deterministic) prior step from the block-reduced gradient and
 // Hessian, so no broadcast barrier is needed; a barrier follows an update only.
-template <int MODEL, typename R>
-__device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R* oeb,
-                         const double* kw, double* denom, const double* nw, double* beta,
-                         double* bound, const double* xjy, const double* pvar,
-                         const int8_t* ptype, const uint8_t* fixed, const Records<R>& rec) {
-  const DevX& d = a.d;
-  __shared__ PassLds S;
+// From coordinate j0; returns the coordinate to resume at when the quantum ran
+// out, else J.
+template <int MODEL, typename R, class LDS>
+__device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus& cs, LDS& S, int j0, long long deadline) {
+  const DevX& d = s.d;
   const int64_t CH = (int64_t)blockDim.x * kIT;
   const int64_t CHL = (int64_t)blockDim.x * kITL;
-  const int J = a.J;
-  const int useW = c.useW;
+  const int J = s.J;
+  const int useW = s.ctl.useW;
   long long cG = 0, cGR = 0, cU = 0, cUR = 0, tS = 0, tL = 0, nL = 0;
-  for (int j = 0; j < J; ++j) {
-    if (fixed[j]) continue;
+  int jEnd = J;
+  for (int j = j0; j < J; ++j) {
+    if (j > j0 && (j & 3) == 0 && quantumOver(deadline)) {
+      jEnd = j;
+      break;
+    }
+    if (s.fixed[j]) continue;
     const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
     const int64_t nj = e - b;
     const int fmt = d.fmt[j];
-    const double bj = beta[j], pv = pvar[j], xy = xjy[j];
-    double bnd = bound[j];
-    const int pt = ptype[j];
+    const double bj = s.beta[j], pv = s.pvar[j], xy = s.xjy[j];
+    double bnd = s.bound[j];
+    const int pt = s.ptype[j];
     const long long runs = MODEL == M_CLR ? (long long)(d.segPtr[j + 1] - d.segPtr[j]) : nj;
     const bool staged = MODEL == M_CLR && runs < nj;
     if (threadIdx.x == 0) S.cValid = 0;
@@ -457,7 +489,7 @@ __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R
     double delta;
     if (nj <= CH) {
       Lanes<kIT, R> L;
-      passLoad<MODEL, kIT, R>(d, fmt, b, e, b, oeb, xb, denom, nw, kw, useW, L);
+      passLoad<MODEL, kIT, R>(s, fmt, b, e, b, useW, L);
       double v[2] = {0.0, 0.0};
       passGather<MODEL, kIT, R>(fmt, S, (int)nj, staged, L, useW, v);
       blockSum<2>(v);
@@ -468,19 +500,19 @@ __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R
       }
       delta = applyBounds(priorDelta(pt, pv, g, h, bj), &bnd);
       if (threadIdx.x == 0) {
-        bound[j] = bnd;
-        if (delta != 0.0) beta[j] = bj + delta;
+        s.bound[j] = bnd;
+        if (delta != 0.0) s.beta[j] = bj + delta;
       }
       if (delta != 0.0 && nj > 0) {
-        passUpdate<MODEL, kIT, R>(S, (int)nj, staged, xb, oeb, denom, L, delta);
+        passUpdate<MODEL, kIT, R>(s, S, (int)nj, staged, L, delta);
         __syncthreads();
       }
     } else {
       double v[2] = {0.0, 0.0};
       for (int64_t c0 = b; c0 < e; c0 += CHL) {
         Lanes<kITL, R> L;
-        passLoad<MODEL, kITL, R>(d, fmt, b, e, c0, oeb, xb, denom, nw, kw, useW, L);
-        storeRecords<MODEL, kITL, R>(rec, c0 - b, L);
+        passLoad<MODEL, kITL, R>(s, fmt, b, e, c0, useW, L);
+        storeRecords<MODEL, kITL, R>(s.rec, c0 - b, L);
         passGather<MODEL, kITL, R>(fmt, S, (int)min(CHL, e - c0), staged, L, useW, v);
       }
       blockSum<2>(v);
@@ -491,16 +523,16 @@ __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R
       }
       delta = applyBounds(priorDelta(pt, pv, g, h, bj), &bnd);
       if (threadIdx.x == 0) {
-        bound[j] = bnd;
-        if (delta != 0.0) beta[j] = bj + delta;
+        s.bound[j] = bnd;
+        if (delta != 0.0) s.beta[j] = bj + delta;
         S.cValid = 0;
       }
       if (delta != 0.0) {
         __syncthreads();  // carry reset visible; records were written by these lanes
         for (int64_t c0 = b; c0 < e; c0 += CHL) {
           Lanes<kITL, R> L;
-          loadRecords<MODEL, kITL, R>(d, fmt, e, c0, c0 - b, rec, L);
-          passUpdate<MODEL, kITL, R>(S, (int)min(CHL, e - c0), staged, xb, oeb, denom, L, delta);
+          loadRecords<MODEL, kITL, R>(d, fmt, e, c0, c0 - b, s.rec, L);
+          passUpdate<MODEL, kITL, R>(s, S, (int)min(CHL, e - c0), staged, L, delta);
         }
         __syncthreads();
       }
@@ -520,7 +552,6 @@ __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R
     }
   }
   if (threadIdx.x == 0) {
-    FitStatus& cs = a.st[f];
     cs.tSmall += tS;
     cs.tLarge += tL;
     cs.nLarge += nL;
@@ -529,67 +560,29 @@ __device__ void cycleFit(const BatchArgs<R>& a, int f, const FitCtl& c, R* xb, R
     cs.updateNnz += cU;
     cs.updateRuns += cUR;
   }
-}
-
-// ------------------------------------------------------------ update pieces
-
-
-template <typename R>
-__device__ void updCox(const BatchArgs<R>& a, int j, int fmt, R* xb, R* oeb, double* denom,
-                       const double* kw, int useW, const int32_t* pid, int N, double delta) {
-  const DevX& d = a.d;
-  const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
-  const R rd = (R)delta;
-  for (int64_t p = b + threadIdx.x; p < e; p += blockDim.x) {
-    int pp;
-    if (!coxHead<R>(d, fmt, b, p, pid, N, &pp)) {
-      if (pp >= N) {
-        const int k = colRow(d, fmt, p, b);
-        const R nx = xb[k] + rd * colValue<R>(d, fmt, p);
-        xb[k] = nx;
-        oeb[k] = rexp(nx);
-      }
-      continue;
-    }
-    double den = denom[pp];
-    for (int64_t q = p; q < e; ++q) {
-      const int k = colRow(d, fmt, q, b);
-      const int pq = pid[k];
-      if (pq >= N) continue;
-      if (pq != pp) break;
-      const R nx = xb[k] + rd * colValue<R>(d, fmt, q);
-      xb[k] = nx;
-      const double w = useW ? kw[k] : 1.0;
-      const double oldE = useW ? w * (double)oeb[k] : (double)oeb[k];
-      const R o = rexp(nx);
-      oeb[k] = o;
-      const double newE = useW ? w * (double)o : (double)o;
-      den += newE - oldE;
-    }
-    denom[pp] = den;
-  }
+  return jEnd;
 }
 
 // ------------------------------------------------------------ whole-fit pieces
 // log likelihood of one fit (ModelSpecifics.hpp:605-650); accumulated
 // denominators for Cox come from an in-block scan
 template <int MODEL, typename R>
-__device__ double fitLogLik(const BatchArgs<R>& a, const R* xb, const R* oeb, const double* kw,
-                            int useW, const double* denom, const double* nw, const uint8_t* reset,
-                            int N) {
-  const DevX& d = a.d;
+__device__ __forceinline__ double fitLogLik(const Slot<R>& s) {
+  const DevX& d = s.d;
+  const int useW = s.ctl.useW;
+  const int N = s.N;
   double v[2] = {0.0, 0.0};
-  for (int k = threadIdx.x; k < a.K; k += blockDim.x) {
-    const double x = (double)xb[k];
+  for (int k = threadIdx.x; k < s.K; k += blockDim.x) {
+    const double x = (double)s.xb[k];
     const int yi = (int)d.y[k];
-    const double w = useW ? kw[k] : 1.0;
+    const double w = useW ? s.kw[k] : 1.0;
     double c = (MODEL == M_COX) ? (yi != 1 ? 0.0 : x) : yi * x;
     v[0] += useW ? c * w : c;
-    if (MODEL == M_LOGISTIC) v[1] += w * log(1.0 + (double)oeb[k]);
-    if (MODEL == M_POISSON) v[1] += w * (double)oeb[k];
+    if (MODEL == M_LOGISTIC) v[1] += w * log(1.0 + (double)s.oeb[k]);
+    if (MODEL == M_POISSON) v[1] += w * (double)s.oeb[k];
   }
   if (MODEL == M_CLR) {
-    for (int i = threadIdx.x; i < N; i += blockDim.x) v[1] += nw[i] * log(denom[i]);
+    for (int i = threadIdx.x; i < N; i += blockDim.x) v[1] += s.nw[i] * log(s.denom[i]);
   }
   if (MODEL == M_COX) {
     Seg<3> carry;
@@ -609,8 +602,8 @@ __device__ double fitLogLik(const BatchArgs<R>& a, const R* xb, const R* oeb, co
         x.f = 0;
         x.v[0] = x.v[1] = x.v[2] = 0.0;
         if (i < N) {
-          x.f = reset[i];
-          x.v[0] = denom[i];
+          x.f = s.reset[i];
+          x.v[0] = s.denom[i];
         }
         items[it] = x;
         agg = segCombine(agg, x);
@@ -622,7 +615,7 @@ __device__ double fitLogLik(const BatchArgs<R>& a, const R* xb, const R* oeb, co
         const int i = base + it;
         if (i >= N) break;
         run = segCombine(run, items[it]);
-        v[1] += nw[i] * log(run.v[0]);
+        v[1] += s.nw[i] * log(run.v[0]);
       }
       carry = segCombine(carry, total);
     }
@@ -632,87 +625,74 @@ __device__ double fitLogLik(const BatchArgs<R>& a, const R* xb, const R* oeb, co
 }
 
 template <typename R>
-__device__ double fitGradObjective(const BatchArgs<R>& a, const R* xb, const double* kw,
-                                   int useW) {
+__device__ __forceinline__ double fitGradObjective(const Slot<R>& s) {
+  const int useW = s.ctl.useW;
   double v[1] = {0.0};
-  for (int k = threadIdx.x; k < a.K; k += blockDim.x) {
-    const double x = (double)xb[k];
-    v[0] += useW ? x * a.d.y[k] * kw[k] : x * a.d.y[k];
+  for (int k = threadIdx.x; k < s.K; k += blockDim.x) {
+    const double x = (double)s.xb[k];
+    v[0] += useW ? x * s.d.y[k] * s.kw[k] : x * s.d.y[k];
   }
   blockSum<1>(v);
   return v[0];
 }
 
 template <int MODEL, typename R>
-__device__ double fitObjective(const BatchArgs<R>& a, int f, double fixedLL) {
-  const FitCtl c = a.ctl[f];
-  R* xb = a.xb + f * a.strideK;
-  R* oeb = a.oeb + f * a.strideK;
-  const double* kw = a.kw + f * a.strideK;
-  if (c.convergenceType == 0) return fitGradObjective<R>(a, xb, kw, c.useW);
-  if (c.convergenceType == 3) return 0.0;
-  const double ll =
-      fitLogLik<MODEL, R>(a, xb, oeb, kw, c.useW, a.denom + f * a.strideN, a.nw + f * a.strideN,
-                          a.reset + f * a.strideN, a.st[f].N) +
-      fixedLL;
-  if (c.convergenceType == 2) return ll;
+__device__ __forceinline__ double fitObjective(const Slot<R>& s) {
+  const int ct = s.ctl.convergenceType;
+  if (ct == 0) return fitGradObjective<R>(s);
+  if (ct == 3) return 0.0;
+  const double ll = fitLogLik<MODEL, R>(s) + s.fixedLL;
+  if (ct == 2) return ll;
   double v[1] = {0.0};
-  for (int j = threadIdx.x; j < a.J; j += blockDim.x)
-    v[0] += priorLogDensity(a.ptype[f * a.strideJ + j], a.pvar[f * a.strideJ + j],
-                            a.beta[f * a.strideJ + j]);
+  for (int j = threadIdx.x; j < s.J; j += blockDim.x)
+    v[0] += priorLogDensity(s.ptype[j], s.pvar[j], s.beta[j]);
   blockSum<1>(v);
   return ll + v[0];
 }
 
-// ------------------------------------------------------------------ kernels
 // Per-fit set-up: event weights, XjY, xBeta from the starting coefficients,
 // exp / denominators, bounds and the starting objective (findMode's lazy part).
 template <int MODEL, typename R>
-__global__ void k_batch_init(BatchArgs<R> a, const double* fixedLL) {
-  const int f = a.active[blockIdx.x];
-  const DevX& d = a.d;
-  const FitCtl c = a.ctl[f];
-  R* xb = a.xb + f * a.strideK;
-  R* oeb = a.oeb + f * a.strideK;
-  const double* kw = a.kw + f * a.strideK;
-  double* denom = a.denom + f * a.strideN;
-  double* nw = a.nw + f * a.strideN;
-  const int N = a.st[f].N;
-  const int32_t* pid = a.pid ? a.pid + f * a.strideK : nullptr;
-  const int32_t* pidFirst = a.pidFirst ? a.pidFirst + f * a.strideN : nullptr;
+__device__ __forceinline__ void initSlot(const Slot<R>& s, FitStatus& st) {
+  const DevX& d = s.d;
+  const int useW = s.ctl.useW;
+  const int N = s.N;
+  R* xb = s.xb;
+  R* oeb = s.oeb;
+  const double* kw = s.kw;
   // nW per group
   if (MODEL == M_CLR) {
-    for (int s = threadIdx.x; s < d.nStrata; s += blockDim.x) {
+    for (int g = threadIdx.x; g < d.nStrata; g += blockDim.x) {
       double acc = 0.0;
-      for (int k = d.stratStart[s]; k < d.stratStart[s + 1]; ++k) acc += d.y[k] * kw[k];
-      nw[s] = acc;
+      for (int k = d.stratStart[g]; k < d.stratStart[g + 1]; ++k) acc += d.y[k] * kw[k];
+      s.nw[g] = acc;
     }
   } else if (MODEL == M_COX) {
     for (int p = threadIdx.x; p < N; p += blockDim.x) {
       double acc = 0.0;
-      for (int k = pidFirst[p]; k < pidFirst[p + 1]; ++k)
-        if (pid[k] == p) acc += d.y[k] * kw[k];
-      nw[p] = acc;
+      for (int k = s.pidFirst[p]; k < s.pidFirst[p + 1]; ++k)
+        if (s.pid[k] == p) acc += d.y[k] * kw[k];
+      s.nw[p] = acc;
     }
   }
   // XjY
-  for (int j = 0; j < a.J; ++j) {
+  for (int j = 0; j < s.J; ++j) {
     const int fmt = d.fmt[j];
     const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
-    double s[1] = {0.0};
+    double t[1] = {0.0};
     for (int64_t p = b + threadIdx.x; p < e; p += blockDim.x) {
       const int k = colRow(d, fmt, p, b);
       const double x = colValue<double>(d, fmt, p);
-      s[0] += c.useW ? x * d.y[k] * kw[k] : x * d.y[k];
+      t[0] += useW ? x * d.y[k] * kw[k] : x * d.y[k];
     }
-    blockSum<1>(s);
-    if (threadIdx.x == 0) a.xjy[f * a.strideJ + j] = s[0];
+    blockSum<1>(t);
+    if (threadIdx.x == 0) s.xjy[j] = t[0];
   }
   // xBeta = sum_j beta_j X_j in column order
-  for (int k = threadIdx.x; k < a.K; k += blockDim.x) xb[k] = (R)0;
+  for (int k = threadIdx.x; k < s.K; k += blockDim.x) xb[k] = (R)0;
   __syncthreads();
-  for (int j = 0; j < a.J; ++j) {
-    const double bj = a.beta[f * a.strideJ + j];
+  for (int j = 0; j < s.J; ++j) {
+    const double bj = s.beta[j];
     if (bj == 0.0) continue;
     const int fmt = d.fmt[j];
     const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
@@ -725,197 +705,375 @@ __global__ void k_batch_init(BatchArgs<R> a, const double* fixedLL) {
   }
   // exp and denominators
   if (MODEL == M_CLR) {
-    for (int s = threadIdx.x; s < d.nStrata; s += blockDim.x) {
+    for (int g = threadIdx.x; g < d.nStrata; g += blockDim.x) {
       double acc = 0.0;
-      for (int k = d.stratStart[s]; k < d.stratStart[s + 1]; ++k) {
+      for (int k = d.stratStart[g]; k < d.stratStart[g + 1]; ++k) {
         const R o = rexp(xb[k]);
         oeb[k] = o;
         acc += (double)o;
       }
-      denom[s] = acc;
+      s.denom[g] = acc;
     }
   } else {
-    for (int k = threadIdx.x; k < a.K; k += blockDim.x) oeb[k] = rexp(xb[k]);
+    for (int k = threadIdx.x; k < s.K; k += blockDim.x) oeb[k] = rexp(xb[k]);
     if (MODEL == M_COX) {
       __syncthreads();
       for (int p = threadIdx.x; p < N; p += blockDim.x) {
         double acc = 0.0;
-        for (int k = pidFirst[p]; k < pidFirst[p + 1]; ++k) {
-          if (pid[k] != p) continue;
-          acc += c.useW ? kw[k] * (double)oeb[k] : (double)oeb[k];
+        for (int k = s.pidFirst[p]; k < s.pidFirst[p + 1]; ++k) {
+          if (s.pid[k] != p) continue;
+          acc += useW ? kw[k] * (double)oeb[k] : (double)oeb[k];
         }
-        denom[p] = acc;
+        s.denom[p] = acc;
       }
     }
   }
-  for (int j = threadIdx.x; j < a.J; j += blockDim.x) a.bound[f * a.strideJ + j] = c.initialBound;
+  for (int j = threadIdx.x; j < s.J; j += blockDim.x) s.bound[j] = s.ctl.initialBound;
   __syncthreads();
-  const double obj = fitObjective<MODEL, R>(a, f, fixedLL[f]);
+  const double obj = fitObjective<MODEL, R>(s);
   if (threadIdx.x == 0) {
-    a.st[f].lastObj = obj;
-    a.st[f].iteration = 0;
-    a.st[f].done = 0;
-    a.st[f].flag = CCDH_SUCCESS;
+    st.lastObj = obj;
+    st.iteration = 0;
+    st.jNext = 0;
+    st.done = 0;
+    st.flag = CCDH_SUCCESS;
   }
 }
 
-// Up to `sweeps` complete CCD cycles plus the convergence check after each.
-template <int MODEL, typename R, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_batch_sweep(BatchArgs<R> a, const double* fixedLL,
-                                                        int sweeps) {
-  const int f = a.active[blockIdx.x];
-  if (a.st[f].done) return;
-  const DevX& d = a.d;
-  const FitCtl c = a.ctl[f];
-  R* xb = a.xb + f * a.strideK;
-  R* oeb = a.oeb + f * a.strideK;
-  const double* kw = a.kw + f * a.strideK;
-  double* denom = a.denom + f * a.strideN;
-  const double* nw = a.nw + f * a.strideN;
-  double* beta = a.beta + f * a.strideJ;
-  double* bound = a.bound + f * a.strideJ;
-  const double* xjy = a.xjy + f * a.strideJ;
-  const double* pvar = a.pvar + f * a.strideJ;
-  const int8_t* ptype = a.ptype + f * a.strideJ;
-  const uint8_t* fixed = a.fixed + f * a.strideJ;
-  const int N = a.st[f].N;
-  const int32_t* pid = MODEL == M_COX ? a.pid + f * a.strideK : nullptr;
-  double* numer = MODEL == M_COX ? a.numer + f * a.strideN : nullptr;
-  double* numer2 = MODEL == M_COX ? a.numer2 + f * a.strideN : nullptr;
-  const uint8_t* reset = MODEL == M_COX ? a.reset + f * a.strideN : nullptr;
-  __shared__ double sDelta;
+// Up to `sweeps` complete CCD cycles plus the convergence check after each;
+// with a deadline, the slot stops at a coordinate boundary once it has passed
+// and the next launch resumes the sweep there.
+template <int MODEL, typename R, class LDS>
+__device__ __forceinline__ void sweepSlot(const Slot<R>& s, FitStatus& st, LDS& S, int sweeps, long long deadline) {
+  const FitCtl c = s.ctl;
+  __shared__ int sDone;
+  int j0 = st.jNext;
   for (int sw = 0; sw < sweeps; ++sw) {
-    if (MODEL != M_COX) {
-      Records<R> rec;
-      rec.row = a.recRow + f * a.strideRec;
-      rec.key = a.recKey ? a.recKey + f * a.strideRec : nullptr;
-      rec.flags = a.recFlags ? a.recFlags + f * a.strideRec : nullptr;
-      rec.o = a.recO + f * a.strideRec;
-      rec.xo = a.recXo + f * a.strideRec;
-      rec.den = a.recDen ? a.recDen + f * a.strideRec : nullptr;
-      cycleFit<MODEL, R>(a, f, c, xb, oeb, kw, denom, nw, beta, bound, xjy, pvar, ptype, fixed,
-                         rec);
-    } else
-    for (int j = 0; j < a.J; ++j) {
-      if (fixed[j]) continue;
-      const int64_t nj = d.colPtr[j + 1] - d.colPtr[j];
-      const int fmt = d.fmt[j];
-      double v[2] = {0.0, 0.0};
-      if (nj > 0) {
-        ghCox<R>(a, j, fmt, oeb, kw, c.useW, pid, N, denom, numer, numer2, reset, nw, v);
-        blockSum<2>(v);
-      }
-      if (threadIdx.x == 0) {
-        FitStatus& cs = a.st[f];
-        const long long runs = MODEL == M_CLR ? (long long)(d.segPtr[j + 1] - d.segPtr[j]) : 0;
-        cs.gatherNnz += nj;
-        cs.gatherRuns += runs;
-        if (MODEL == M_COX && nj > 0) cs.scanPids += N;
-        double g = nj > 0 ? v[0] - xjy[j] : 0.0;
-        double h = nj > 0 ? v[1] : 0.0;
-        if (h < 0.0) {
-          g = 0.0;
-          h = 0.0;
-        }
-        double delta = priorDelta(ptype[j], pvar[j], g, h, beta[j]);
-        delta = applyBounds(delta, &bound[j]);
-        if (delta != 0.0) {
-          beta[j] += delta;
-          cs.updateNnz += nj;
-          cs.updateRuns += runs;
-        }
-        sDelta = delta;
-      }
-      __syncthreads();
-      const double delta = sDelta;
-      if (delta != 0.0) {
-        updCox<R>(a, j, fmt, xb, oeb, denom, kw, c.useW, pid, N, delta);
-      }
-      __syncthreads();
+    const int jEnd = MODEL == M_COX ? cycleCox<R>(s, st, j0, deadline)
+                                    : cycleFit<MODEL, R>(s, st, S, j0, deadline);
+    __syncthreads();
+    if (jEnd < s.J) {
+      if (threadIdx.x == 0) st.jNext = jEnd;
+      return;
     }
+    j0 = 0;
+    if (threadIdx.x == 0) st.jNext = 0;
     // performCheckConvergence
     const long long tq0 = wall_clock64();
-    const double obj = fitObjective<MODEL, R>(a, f, fixedLL[f]);
-    if (threadIdx.x == 0) a.st[f].tObj += wall_clock64() - tq0;
-    __shared__ int sDone;
+    const double obj = fitObjective<MODEL, R>(s);
     if (threadIdx.x == 0) {
-      FitStatus& s = a.st[f];
-      s.iteration += 1;
-      s.objRows += (c.convergenceType == 0 ? a.K : (c.convergenceType == 3 ? 0 : a.K + N));
+      st.tObj += wall_clock64() - tq0;
+      st.iteration += 1;
+      st.objRows += (c.convergenceType == 0 ? s.K : (c.convergenceType == 3 ? 0 : s.K + s.N));
       double conv;
       bool ill = false;
       if (obj != obj) {
         conv = 0.0;
         ill = true;
       } else {
-        conv = fabs(obj - s.lastObj) / (fabs(obj) + 1.0);
+        conv = fabs(obj - st.lastObj) / (fabs(obj) + 1.0);
       }
-      s.lastObj = obj;
+      st.lastObj = obj;
       int done = 0;
       if (c.tolerance > 0 && conv < c.tolerance) {
-        s.flag = ill ? CCDH_ILLCONDITIONED : CCDH_SUCCESS;
+        st.flag = ill ? CCDH_ILLCONDITIONED : CCDH_SUCCESS;
         done = 1;
-      } else if (s.iteration == c.maxIterations) {
-        s.flag = CCDH_MAX_ITERATIONS;
+      } else if (st.iteration == c.maxIterations) {
+        st.flag = CCDH_MAX_ITERATIONS;
         done = 1;
       }
-      s.done = done;
+      st.done = done;
       sDone = done;
     }
     __syncthreads();
     if (sDone) return;
+    if (quantumOver(deadline)) return;
   }
 }
 
 // Final log likelihood and, when held-out weights are given, the predictive
 // log likelihood (ModelSpecifics.hpp:652-690; LR / Poisson / CLR forms).
 template <int MODEL, typename R>
-__global__ void k_batch_finish(BatchArgs<R> a, const double* fixedLL) {
-  const int f = a.active[blockIdx.x];
-  const FitCtl c = a.ctl[f];
-  R* xb = a.xb + f * a.strideK;
-  R* oeb = a.oeb + f * a.strideK;
-  const double* kw = a.kw + f * a.strideK;
-  const double* denom = a.denom + f * a.strideN;
-  const double ll = fitLogLik<MODEL, R>(a, xb, oeb, kw, c.useW, denom, a.nw + f * a.strideN,
-                                        a.reset ? a.reset + f * a.strideN : nullptr, a.st[f].N);
+__device__ __forceinline__ void finishSlot(const Slot<R>& s, FitStatus& st) {
+  const double ll = fitLogLik<MODEL, R>(s);
   double v[1] = {0.0};
-  if (a.heldout && MODEL != M_COX) {
-    const double* w2 = a.heldout + f * a.strideK;
-    for (int k = threadIdx.x; k < a.K; k += blockDim.x) {
-      const double x = (double)xb[k];
-      const double y = a.d.y[k];
-      if (MODEL == M_LOGISTIC) v[0] += y * w2[k] * (x - log(1.0 + (double)oeb[k]));
+  if (s.heldout && MODEL != M_COX) {
+    const double* w2 = s.heldout;
+    for (int k = threadIdx.x; k < s.K; k += blockDim.x) {
+      const double x = (double)s.xb[k];
+      const double y = s.d.y[k];
+      if (MODEL == M_LOGISTIC) v[0] += y * w2[k] * (x - log(1.0 + (double)s.oeb[k]));
       if (MODEL == M_POISSON) v[0] += (y * x - exp(x)) * w2[k];
-      if (MODEL == M_CLR) v[0] += y * w2[k] * (x - log(denom[a.d.strata[k]]));
+      if (MODEL == M_CLR) v[0] += y * w2[k] * (x - log(s.denom[s.d.strata[k]]));
     }
   }
   blockSum<1>(v);
   if (threadIdx.x == 0) {
-    a.st[f].logLik = ll + (MODEL == M_POISSON ? fixedLL[f] : 0.0);
-    a.st[f].predLogLik = v[0];
+    st.logLik = ll + (MODEL == M_POISSON ? s.fixedLL : 0.0);
+    st.predLogLik = v[0];
   }
 }
 
+// ------------------------------------------------------------------ kernels
+// Each kernel runs one workgroup per listed slot and dispatches on the slot's
+// model (uniform per workgroup).
+
+template <typename R>
+__global__ void k_slot_init(const Slot<R>* slots, FitStatus* st, const int32_t* active) {
+  const int f = active[blockIdx.x];
+  const Slot<R>& s = slots[f];
+  switch (s.model) {
+    case M_LOGISTIC: initSlot<M_LOGISTIC, R>(s, st[f]); break;
+    case M_POISSON: initSlot<M_POISSON, R>(s, st[f]); break;
+    case M_CLR: initSlot<M_CLR, R>(s, st[f]); break;
+    case M_COX: initSlot<M_COX, R>(s, st[f]); break;
+  }
+}
+
+template <typename R, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+    k_slot_sweep(const Slot<R>* slots, FitStatus* st, const int32_t* active, int sweeps,
+                 long long quantum) {
+  __shared__ PassLds<BLOCK * kITL> S;
+  __shared__ long long sStart;
+  const int f = active[blockIdx.x];
+  if (st[f].done) return;
+  if (threadIdx.x == 0) sStart = wall_clock64();
+  __syncthreads();
+  const long long deadline = quantum > 0 ? sStart + quantum : 0;
+  const Slot<R>& s = slots[f];
+  switch (s.model) {
+    case M_LOGISTIC: sweepSlot<M_LOGISTIC, R>(s, st[f], S, sweeps, deadline); break;
+    case M_POISSON: sweepSlot<M_POISSON, R>(s, st[f], S, sweeps, deadline); break;
+    case M_CLR: sweepSlot<M_CLR, R>(s, st[f], S, sweeps, deadline); break;
+    case M_COX: sweepSlot<M_COX, R>(s, st[f], S, sweeps, deadline); break;
+  }
+}
+
+template <typename R>
+__global__ void k_slot_finish(const Slot<R>* slots, FitStatus* st, const int32_t* active) {
+  const int f = active[blockIdx.x];
+  const Slot<R>& s = slots[f];
+  switch (s.model) {
+    case M_LOGISTIC: finishSlot<M_LOGISTIC, R>(s, st[f]); break;
+    case M_POISSON: finishSlot<M_POISSON, R>(s, st[f]); break;
+    case M_CLR: finishSlot<M_CLR, R>(s, st[f]); break;
+    case M_COX: finishSlot<M_COX, R>(s, st[f]); break;
+  }
+}
+
+__global__ void k_fill_d(double* p, int64_t n, double v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+int clampThreads(int threads) { return threads <= 256 ? 256 : 512; }
+
 }  // namespace
 
-// ================================================================ host side
+// ================================================================ slot host side
+namespace ccdamd {
+
+int slotLaunchInit(int /*model*/, bool mixed, const void* slots, FitStatus* st,
+                   const int32_t* active, int n, int threads, hipStream_t s) {
+  if (n <= 0) return 0;
+  threads = clampThreads(threads);
+  if (mixed)
+    k_slot_init<float><<<n, threads, 0, s>>>((const Slot<float>*)slots, st, active);
+  else
+    k_slot_init<double><<<n, threads, 0, s>>>((const Slot<double>*)slots, st, active);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int slotLaunchSweeps(int /*model*/, bool mixed, const void* slots, FitStatus* st,
+                     const int32_t* active, int n, int threads, int sweeps,
+                     long long quantumTicks, hipStream_t s) {
+  if (n <= 0) return 0;
+  const long long q = quantumTicks;
+  if (clampThreads(threads) == 256) {
+    if (mixed)
+      k_slot_sweep<float, 256><<<n, 256, 0, s>>>((const Slot<float>*)slots, st, active, sweeps, q);
+    else
+      k_slot_sweep<double, 256><<<n, 256, 0, s>>>((const Slot<double>*)slots, st, active, sweeps, q);
+  } else {
+    if (mixed)
+      k_slot_sweep<float, 512><<<n, 512, 0, s>>>((const Slot<float>*)slots, st, active, sweeps, q);
+    else
+      k_slot_sweep<double, 512><<<n, 512, 0, s>>>((const Slot<double>*)slots, st, active, sweeps, q);
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int slotLaunchFinish(int /*model*/, bool mixed, const void* slots, FitStatus* st,
+                     const int32_t* active, int n, int threads, hipStream_t s) {
+  if (n <= 0) return 0;
+  threads = clampThreads(threads);
+  if (mixed)
+    k_slot_finish<float><<<n, threads, 0, s>>>((const Slot<float>*)slots, st, active);
+  else
+    k_slot_finish<double><<<n, threads, 0, s>>>((const Slot<double>*)slots, st, active);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+namespace {
+constexpr size_t kNone = ~(size_t)0;
+size_t roundUp(size_t n, size_t m) { return (n + m - 1) / m * m; }
+}  // namespace
+
+SlotLayout slotLayout(int model, int K, int J, int nCap, int64_t maxLen, bool heldout, size_t rs) {
+  SlotLayout L;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += roundUp(bytes > 0 ? bytes : 1, 256);
+    return at;
+  };
+  const bool grouped = model == CCDH_CONDITIONAL_LOGISTIC || model == CCDH_COX;
+  const bool cox = model == CCDH_COX;
+  const size_t nG = grouped ? (size_t)nCap + 1 : 1;
+  const size_t nR = cox ? 1 : (size_t)std::max<int64_t>(maxLen, 1);
+  L.xb = take((size_t)K * rs);
+  L.oeb = take((size_t)K * rs);
+  L.kw = take((size_t)K * 8);
+  L.heldout = heldout && !cox ? take((size_t)K * 8) : kNone;
+  L.denom = grouped ? take(nG * 8) : kNone;
+  L.nw = grouped ? take(nG * 8) : kNone;
+  L.numer = cox ? take(nG * 8) : kNone;
+  L.numer2 = cox ? take(nG * 8) : kNone;
+  L.pid = cox ? take((size_t)K * 4) : kNone;
+  L.pidFirst = cox ? take(nG * 4) : kNone;
+  L.reset = cox ? take(nG) : kNone;
+  L.xjy = take((size_t)J * 8);
+  L.beta = take((size_t)J * 8);
+  L.bound = take((size_t)J * 8);
+  L.pvar = take((size_t)J * 8);
+  L.ptype = take((size_t)J);
+  L.fixed = take((size_t)J);
+  L.recRow = cox ? kNone : take(nR * 4);
+  L.recO = cox ? kNone : take(nR * rs);
+  L.recXo = cox ? kNone : take(nR * rs);
+  const bool clr = model == CCDH_CONDITIONAL_LOGISTIC;
+  L.recKey = clr ? take(nR * 4) : kNone;
+  L.recFlags = clr ? take(nR) : kNone;
+  L.recDen = clr ? take(nR * 8) : kNone;
+  L.bytes = off;
+  return L;
+}
+
+int slotGroups(const ccdh_data* d, const double* w) {
+  if (d->model == CCDH_CONDITIONAL_LOGISTIC) return d->nStrata;
+  if (d->model != CCDH_COX) return d->K;
+  std::vector<int32_t> pid, pidFirst;
+  std::vector<uint8_t> reset;
+  int N = 0;
+  coxPids(d, w, pid, &N, reset, pidFirst);
+  return N;
+}
+
+int64_t maxColumnLength(const ccdh_data* d) {
+  int64_t m = 1;
+  for (int j = 0; j < d->J; ++j) m = std::max(m, d->hColPtr[j + 1] - d->hColPtr[j]);
+  return m;
+}
+
+template <typename T>
+static T* at(char* block, size_t off) {
+  return off == kNone ? nullptr : (T*)(block + off);
+}
+
+template <typename R>
+int slotPrepare(const ccdh_data* d, const ccdh_fit_spec& sp, char* block, const SlotLayout& L,
+                Slot<R>* out, FitStatus* st, hipStream_t s) {
+  const int K = d->K, J = d->J;
+  Slot<R> x;
+  std::memset(&x, 0, sizeof(x));
+  x.d = d->view();
+  x.model = d->model;
+  x.K = K;
+  x.J = J;
+  x.xb = at<R>(block, L.xb);
+  x.oeb = at<R>(block, L.oeb);
+  x.kw = at<double>(block, L.kw);
+  x.heldout = sp.heldout ? at<double>(block, L.heldout) : nullptr;
+  x.denom = at<double>(block, L.denom);
+  x.nw = at<double>(block, L.nw);
+  x.numer = at<double>(block, L.numer);
+  x.numer2 = at<double>(block, L.numer2);
+  x.xjy = at<double>(block, L.xjy);
+  x.beta = at<double>(block, L.beta);
+  x.bound = at<double>(block, L.bound);
+  x.pvar = at<double>(block, L.pvar);
+  x.ptype = at<int8_t>(block, L.ptype);
+  x.fixed = at<uint8_t>(block, L.fixed);
+  x.pid = at<int32_t>(block, L.pid);
+  x.pidFirst = at<int32_t>(block, L.pidFirst);
+  x.reset = at<uint8_t>(block, L.reset);
+  x.rec.row = at<int32_t>(block, L.recRow);
+  x.rec.key = at<int32_t>(block, L.recKey);
+  x.rec.flags = at<uint8_t>(block, L.recFlags);
+  x.rec.o = at<R>(block, L.recO);
+  x.rec.xo = at<R>(block, L.recXo);
+  x.rec.den = at<double>(block, L.recDen);
+  x.ctl = FitCtl{sp.tolerance, sp.initial_bound, sp.max_iterations, sp.convergence_type,
+                 sp.weights != nullptr, 0};
+  // groups
+  int N = d->model == CCDH_CONDITIONAL_LOGISTIC ? d->nStrata : K;
+  if (d->model == CCDH_COX) {
+    std::vector<int32_t> pid, pidFirst;
+    std::vector<uint8_t> reset;
+    coxPids(d, sp.weights, pid, &N, reset, pidFirst);
+    HIP_TRY(hipMemcpyAsync(x.pid, pid.data(), K * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(x.pidFirst, pidFirst.data(), (N + 1) * sizeof(int32_t),
+                           hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(x.reset, reset.data(), N + 1, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(x.numer, 0, (N + 1) * sizeof(double), s));
+    HIP_TRY(hipMemsetAsync(x.numer2, 0, (N + 1) * sizeof(double), s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  x.N = N;
+  // fixed term of the Poisson log likelihood: sum of w * -log(y!)
+  x.fixedLL = 0.0;
+  if (d->model == CCDH_POISSON) {
+    double t = 0.0;
+    for (int k = 0; k < K; ++k) t += sp.weights ? d->hLogFact[k] * sp.weights[k] : d->hLogFact[k];
+    x.fixedLL = t;
+  }
+  std::vector<int8_t> ht(J);
+  for (int j = 0; j < J; ++j) ht[j] = (int8_t)sp.prior_type[j];
+  HIP_TRY(hipMemcpyAsync(x.beta, sp.beta0, J * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(x.pvar, sp.prior_var, J * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(x.ptype, ht.data(), J, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(x.fixed, sp.fixed, J, hipMemcpyHostToDevice, s));
+  if (sp.weights) {
+    HIP_TRY(hipMemcpyAsync(x.kw, sp.weights, K * sizeof(double), hipMemcpyHostToDevice, s));
+  } else {
+    k_fill_d<<<std::min(1024, (K + 255) / 256), 256, 0, s>>>(x.kw, K, 1.0);
+    HIP_TRY(hipGetLastError());
+  }
+  if (x.heldout)
+    HIP_TRY(hipMemcpyAsync(x.heldout, sp.heldout, K * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));  // host staging (ht) goes out of scope
+  *out = x;
+  std::memset(st, 0, sizeof(FitStatus));
+  st->N = N;
+  return 0;
+}
+
+template int slotPrepare<double>(const ccdh_data*, const ccdh_fit_spec&, char*, const SlotLayout&,
+                                 Slot<double>*, FitStatus*, hipStream_t);
+template int slotPrepare<float>(const ccdh_data*, const ccdh_fit_spec&, char*, const SlotLayout&,
+                                Slot<float>*, FitStatus*, hipStream_t);
+
+}  // namespace ccdamd
+
+// ================================================================ batch host side
 namespace {
 
 thread_local ccdh_batch_stats g_stats;
-
-size_t roundUp(size_t n, size_t m) { return (n + m - 1) / m * m; }
-
-template <typename T>
-int allocZero(T** p, size_t n, hipStream_t s, std::vector<void*>& owned) {
-  *p = nullptr;
-  if (n == 0) return 0;
-  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
-  if (e != hipSuccess) return setError(CCDH_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-  owned.push_back(*p);
-  HIP_TRY(hipMemsetAsync(*p, 0, n * sizeof(T), s));
-  return 0;
-}
 
 struct Owned {
   std::vector<void*> ptrs;
@@ -936,11 +1094,22 @@ struct Owned {
   }
 };
 
-template <int MODEL, typename R>
+template <typename T>
+int devAllocOwned(T** p, size_t n, std::vector<void*>& owned) {
+  *p = nullptr;
+  hipError_t e = hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T));
+  if (e != hipSuccess) return setError(CCDH_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  owned.push_back(*p);
+  return 0;
+}
+
+template <typename R>
 int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
              ccdh_fit_result* results, const ccdh_batch_options* opt) {
   const auto t0 = std::chrono::steady_clock::now();
-  const int K = d->K, J = d->J;
+  const int J = d->J;
+  const bool mixed = std::is_same<R, float>::value;
+  const int model = d->model;
   int threads = (opt && opt->threads) ? opt->threads : 256;
   threads = std::max(64, std::min(1024, threads / 64 * 64));
   const int sweepsPerSync = (opt && opt->sweeps_per_sync > 0) ? opt->sweeps_per_sync : 4;
@@ -950,149 +1119,45 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
   HIP_TRY(hipStreamCreateWithFlags(&o.stream, hipStreamNonBlocking));
   hipStream_t s = o.stream;
 
-  // groups per fit
-  std::vector<int> Nf(F, MODEL == M_CLR ? d->nStrata : K);
-  std::vector<std::vector<int32_t>> pids, pidFirsts;
-  std::vector<std::vector<uint8_t>> resets;
-  if (MODEL == M_COX) {
-    pids.resize(F);
-    pidFirsts.resize(F);
-    resets.resize(F);
-    for (int f = 0; f < F; ++f)
-      coxPids(d, specs[f].weights, pids[f], &Nf[f], resets[f], pidFirsts[f]);
-  }
+  // one block per fit, all with the layout of the largest group count
   int maxN = 1;
-  for (int f = 0; f < F; ++f) maxN = std::max(maxN, Nf[f]);
-  const size_t sK = roundUp(K, 64), sJ = roundUp(J, 64);
-  const size_t sN = (MODEL == M_CLR || MODEL == M_COX) ? roundUp(maxN + 1, 64) : 1;
   bool anyHeld = false;
-  for (int f = 0; f < F; ++f) anyHeld |= specs[f].heldout != nullptr;
-
-  BatchArgs<R> a;
-  std::memset(&a, 0, sizeof(a));
-  a.d = d->view();
-  a.model = MODEL;
-  a.K = K;
-  a.J = J;
-  a.nFits = F;
-  a.strideK = sK;
-  a.strideN = sN;
-  a.strideJ = sJ;
-  int rc = 0;
-  auto& ow = o.ptrs;
-  rc = rc ? rc : allocZero(&a.xb, F * sK, s, ow);
-  rc = rc ? rc : allocZero(&a.oeb, F * sK, s, ow);
-  rc = rc ? rc : allocZero(&a.kw, F * sK, s, ow);
-  if (anyHeld && MODEL != M_COX) rc = rc ? rc : allocZero(&a.heldout, F * sK, s, ow);
-  if (MODEL == M_CLR || MODEL == M_COX) {
-    rc = rc ? rc : allocZero(&a.denom, F * sN, s, ow);
-    rc = rc ? rc : allocZero(&a.nw, F * sN, s, ow);
+  for (int f = 0; f < F; ++f) {
+    maxN = std::max(maxN, slotGroups(d, specs[f].weights));
+    anyHeld |= specs[f].heldout != nullptr;
   }
-  if (MODEL == M_COX) {
-    rc = rc ? rc : allocZero(&a.numer, F * sN, s, ow);
-    rc = rc ? rc : allocZero(&a.numer2, F * sN, s, ow);
-    rc = rc ? rc : allocZero(&a.pid, F * sK, s, ow);
-    rc = rc ? rc : allocZero(&a.pidFirst, F * sN, s, ow);
-    rc = rc ? rc : allocZero(&a.reset, F * sN, s, ow);
-  }
-  if (MODEL != M_COX) {
-    int64_t maxLen = 1;
-    for (int j = 0; j < J; ++j) maxLen = std::max(maxLen, d->hColPtr[j + 1] - d->hColPtr[j]);
-    const size_t sR = roundUp((size_t)maxLen, 64);
-    a.strideRec = sR;
-    rc = rc ? rc : allocZero(&a.recRow, F * sR, s, ow);
-    rc = rc ? rc : allocZero(&a.recO, F * sR, s, ow);
-    rc = rc ? rc : allocZero(&a.recXo, F * sR, s, ow);
-    if (MODEL == M_CLR) {
-      rc = rc ? rc : allocZero(&a.recKey, F * sR, s, ow);
-      rc = rc ? rc : allocZero(&a.recFlags, F * sR, s, ow);
-      rc = rc ? rc : allocZero(&a.recDen, F * sR, s, ow);
-    }
-  }
-  rc = rc ? rc : allocZero(&a.xjy, F * sJ, s, ow);
-  rc = rc ? rc : allocZero(&a.beta, F * sJ, s, ow);
-  rc = rc ? rc : allocZero(&a.bound, F * sJ, s, ow);
-  rc = rc ? rc : allocZero(&a.pvar, F * sJ, s, ow);
-  rc = rc ? rc : allocZero(&a.ptype, F * sJ, s, ow);
-  rc = rc ? rc : allocZero(&a.fixed, F * sJ, s, ow);
-  FitCtl* dctl = nullptr;
+  const SlotLayout L = slotLayout(model, d->K, J, maxN, maxColumnLength(d), anyHeld, sizeof(R));
+  char* arena = nullptr;
+  Slot<R>* dslots = nullptr;
   FitStatus* dst = nullptr;
   int32_t* dactive = nullptr;
-  double* dfixedLL = nullptr;
-  rc = rc ? rc : allocZero(&dctl, F, s, ow);
-  rc = rc ? rc : allocZero(&dst, F, s, ow);
-  rc = rc ? rc : allocZero(&dactive, F, s, ow);
-  rc = rc ? rc : allocZero(&dfixedLL, F, s, ow);
+  int rc = 0;
+  rc = rc ? rc : devAllocOwned(&arena, (size_t)F * L.bytes, o.ptrs);
+  rc = rc ? rc : devAllocOwned(&dslots, F, o.ptrs);
+  rc = rc ? rc : devAllocOwned(&dst, F, o.ptrs);
+  rc = rc ? rc : devAllocOwned(&dactive, F, o.ptrs);
   if (rc) return rc;
-  a.ctl = dctl;
-  a.st = dst;
-  a.active = dactive;
   HIP_TRY(hipHostMalloc((void**)&o.hst, F * sizeof(FitStatus), hipHostMallocDefault));
 
-  // per-fit host inputs
-  std::vector<FitCtl> ctl(F);
+  std::vector<Slot<R>> slots(F);
   std::vector<FitStatus> st(F);
-  std::vector<double> fixedLL(F, 0.0), hb(sJ), hv(sJ);
-  std::vector<int8_t> ht(sJ);
-  std::vector<uint8_t> hf(sJ);
   std::vector<int32_t> active(F);
   std::vector<int> boundCount(F, 1);
-  std::vector<double> initialBound(F);
   for (int f = 0; f < F; ++f) {
-    const ccdh_fit_spec& sp = specs[f];
-    ctl[f] = FitCtl{sp.tolerance, sp.initial_bound, sp.max_iterations, sp.convergence_type,
-                    sp.weights != nullptr, 0};
-    initialBound[f] = sp.initial_bound;
-    std::memset(&st[f], 0, sizeof(FitStatus));
-    st[f].N = Nf[f];
+    rc = slotPrepare<R>(d, specs[f], arena + (size_t)f * L.bytes, L, &slots[f], &st[f], s);
+    if (rc) return rc;
     active[f] = f;
-    for (int j = 0; j < J; ++j) {
-      hb[j] = sp.beta0[j];
-      hv[j] = sp.prior_var[j];
-      ht[j] = (int8_t)sp.prior_type[j];
-      hf[j] = sp.fixed[j];
-    }
-    HIP_TRY(hipMemcpyAsync(a.beta + f * sJ, hb.data(), J * sizeof(double), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(a.pvar + f * sJ, hv.data(), J * sizeof(double), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(a.ptype + f * sJ, ht.data(), J, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(a.fixed + f * sJ, hf.data(), J, hipMemcpyHostToDevice, s));
-    if (sp.weights) {
-      HIP_TRY(hipMemcpyAsync(a.kw + f * sK, sp.weights, K * sizeof(double), hipMemcpyHostToDevice, s));
-    } else {
-      std::vector<double> ones(K, 1.0);
-      HIP_TRY(hipMemcpyAsync(a.kw + f * sK, ones.data(), K * sizeof(double), hipMemcpyHostToDevice, s));
-      HIP_TRY(hipStreamSynchronize(s));
-    }
-    if (sp.heldout && a.heldout)
-      HIP_TRY(hipMemcpyAsync(a.heldout + f * sK, sp.heldout, K * sizeof(double), hipMemcpyHostToDevice, s));
-    if (MODEL == M_POISSON) {
-      double t = 0.0;
-      for (int k = 0; k < K; ++k)
-        t += sp.weights ? d->hLogFact[k] * sp.weights[k] : d->hLogFact[k];
-      fixedLL[f] = t;
-    }
-    if (MODEL == M_COX) {
-      HIP_TRY(hipMemcpyAsync(a.pid + f * sK, pids[f].data(), K * sizeof(int32_t), hipMemcpyHostToDevice, s));
-      HIP_TRY(hipMemcpyAsync(a.pidFirst + f * sN, pidFirsts[f].data(), (Nf[f] + 1) * sizeof(int32_t), hipMemcpyHostToDevice, s));
-      HIP_TRY(hipMemcpyAsync(a.reset + f * sN, resets[f].data(), Nf[f] + 1, hipMemcpyHostToDevice, s));
-    }
-    HIP_TRY(hipStreamSynchronize(s));  // host vectors are reused per fit
   }
-  HIP_TRY(hipMemcpyAsync(dctl, ctl.data(), F * sizeof(FitCtl), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dslots, slots.data(), F * sizeof(Slot<R>), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(dst, st.data(), F * sizeof(FitStatus), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(dactive, active.data(), F * sizeof(int32_t), hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(dfixedLL, fixedLL.data(), F * sizeof(double), hipMemcpyHostToDevice, s));
-
-  k_batch_init<MODEL, R><<<F, threads, 0, s>>>(a, dfixedLL);
-  HIP_TRY(hipGetLastError());
+  rc = slotLaunchInit(model, mixed, dslots, dst, dactive, F, threads, s);
+  if (rc) return rc;
 
   // sweep launches, optionally replayed from a captured graph
   auto enqueueSweeps = [&]() -> int {
-    if (threads <= 256)
-      k_batch_sweep<MODEL, R, 256><<<F, 256, 0, s>>>(a, dfixedLL, sweepsPerSync);
-    else
-      k_batch_sweep<MODEL, R, 512><<<F, 512, 0, s>>>(a, dfixedLL, sweepsPerSync);
-    HIP_TRY(hipGetLastError());
+    int r = slotLaunchSweeps(model, mixed, dslots, dst, dactive, F, threads, sweepsPerSync, 0, s);
+    if (r) return r;
     HIP_TRY(hipMemcpyAsync(o.hst, dst, F * sizeof(FitStatus), hipMemcpyDeviceToHost, s));
     return 0;
   };
@@ -1147,20 +1212,18 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
       // CyclicCoordinateDescent::update: ill-conditioned -> bound / 10, cold restart
       if (h.flag == CCDH_ILLCONDITIONED && boundCount[f] < specs[f].max_bound_count) {
         boundCount[f]++;
-        initialBound[f] /= 10.0;
-        ctl[f].initialBound = initialBound[f];
-        for (int j = 0; j < J; ++j)
-          hb[j] = (d->hasOffset && j == 0) ? specs[f].beta0[0] : 0.0;
-        HIP_TRY(hipMemcpyAsync(a.beta + f * sJ, hb.data(), J * sizeof(double), hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(dctl + f, &ctl[f], sizeof(FitCtl), hipMemcpyHostToDevice, s));
-        HIP_TRY(hipStreamSynchronize(s));
+        slots[f].ctl.initialBound /= 10.0;
+        std::vector<double> hb(J, 0.0);
+        if (d->hasOffset) hb[0] = specs[f].beta0[0];
+        HIP_TRY(hipMemcpyAsync(slots[f].beta, hb.data(), J * sizeof(double), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(dslots + f, &slots[f], sizeof(Slot<R>), hipMemcpyHostToDevice, s));
         doneAt[f] = 0.0;
         lastIter[f] = 0;
         retry = true;
         active[0] = f;
         HIP_TRY(hipMemcpyAsync(dactive, active.data(), sizeof(int32_t), hipMemcpyHostToDevice, s));
-        k_batch_init<MODEL, R><<<1, threads, 0, s>>>(a, dfixedLL);
-        HIP_TRY(hipGetLastError());
+        rc = slotLaunchInit(model, mixed, dslots, dst, dactive, 1, threads, s);
+        if (rc) return rc;
         HIP_TRY(hipStreamSynchronize(s));
       }
     }
@@ -1172,12 +1235,12 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
     if (allDone) break;
     if (++guard > 100000000) return setError(CCDH_E_STATE, "batch did not terminate");
   }
-  k_batch_finish<MODEL, R><<<F, threads, 0, s>>>(a, dfixedLL);
-  HIP_TRY(hipGetLastError());
+  rc = slotLaunchFinish(model, mixed, dslots, dst, dactive, F, threads, s);
+  if (rc) return rc;
   std::vector<FitStatus> fin(F);
   HIP_TRY(hipMemcpyAsync(fin.data(), dst, F * sizeof(FitStatus), hipMemcpyDeviceToHost, s));
   for (int f = 0; f < F; ++f)
-    HIP_TRY(hipMemcpyAsync(betasOut + (size_t)f * J, a.beta + f * sJ, J * sizeof(double),
+    HIP_TRY(hipMemcpyAsync(betasOut + (size_t)f * J, slots[f].beta, J * sizeof(double),
                            hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   const double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1210,22 +1273,11 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
   }
   // Cox held-out scoring regroups the risk sets under the held-out weights:
   // run it on the per-coordinate engine with the batch's coefficients.
-  if (MODEL == M_COX && anyHeld) {
+  if (model == CCDH_COX && anyHeld) {
     for (int f = 0; f < F; ++f) {
       if (!specs[f].heldout) continue;
-      ccdh_fit* fh = nullptr;
-      rc = fitCreate(d, &fh);
-      if (rc) return rc;
-      const int useW = specs[f].weights != nullptr;
-      rc = setWeights(fh, specs[f].weights, useW);
-      if (!rc) {
-        rc = zeroXBeta(fh);
-        for (int j = 0; !rc && j < J; ++j) rc = axpyXBeta(fh, betasOut[(size_t)f * J + j], j);
-      }
-      rc = rc ? rc : remainingStats(fh, useW);
       double v = 0.0;
-      rc = rc ? rc : predLogLik(fh, specs[f].heldout, &v);
-      fitDestroy(fh);
+      rc = coxHeldOut(d, specs[f].weights, specs[f].heldout, betasOut + (size_t)f * J, &v);
       if (rc) return rc;
       results[f].pred_log_likelihood = v;
     }
@@ -1234,6 +1286,40 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
 }
 
 }  // namespace
+
+namespace ccdamd {
+
+// Predictive log likelihood of a Cox fit on held-out weights, by the
+// per-coordinate engine (the risk sets are regrouped under the held-out weights).
+int coxHeldOut(ccdh_data* d, const double* weights, const double* heldout, const double* beta,
+               double* out) {
+  ccdh_fit* fh = nullptr;
+  int rc = fitCreate(d, &fh);
+  if (rc) return rc;
+  const int useW = weights != nullptr;
+  rc = setWeights(fh, weights, useW);
+  if (!rc) {
+    rc = zeroXBeta(fh);
+    for (int j = 0; !rc && j < d->J; ++j) rc = axpyXBeta(fh, beta[j], j);
+  }
+  rc = rc ? rc : remainingStats(fh, useW);
+  rc = rc ? rc : predLogLik(fh, heldout, out);
+  fitDestroy(fh);
+  return rc;
+}
+
+int validateSpec(const ccdh_data* d, const ccdh_fit_spec& sp) {
+  if (!sp.prior_type || !sp.prior_var || !sp.fixed || !sp.beta0)
+    return setError(CCDH_E_INVALID, "fit spec arrays required");
+  if (sp.convergence_type < 0 || sp.convergence_type > 3)
+    return setError(CCDH_E_INVALID, "unsupported convergence type");
+  for (int j = 0; j < d->J; ++j)
+    if (sp.prior_type[j] < 0 || sp.prior_type[j] > 2)
+      return setError(CCDH_E_INVALID, "unsupported prior type");
+  return 0;
+}
+
+}  // namespace ccdamd
 
 extern "C" int ccdh_batch_last_stats(ccdh_batch_stats* out) {
   if (!out) return setError(CCDH_E_INVALID, "null argument");
@@ -1247,30 +1333,14 @@ extern "C" int ccdh_batch_fit(ccdh_data_t data, int n_fits, const ccdh_fit_spec*
   if (!data || !specs || !betas_out || !results || n_fits <= 0)
     return setError(CCDH_E_INVALID, "null argument");
   for (int f = 0; f < n_fits; ++f) {
-    const ccdh_fit_spec& sp = specs[f];
-    if (!sp.prior_type || !sp.prior_var || !sp.fixed || !sp.beta0)
-      return setError(CCDH_E_INVALID, "fit spec arrays required");
-    if (sp.convergence_type < 0 || sp.convergence_type > 3)
-      return setError(CCDH_E_INVALID, "unsupported convergence type");
+    const int rc = validateSpec(data, specs[f]);
+    if (rc) return rc;
   }
-  const bool mixed = data->precision == CCDH_MIXED;
   try {
-    switch (data->model) {
-      case CCDH_LOGISTIC:
-        return mixed ? runBatch<M_LOGISTIC, float>(data, n_fits, specs, betas_out, results, options)
-                     : runBatch<M_LOGISTIC, double>(data, n_fits, specs, betas_out, results, options);
-      case CCDH_POISSON:
-        return mixed ? runBatch<M_POISSON, float>(data, n_fits, specs, betas_out, results, options)
-                     : runBatch<M_POISSON, double>(data, n_fits, specs, betas_out, results, options);
-      case CCDH_CONDITIONAL_LOGISTIC:
-        return mixed ? runBatch<M_CLR, float>(data, n_fits, specs, betas_out, results, options)
-                     : runBatch<M_CLR, double>(data, n_fits, specs, betas_out, results, options);
-      case CCDH_COX:
-        return mixed ? runBatch<M_COX, float>(data, n_fits, specs, betas_out, results, options)
-                     : runBatch<M_COX, double>(data, n_fits, specs, betas_out, results, options);
-    }
+    return data->precision == CCDH_MIXED
+               ? runBatch<float>(data, n_fits, specs, betas_out, results, options)
+               : runBatch<double>(data, n_fits, specs, betas_out, results, options);
   } catch (const std::exception& e) {
     return setError(CCDH_E_DEVICE, e.what());
   }
-  return setError(CCDH_E_INVALID, "unsupported model");
 }

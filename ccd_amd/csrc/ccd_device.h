@@ -37,7 +37,6 @@ struct DevX {
   const int32_t* segKey;    // [nseg] stratum of the run
   const int32_t* stratStart;// [nStrata+1] first row of each stratum
   const int32_t* nzKey;     // [nnz] stratum of each nonzero's row (grouped models)
-  const int8_t* colMode;    // [J] 1: column walked by stratum runs (runs average >= 1.5 rows)
 };
 
 template <typename R>

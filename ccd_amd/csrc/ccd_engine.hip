@@ -538,31 +538,55 @@ DevX ccdh_data::view() const {
   x.segKey = segKey;
   x.stratStart = stratStart;
   x.nzKey = nzKey;
-  x.colMode = colMode;
   return x;
 }
 
 namespace ccdamd {
 
+// Device arrays come from hipMalloc, or are carved from `place` (a block the
+// caller owns, e.g. a served request's arena block).
 template <typename T>
-static int devAlloc(T** p, size_t n, int64_t* acc) {
+static int devAlloc(T** p, size_t n, int64_t* acc, Bump* place = nullptr) {
   *p = nullptr;
   if (n == 0) n = 1;
-  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
-  if (e != hipSuccess) return setError(CCDH_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  if (place) {
+    *p = (T*)place->take(n * sizeof(T));
+    if (!*p) return setError(CCDH_E_NOMEM, "data does not fit its device block");
+  } else {
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess)
+      return setError(CCDH_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
   if (acc) *acc += (int64_t)(n * sizeof(T));
   return 0;
 }
 
 template <typename T>
-static int devUpload(T** p, const T* h, size_t n, int64_t* acc) {
-  int rc = devAlloc(p, n, acc);
+static int devUpload(T** p, const T* h, size_t n, int64_t* acc, Bump* place) {
+  int rc = devAlloc(p, n, acc, place);
   if (rc) return rc;
   if (n && h) HIP_TRY(hipMemcpy(*p, h, n * sizeof(T), hipMemcpyHostToDevice));
   return 0;
 }
 
-int dataCreate(const ccdh_data_desc* desc, ccdh_data** out) {
+// Device bytes dataCreate will carve for this description when placed in a
+// block (an upper bound: stratum runs are counted as one per nonzero).
+int64_t dataFootprint(const ccdh_data_desc* desc) {
+  const int64_t K = desc->K, J = desc->J, nnz = desc->col_ptr[J];
+  bool needVals = false;
+  for (int j = 0; j < J; ++j) needVals |= desc->fmt[j] == CCDH_SPARSE || desc->fmt[j] == CCDH_DENSE;
+  int64_t nStrata = 1;
+  if (desc->strata)
+    for (int64_t k = 0; k < K; ++k) nStrata = std::max<int64_t>(nStrata, desc->strata[k] + 1);
+  auto r = [](int64_t b) { return (int64_t)Arena::round((size_t)std::max<int64_t>(b, 1)); };
+  int64_t t = r((J + 1) * 8) + r(nnz * 4) + r(J) + r(K * 8) + r(K * 4);
+  if (needVals) t += r(nnz * 8) + (desc->precision == CCDH_MIXED ? r(nnz * 4) : 0);
+  if (desc->model == CCDH_CONDITIONAL_LOGISTIC)
+    t += r(nnz * 4) + r((J + 1) * 8) + 2 * r(nnz * 4) + r((nStrata + 1) * 4);
+  return t;
+}
+
+int dataCreate(const ccdh_data_desc* desc, ccdh_data** out, Bump* place) {
   if (!desc || !out) return setError(CCDH_E_INVALID, "null argument");
   const int model = desc->model;
   if (model != CCDH_POISSON && model != CCDH_LOGISTIC && model != CCDH_CONDITIONAL_LOGISTIC &&
@@ -599,6 +623,7 @@ int dataCreate(const ccdh_data_desc* desc, ccdh_data** out) {
     return setError(CCDH_E_INVALID, "offset must be a DENSE column 0");
 
   auto* d = new ccdh_data();
+  d->placed = place != nullptr;
   d->model = model;
   d->precision = desc->precision;
   d->K = K;
@@ -638,20 +663,20 @@ int dataCreate(const ccdh_data_desc* desc, ccdh_data** out) {
 
   int64_t& acc = d->devBytes;
   int rc = 0;
-  rc = rc ? rc : devUpload(&d->colPtr, desc->col_ptr, J + 1, &acc);
+  rc = rc ? rc : devUpload(&d->colPtr, desc->col_ptr, J + 1, &acc, place);
   std::vector<int32_t> rowsFull;
   const int32_t* rowsSrc = desc->rows;
-  rc = rc ? rc : devUpload(&d->rows, rowsSrc, (size_t)nnz, &acc);
+  rc = rc ? rc : devUpload(&d->rows, rowsSrc, (size_t)nnz, &acc, place);
   if (!rc && needVals) {
-    rc = devUpload(&d->vals, desc->vals, (size_t)nnz, &acc);
+    rc = devUpload(&d->vals, desc->vals, (size_t)nnz, &acc, place);
     if (!rc && d->precision == CCDH_MIXED) {
       std::vector<float> vf(desc->vals, desc->vals + nnz);
-      rc = devUpload(&d->valsF, vf.data(), (size_t)nnz, &acc);
+      rc = devUpload(&d->valsF, vf.data(), (size_t)nnz, &acc, place);
     }
   }
-  rc = rc ? rc : devUpload(&d->fmt, desc->fmt, J, &acc);
-  rc = rc ? rc : devUpload(&d->y, desc->y, K, &acc);
-  rc = rc ? rc : devUpload(&d->strata, d->hStrata.data(), K, &acc);
+  rc = rc ? rc : devUpload(&d->fmt, desc->fmt, J, &acc, place);
+  rc = rc ? rc : devUpload(&d->y, desc->y, K, &acc, place);
+  rc = rc ? rc : devUpload(&d->strata, d->hStrata.data(), K, &acc, place);
   if (!rc && model == CCDH_CONDITIONAL_LOGISTIC) {
     // runs of equal stratum inside every column (rows are sorted by stratum)
     std::vector<int64_t> segPtr(J + 1, 0);
@@ -684,7 +709,7 @@ int dataCreate(const ccdh_data_desc* desc, ccdh_data** out) {
           key[p] = d->hStrata[k];
         }
       }
-      rc = devUpload(&d->nzKey, key.data(), key.size(), &acc);
+      rc = devUpload(&d->nzKey, key.data(), key.size(), &acc, place);
       if (rc) {
         dataDestroy(d);
         return rc;
@@ -694,17 +719,10 @@ int dataCreate(const ccdh_data_desc* desc, ccdh_data** out) {
     for (int k = 0; k < K; ++k) ss[d->hStrata[k] + 1]++;
     for (int s = 0; s < nStrata; ++s) ss[s + 1] += ss[s];
     d->hSegPtr = segPtr;
-    std::vector<int8_t> mode(J, 0);
-    for (int j = 0; j < J; ++j) {
-      const int64_t n = desc->col_ptr[j + 1] - desc->col_ptr[j];
-      const int64_t r = segPtr[j + 1] - segPtr[j];
-      mode[j] = (r > 0 && 2 * n >= 3 * r) ? 1 : 0;
-    }
-    rc = devUpload(&d->colMode, mode.data(), J, &acc);
-    rc = rc ? rc : devUpload(&d->segPtr, segPtr.data(), J + 1, &acc);
-    rc = rc ? rc : devUpload(&d->segOff, segOff.data(), segOff.size(), &acc);
-    rc = rc ? rc : devUpload(&d->segKey, segKey.data(), segKey.size(), &acc);
-    rc = rc ? rc : devUpload(&d->stratStart, ss.data(), ss.size(), &acc);
+    rc = devUpload(&d->segPtr, segPtr.data(), J + 1, &acc, place);
+    rc = rc ? rc : devUpload(&d->segOff, segOff.data(), segOff.size(), &acc, place);
+    rc = rc ? rc : devUpload(&d->segKey, segKey.data(), segKey.size(), &acc, place);
+    rc = rc ? rc : devUpload(&d->stratStart, ss.data(), ss.size(), &acc, place);
     d->nSeg = (int64_t)segOff.size();
   }
   if (rc) {
@@ -718,9 +736,10 @@ int dataCreate(const ccdh_data_desc* desc, ccdh_data** out) {
 void dataDestroy(ccdh_data* d) {
   if (!d) return;
   void* ptrs[] = {d->colPtr, d->rows, d->vals, d->valsF, d->fmt, d->y, d->strata,
-                  d->segPtr, d->segOff, d->segKey, d->stratStart, d->nzKey, d->colMode};
-  for (void* p : ptrs)
-    if (p) (void)hipFree(p);
+                  d->segPtr, d->segOff, d->segKey, d->stratStart, d->nzKey};
+  if (!d->placed)
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
   delete d;
 }
 

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/ccdhip.h"
+#include "ccd_arena.h"
 #include "ccd_device.h"
 
 struct ccdh_data {
@@ -27,7 +28,7 @@ struct ccdh_data {
   int32_t* segKey = nullptr;
   int32_t* stratStart = nullptr;
   int32_t* nzKey = nullptr;
-  int8_t* colMode = nullptr;
+  bool placed = false;  // device arrays carved from a caller's block (not freed here)
   // host copies of the small per-column / per-row vectors
   std::vector<int64_t> hColPtr, hSegPtr;
   std::vector<int8_t> hFmt;
@@ -72,7 +73,8 @@ namespace ccdamd {
 
 int setError(int code, const std::string& msg);
 
-int dataCreate(const ccdh_data_desc* desc, ccdh_data** out);
+int dataCreate(const ccdh_data_desc* desc, ccdh_data** out, Bump* place = nullptr);
+int64_t dataFootprint(const ccdh_data_desc* desc);
 void dataDestroy(ccdh_data* d);
 int fitCreate(ccdh_data* d, ccdh_fit** out);
 void fitDestroy(ccdh_fit* f);
@@ -95,5 +97,10 @@ int logLik(ccdh_fit* f, int useCV, double* out);
 int gradientObjective(ccdh_fit* f, int useCV, double* out);
 int predLogLik(ccdh_fit* f, const double* w, double* out);
 int getXBeta(ccdh_fit* f, double* out);
+
+// batched / served engine helpers
+int validateSpec(const ccdh_data* d, const ccdh_fit_spec& sp);
+int coxHeldOut(ccdh_data* d, const double* weights, const double* heldout, const double* beta,
+               double* out);
 
 }  // namespace ccdamd

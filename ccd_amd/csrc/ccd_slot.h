@@ -1,0 +1,125 @@
+// ccd_slot.h -- fit slots of the device-resident engine.
+//
+// A slot is one fit in flight: a view of its (resident, read-only) data and
+// pointers to its own per-row, per-group and per-coordinate state.  A sweep
+// launch runs one workgroup per active slot, so fits over one data set
+// (cross-validation folds x prior variances) and fits over different data
+// sets (served requests of different N and P) share the same kernels.  All
+// of a slot's state lives in one block of device memory, laid out by
+// SlotLayout; the block is carved from a batch allocation or from the serving
+// arena.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/ccdhip.h"
+#include "ccd_device.h"
+
+struct ccdh_data;
+
+namespace ccdamd {
+
+constexpr int M_ANY = -1;  // sweep kernel dispatching on each slot's model
+
+struct FitCtl {
+  double tolerance;
+  double initialBound;
+  int maxIterations;
+  int convergenceType;
+  int useW;
+  int pad;
+};
+
+struct FitStatus {
+  double lastObj;
+  double logLik;
+  double predLogLik;
+  int iteration;
+  int flag;
+  int done;
+  int N;      // groups of this fit (Cox: tie groups under its weights)
+  int jNext;  // coordinate the current sweep resumes at (a launch's time quantum ran out)
+  int pad;
+  // work counters (thread 0), read by the host for the bandwidth figure
+  long long gatherNnz, gatherRuns, updateNnz, updateRuns, scanPids, objRows;
+  // wall-clock ticks (thread 0) spent per phase: single-pass columns, long columns, objective
+  long long tSmall, tLarge, tObj, nLarge;
+};
+
+// gather-pass records of long columns, read back by the update pass
+template <typename R>
+struct Records {
+  int32_t* row;
+  int32_t* key;
+  uint8_t* flags;  // bit 0 own, bit 1 start
+  R* o;
+  R* xo;
+  double* den;
+};
+
+template <typename R>
+struct Slot {
+  DevX d;
+  int model, K, J, N;
+  R* xb;
+  R* oeb;
+  double* kw;
+  double* heldout;  // nullptr when the fit has no held-out weights
+  double* denom;
+  double* nw;
+  double* numer;
+  double* numer2;
+  double* xjy;
+  double* beta;
+  double* bound;
+  double* pvar;
+  int8_t* ptype;
+  uint8_t* fixed;
+  int32_t* pid;
+  int32_t* pidFirst;
+  uint8_t* reset;
+  Records<R> rec;
+  FitCtl ctl;
+  double fixedLL;
+};
+
+// Byte offsets of every array of one slot inside its block (256-byte aligned).
+struct SlotLayout {
+  size_t xb, oeb, kw, heldout, denom, nw, numer, numer2, xjy, beta, bound, pvar, ptype, fixed,
+      pid, pidFirst, reset, recRow, recKey, recFlags, recO, recXo, recDen;
+  size_t bytes;
+};
+
+// Layout for a fit of `model` over K rows, J coordinates, at most nCap groups
+// and a longest column of maxLen nonzeros; rs = bytes of a per-row real.
+SlotLayout slotLayout(int model, int K, int J, int nCap, int64_t maxLen, bool heldout, size_t rs);
+
+// Host-side preparation of one slot: groups (Cox pids under the fit's weights),
+// the fixed log-likelihood term, and the uploads of coefficients, priors,
+// weights and groupings into the slot's block.  `out` receives the device
+// pointers; `st` its initial status.  Returns a ccdh status code.
+template <typename R>
+int slotPrepare(const ccdh_data* d, const ccdh_fit_spec& sp, char* block, const SlotLayout& L,
+                Slot<R>* out, FitStatus* st, hipStream_t s);
+
+// Groups a fit of this data has under weights w (K for independent rows,
+// strata for CLR, tie groups for Cox).
+int slotGroups(const ccdh_data* d, const double* w);
+int64_t maxColumnLength(const ccdh_data* d);
+
+// Kernel launches over `n` slots listed in `active` (device array of slot
+// indices into `slots`).  model is a CCDH model code or M_ANY.
+int slotLaunchInit(int model, bool mixed, const void* slots, FitStatus* st,
+                   const int32_t* active, int n, int threads, hipStream_t s);
+// quantumTicks > 0: a slot stops at the first coordinate boundary after that
+// many device clock ticks (100 MHz) and resumes there in the next launch.
+int slotLaunchSweeps(int model, bool mixed, const void* slots, FitStatus* st,
+                     const int32_t* active, int n, int threads, int sweeps,
+                     long long quantumTicks, hipStream_t s);
+int slotLaunchFinish(int model, bool mixed, const void* slots, FitStatus* st,
+                     const int32_t* active, int n, int threads, hipStream_t s);
+
+}  // namespace ccdamd

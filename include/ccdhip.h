@@ -216,6 +216,62 @@ int ccdh_cv_fold_weights(const int32_t* ids, int K, int by_pid, int n_fold, long
 /* GridSearchCrossValidationDriver::computeGridPoint (GridSearchCrossValidationDriver.cpp:38-48) */
 double ccdh_cv_grid_point(double lower, double upper, int size, int step);
 
+/* ---- serving: continuous batching of fit requests ----
+ * No reference counterpart (Cyclops fits one model per call; its parallel
+ * drivers run whole fits on std::threads, AbstractCrossValidationDriver.cpp:65-74).
+ * A server owns one device memory arena (its HBM budget) and two host threads:
+ * admission (FCFS; a request is admitted when its data + fit state fit in the
+ * arena and a slot is free, then uploaded and set up on its own stream) and
+ * scheduling (one sweep launch over every fit in flight, each fit running for a
+ * time quantum and resuming at the next launch; finished fits leave, admitted
+ * fits join at every launch boundary).  A fit's arithmetic does not depend on
+ * the other fits in flight or on where its sweeps were paused. */
+typedef struct {
+  int precision;            /* CCDH_FP64 or CCDH_MIXED, for every request */
+  int max_slots;            /* fits in flight at once (0: 1024) */
+  int threads;              /* workgroup size per fit (0: 256) */
+  int _pad;
+  double quantum_ms;        /* device time per fit per launch (0: 20 ms) */
+  int64_t hbm_budget_bytes; /* arena size (0: half of the free device memory) */
+} ccdh_server_options;
+
+typedef struct {
+  double latency;           /* submit -> result ready, seconds (host clock) */
+  double queued;            /* submit -> admitted (memory and slot granted) */
+  double prepare;           /* admission: upload and set-up */
+  int64_t footprint;        /* device bytes held while in flight */
+  int64_t launches;         /* sweep launches the fit took part in */
+} ccdh_request_info;
+
+typedef struct {
+  int64_t submitted, completed, failed;
+  int64_t in_flight, queued;
+  int64_t launches;         /* sweep launches so far */
+  double kernel_ms;         /* summed HIP-event time of those launches */
+  int64_t arena_bytes, arena_used, arena_peak;
+} ccdh_server_stats;
+
+int ccdh_server_create(const ccdh_server_options* options, ccdh_server_t* out);
+/* Queue a fit over host data; the server uploads it when the request is
+ * admitted.  desc's arrays must stay valid until the request completes; the
+ * spec's arrays are copied.  desc->precision must equal the server's. */
+int ccdh_server_submit(ccdh_server_t server, const ccdh_data_desc* desc, const ccdh_fit_spec* spec,
+                       int64_t* request_id);
+/* Queue a fit over resident data (the handle must outlive the request). */
+int ccdh_server_submit_resident(ccdh_server_t server, ccdh_data_t data, const ccdh_fit_spec* spec,
+                                int64_t* request_id);
+/* 1 when the request has completed (or failed), 0 while queued or in flight. */
+int ccdh_server_poll(ccdh_server_t server, int64_t request_id);
+/* Wait for a request (timeout_ms < 0: no limit; CCDH_E_STATE on timeout), copy
+ * its coefficients (beta_out [J], may be NULL), result and timing, and release
+ * it.  Returns the request's own status (CCDH_OK or the error it failed with). */
+int ccdh_server_wait(ccdh_server_t server, int64_t request_id, double timeout_ms, double* beta_out,
+                     ccdh_fit_result* result, ccdh_request_info* info);
+int ccdh_server_get_stats(ccdh_server_t server, ccdh_server_stats* out);
+/* Finish every queued and in-flight request, stop the threads, free the arena.
+ * Results not yet collected are discarded. */
+int ccdh_server_destroy(ccdh_server_t server);
+
 #ifdef __cplusplus
 }
 #endif

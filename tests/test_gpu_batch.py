@@ -79,7 +79,7 @@ def test_batch_graph_and_direct_launch_agree():
     assert np.array_equal(b1, b2)
 
 
-@pytest.mark.parametrize("threads", [64, 512, 1024])
+@pytest.mark.parametrize("threads", [256, 512])
 def test_batch_workgroup_sizes(threads):
     p, prior, var, maxit = SYNTH["lr"]()
     spec = default_spec(p, prior=prior, variance=var)
