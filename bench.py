@@ -14,7 +14,14 @@ fits, the CPU oracle on a sample (1 core), the largest coefficient difference of
 the batch against the CPU oracle on those fits, per-fit latency p50/p99 inside
 the batch, and the sweep kernel's achieved algorithmic bandwidth.
 
+Serving leg ("serving" in the same line): a stream of fit requests arriving as a
+Poisson process (logistic / Poisson / Cox over Zipf designs, rows and columns
+log-uniform, 20-60 nonzeros per row, each request with its own prior variance)
+served by the continuous-batching server; fits/s over the stream, p50/p99
+submit-to-result latency, and plain sequential GPU fits of sampled requests.
+
   python bench.py [--steps 1] [--warmup 0] [--scale 1.0] [--plain-fits 2] [--cpu-fits 1]
+                  [--serve-requests 200] [--serve-rate 50]
 """
 import argparse
 import json
@@ -50,6 +57,92 @@ def algorithmic_bytes(stats, nnz_free, precision):
     return state + index
 
 
+def serve_leg(args, api, synth):
+    """Continuous-batching server on a Poisson stream of mixed fit requests."""
+    from ccd_amd.data import COX, LOGISTIC, POISSON, PRIOR_LAPLACE, PRIOR_NORMAL, default_spec
+    t = time.perf_counter()
+    pool, reqs = synth.request_stream(
+        n_requests=args.serve_requests, rate=args.serve_rate,
+        K_range=(10_000, args.serve_kmax), P_range=(1_000, args.serve_pmax),
+        per_row_range=(20, 60), cox_K_max=args.serve_cox_kmax, seed=20261208)
+    log(f"serving pool: {len(pool)} designs, nnz {min(p.nnz for p in pool)}.."
+        f"{max(p.nnz for p in pool)}, generated in {time.perf_counter() - t:.1f}s")
+
+    def spec_of(i, var):
+        p = pool[i]
+        prior = PRIOR_NORMAL if p.model == POISSON else PRIOR_LAPLACE
+        return default_spec(p, prior=prior, variance=var)
+
+    specs = [spec_of(i, v) for _, i, v in reqs]
+    srv = api.Server(precision=api.FP64, quantum_ms=args.serve_quantum_ms)
+    srv.wait(srv.submit(pool[0], spec_of(0, 1.0)))  # load code objects (untimed)
+    t0 = time.perf_counter()
+    ids = []
+    for (ta, i, _), sp in zip(reqs, specs):
+        now = time.perf_counter() - t0
+        if ta > now:
+            time.sleep(ta - now)
+        ids.append(srv.submit(pool[i], sp))
+    log(f"serving: {len(ids)} requests submitted over {time.perf_counter() - t0:.1f}s")
+    results = []
+    for k, rid in enumerate(ids):
+        results.append(srv.wait(rid))
+        if (k + 1) % 50 == 0:
+            log(f"serving: {k + 1} results after {time.perf_counter() - t0:.1f}s")
+    span = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r[2]["latency"] for r in results])
+    queued = np.array([r[2]["queued"] for r in results])
+    iters = np.array([r[1]["iterations"] for r in results])
+    by_model = {}
+    for (_, i, _), r in zip(reqs, results):
+        by_model.setdefault(pool[i].model, []).append(r[2]["latency"])
+    names = {LOGISTIC: "logistic", POISSON: "poisson", COX: "cox"}
+    fps = len(ids) / span
+    log(f"serving: {fps:.2f} fits/s, latency p50 {np.percentile(lat, 50):.2f}s "
+        f"p99 {np.percentile(lat, 99):.2f}s, {st['launches']} launches, "
+        f"arena peak {st['arena_peak'] / 1e9:.2f} GB")
+
+    # plain sequential GPU fits of requests sampled from the same stream
+    rng = np.random.default_rng(7)
+    order = rng.permutation(len(reqs))
+    plain_t, plain_diff = [], 0.0
+    tp = time.perf_counter()
+    for k in order:
+        if time.perf_counter() - tp > args.serve_plain_seconds and plain_t:
+            break
+        _, i, _ = reqs[k]
+        t1 = time.perf_counter()
+        d = api.Data(pool[i])
+        bp, ip = api.fit_plain(d, specs[k])
+        d.close()
+        plain_t.append(time.perf_counter() - t1)
+        b = results[k][0]
+        rel = np.abs(bp - b) / np.maximum(np.abs(b), 1e-12)
+        plain_diff = max(plain_diff, float(np.max(np.where(np.abs(b) > 0, rel, np.abs(bp)))))
+        log(f"plain request {k} ({names[pool[i].model]}, K={pool[i].K}, J={pool[i].J}): "
+            f"{plain_t[-1]:.2f}s, {ip['iterations']} iterations (served {results[k][1]['iterations']})")
+    plain_fps = len(plain_t) / sum(plain_t)
+    return {
+        "workload": "served fit requests: logistic / Poisson / Cox, Zipf(1.1) designs",
+        "requests": len(ids), "arrival_rate_per_s": args.serve_rate,
+        "rows": [10_000, args.serve_kmax], "cox_rows_max": args.serve_cox_kmax,
+        "covariates": [1_000, args.serve_pmax], "nnz_per_row": [20, 60],
+        "designs": len(pool), "quantum_ms": args.serve_quantum_ms, "dtype": "f64",
+        "fits_per_sec": fps, "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "p50_queue_s": float(np.percentile(queued, 50)),
+        "latency_p50_by_model_s": {names[m]: float(np.percentile(v, 50))
+                                   for m, v in sorted(by_model.items())},
+        "iterations_median": int(np.median(iters)), "launches": st["launches"],
+        "arena_peak_bytes": st["arena_peak"], "failed": st["failed"],
+        "plain_sequential_fits_per_sec": plain_fps, "plain_sample": len(plain_t),
+        "speedup_vs_plain_gpu": fps / plain_fps,
+        "max_rel_coef_diff_plain_vs_served": plain_diff,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +160,15 @@ def main():
     ap.add_argument("--max-iter", type=int, default=1000, help="iteration cap (probe runs)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a device copy of this size (torch) for counter calibration")
+    ap.add_argument("--serve-requests", type=int, default=200, help="0: skip the serving leg")
+    ap.add_argument("--serve-rate", type=float, default=50.0, help="arrivals per second")
+    ap.add_argument("--serve-kmax", type=int, default=200_000, help="largest request rows")
+    ap.add_argument("--serve-pmax", type=int, default=20_000, help="largest request columns")
+    ap.add_argument("--serve-cox-kmax", type=int, default=20_000, help="largest Cox request rows")
+    ap.add_argument("--serve-quantum-ms", type=float, default=20.0)
+    ap.add_argument("--serve-plain-seconds", type=float, default=20.0,
+                    help="time budget of the plain sequential sample")
+    ap.add_argument("--skip-batch", action="store_true", help="serving leg only (probe)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -78,6 +180,10 @@ def main():
 
     name, n_cu, hbm = api.device_info()
     log("device", name, n_cu, "CUs", f"{hbm:.0f} GB")
+    serving = serve_leg(args, api, synth) if args.serve_requests > 0 else None
+    if args.skip_batch:
+        print(json.dumps({"serving": serving}), flush=True)
+        return
     n_strata = max(100, int(200_000 * args.scale))
     P = max(50, int(20_000 * args.scale))
     t = time.perf_counter()
@@ -214,9 +320,10 @@ def main():
         "max_rel_coef_diff_plain_vs_batched": plain_err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_batch_sweep", "avg_launch_ms": per_launch_ms,
+                     "kernel": "k_slot_sweep", "avg_launch_ms": per_launch_ms,
                      "launches": stats["launches"]},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -7,6 +7,9 @@
                       (2,000,000 x 50,000, 30 nnz/row)
   clr_strata ........ conditional logistic, strata of 1 case + U{1..10} controls
                       (200,000 strata x 20,000 columns, 25 nnz/row)
+  lr_zipf ........... logistic over Zipf(s) columns (served requests)
+  request_stream .... Poisson arrivals of logistic / Poisson / Cox fit requests with
+                      log-uniform rows and columns and U{20..60} nonzeros per row
 
 All generators are vectorised numpy and deterministic in their seed.
 """
@@ -164,3 +167,51 @@ def clr_strata(n_strata=200_000, P=20_000, per_row=25, s=1.1, max_controls=10, n
     y = is_case.astype(np.float64)
     prob = _assemble(CONDITIONAL_LOGISTIC, K, P, col_ptr, rows, y, strata=strata)
     return prob, dict(beta_cols=bc, beta_vals=bv, sizes=sizes)
+
+
+def lr_zipf(K=100_000, P=10_000, per_row=40, s=1.1, n_true=100, sd=0.5, intercept=-1.0,
+            seed=DEFAULT_SEED):
+    rng = np.random.default_rng(seed)
+    r, c = zipf_pairs(rng, K, P, per_row, s)
+    col_ptr, rows = _csc_from_pairs(r, c, K, P)
+    bc = rng.choice(P, min(n_true, P), replace=False)
+    bv = rng.normal(0.0, sd, len(bc))
+    eta = _xbeta(col_ptr, rows, K, bc, bv) + intercept
+    y = (rng.random(K) < 1.0 / (1.0 + np.exp(-eta))).astype(np.float64)
+    prob = _assemble(LOGISTIC, K, P, col_ptr, rows, y, intercept=True)
+    return prob, dict(beta_cols=bc, beta_vals=bv)
+
+
+def request_stream(n_requests=200, rate=50.0, K_range=(10_000, 200_000),
+                   P_range=(1_000, 20_000), per_row_range=(20, 60), cox_K_max=None,
+                   pool_per_model=4, models=(LOGISTIC, POISSON, COX), seed=DEFAULT_SEED):
+    """Served fit requests: a pool of designs per model (rows and columns
+    log-uniform in the ranges, nonzeros per row uniform, Zipf(1.1) columns) and
+    `n_requests` arrivals at Poisson `rate` per second, each naming a pool design
+    and its own prior variance (an outcome model / hyperparameter of its own).
+    Returns (pool, requests) with requests = [(arrival_s, pool_index, variance)]."""
+    rng = np.random.default_rng(seed)
+
+    def logu(lo, hi):
+        return int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+
+    pool = []
+    for m in models:
+        for i in range(pool_per_model):
+            k_hi = K_range[1] if (m != COX or cox_K_max is None) else min(K_range[1], cox_K_max)
+            K = logu(K_range[0], k_hi)
+            P = logu(*P_range)
+            per_row = int(rng.integers(per_row_range[0], per_row_range[1] + 1))
+            sd = int(rng.integers(1 << 30))
+            if m == LOGISTIC:
+                prob = lr_zipf(K=K, P=P, per_row=per_row, n_true=min(100, P // 10), seed=sd)[0]
+            elif m == POISSON:
+                prob = poisson_zipf(K=K, P=P, per_row=per_row, n_true=min(100, P // 10),
+                                    base=-5.0, seed=sd)[0]
+            else:
+                prob = cox_zipf(K=K, P=P, per_row=per_row, n_true=min(100, P // 10), seed=sd)[0]
+            pool.append(prob)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    idx = rng.integers(0, len(pool), n_requests)
+    var = np.exp(rng.uniform(np.log(0.01), np.log(1.0), n_requests))
+    return pool, [(float(t[i]), int(idx[i]), float(var[i])) for i in range(n_requests)]
