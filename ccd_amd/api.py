@@ -81,12 +81,13 @@ class BatchStats(ctypes.Structure):
 
 class BatchOptions(ctypes.Structure):
     _fields_ = [("threads", ctypes.c_int), ("sweeps_per_sync", ctypes.c_int),
-                ("use_graph", ctypes.c_int), ("_pad", ctypes.c_int)]
+                ("use_graph", ctypes.c_int), ("max_gang", ctypes.c_int),
+                ("gang_min_nnz", ctypes.c_int64)]
 
 
 class ServerOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("max_slots", ctypes.c_int),
-                ("threads", ctypes.c_int), ("_pad", ctypes.c_int),
+                ("threads", ctypes.c_int), ("max_gang", ctypes.c_int),
                 ("quantum_ms", ctypes.c_double), ("hbm_budget_bytes", ctypes.c_int64)]
 
 
@@ -257,8 +258,10 @@ def fit_plain(data: Data, spec: FitSpec, heldout=None):
     return beta, res.as_dict()
 
 
-def fit_batch(data: Data, specs, heldouts=None, threads=0, sweeps_per_sync=0, use_graph=1):
-    """Many fits over one resident data set with device-resident sweeps."""
+def fit_batch(data: Data, specs, heldouts=None, threads=0, sweeps_per_sync=0, use_graph=1,
+              max_gang=0, gang_min_nnz=0):
+    """Many fits over one resident data set with device-resident sweeps; a fit
+    runs on up to `max_gang` workgroups when fewer fits than CUs remain."""
     lib = load_library()
     n = len(specs)
     keep = []
@@ -267,7 +270,7 @@ def fit_batch(data: Data, specs, heldouts=None, threads=0, sweeps_per_sync=0, us
         arr[i] = _spec_struct(sp, keep, None if heldouts is None else heldouts[i])
     betas = np.zeros((n, data.problem.J))
     res = (FitResult * n)()
-    opt = BatchOptions(threads, sweeps_per_sync, use_graph, 0)
+    opt = BatchOptions(threads, sweeps_per_sync, use_graph, max_gang, gang_min_nnz)
     _check(lib.ccdh_batch_fit(data.handle, n, arr, _p(betas), res, ctypes.byref(opt)))
     return betas, [r.as_dict() for r in res]
 
@@ -368,9 +371,9 @@ class Server:
     request is collected."""
 
     def __init__(self, precision=FP64, max_slots=0, threads=0, quantum_ms=0.0,
-                 hbm_budget_bytes=0):
+                 hbm_budget_bytes=0, max_gang=0):
         self.lib = load_library()
-        opt = ServerOptions(precision, max_slots, threads, 0, quantum_ms, hbm_budget_bytes)
+        opt = ServerOptions(precision, max_slots, threads, max_gang, quantum_ms, hbm_budget_bytes)
         h = ctypes.c_void_p()
         _check(self.lib.ccdh_server_create(ctypes.byref(opt), ctypes.byref(h)))
         self.handle = h

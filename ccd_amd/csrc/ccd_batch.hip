@@ -456,82 +456,181 @@ __device__ __forceinline__ void loadRecords(const DevX& d, int fmt, int64_t e, i
   }
 }
 
-// One full coordinate cycle of a fit (independent rows or grouped).  Every lane
-// takes the (deterministic) prior step from the block-reduced gradient and
-// Hessian, so no broadcast barrier is needed; a barrier follows an update only.
-// From coordinate j0; returns the coordinate to resume at when the quantum ran
-// out, else J.
+// ------------------------------------------------------------ gangs
+// A slot may be run by a gang of G workgroups (G a power of two <= kShards).
+// Rank r owns the rows of shards [r*S/G, (r+1)*S/G) -- their xBeta, exp(xBeta),
+// weights and (CLR) strata -- and walks only its part of every column.  Per
+// coordinate the ranks exchange their block-reduced (gradient, Hessian) through
+// the slot's exchange buffer and all sum them in rank order, so every rank takes
+// the same prior step and updates its own rows; no rank reads another's rows.
+// Exchanged values travel as agent-scope atomics (coherent across XCDs without
+// L2 write-back or invalidation); a rank's arrival counter increment follows
+// the completion of its stores.  Spins are bounded: a rank that waits longer
+// than kSpinLimit polls gives up and the launch reports the failure.
+struct Gang {
+  int rank, G;
+  int rowLo, rowHi;  // rows owned by this rank
+  int s0, s1;        // shard range
+  unsigned seq;      // exchanges done in this launch
+  Xch* x;
+  bool ok;
+};
+
+constexpr long long kSpinLimit = 1ll << 24;
+
+__device__ __forceinline__ double loadShared(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void storeShared(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum NV block totals over the gang (rank order); every thread gets the sums.
+template <int NV>
+__device__ __forceinline__ bool gangSum(Gang& g, double (&v)[NV]) {
+  static_assert(NV <= 4, "exchange carries at most 4 values");
+  __shared__ double sOut[2][4];
+  __shared__ int sOk[2];
+  const int par = (int)(g.seq & 1u);
+  if (threadIdx.x == 0) {
+    double* mine = &g.x->v[par][g.rank][0];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) storeShared(mine + i, v[i]);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);  // the values have reached the coherence point
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __hip_atomic_fetch_add(&g.x->cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned target = (unsigned)g.G * (g.seq + 1u);
+    long long spins = 0;
+    int ok = 1;
+    while (__hip_atomic_load(&g.x->cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > kSpinLimit) {
+        ok = 0;
+        break;
+      }
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    sOk[par] = ok;
+  }
+  __syncthreads();
+  const bool ok = sOk[par] != 0;
+  if (threadIdx.x < kWave) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double t = (ok && (int)threadIdx.x < g.G) ? loadShared(&g.x->v[par][threadIdx.x][i]) : 0.0;
+      t = waveSum(t);
+      if (threadIdx.x == 0) sOut[par][i] = t;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = sOut[par][i];
+  g.seq++;
+  g.ok = g.ok && ok;
+  return ok;
+}
+
+// One coordinate cycle of a fit (independent rows or grouped) over this rank's
+// rows, from coordinate j0.  Every lane takes the (deterministic) prior step
+// from the reduced gradient and Hessian, so no broadcast barrier is needed; a
+// barrier follows an update only.  Returns the coordinate to resume at when the
+// quantum ran out (J when the cycle completed), or -1 when a gang exchange
+// failed.
 template <int MODEL, typename R, class LDS>
-__device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus& cs, LDS& S, int j0, long long deadline) {
+__device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus& cs, LDS& S, int j0,
+                                        long long deadline, Gang& gg) {
   const DevX& d = s.d;
   const int64_t CH = (int64_t)blockDim.x * kIT;
   const int64_t CHL = (int64_t)blockDim.x * kITL;
   const int J = s.J;
   const int useW = s.ctl.useW;
+  const bool multi = gg.G > 1;
+  const bool lead = gg.rank == 0;
   long long cG = 0, cGR = 0, cU = 0, cUR = 0, tS = 0, tL = 0, nL = 0;
   int jEnd = J;
   for (int j = j0; j < J; ++j) {
-    if (j > j0 && (j & 3) == 0 && quantumOver(deadline)) {
+    if (!multi && j > j0 && (j & 3) == 0 && quantumOver(deadline)) {
       jEnd = j;
       break;
     }
     if (s.fixed[j]) continue;
-    const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
-    const int64_t nj = e - b;
+    const int64_t cb = d.colPtr[j], ce = d.colPtr[j + 1];
+    int64_t b = cb, e = ce;
+    if (multi) {
+      const int32_t* off = d.colShard + (size_t)j * (d.nShards + 1);
+      b = cb + off[gg.s0];
+      e = cb + off[gg.s1];
+    }
+    const int64_t nj = e - b;      // this rank's nonzeros
+    const int64_t njAll = ce - cb;  // the column's
     const int fmt = d.fmt[j];
-    const double bj = s.beta[j], pv = s.pvar[j], xy = s.xjy[j];
-    double bnd = s.bound[j];
+    const double bj = multi ? loadShared(&s.beta[j]) : s.beta[j];
+    double bnd = multi ? loadShared(&s.bound[j]) : s.bound[j];
+    const double pv = s.pvar[j], xy = s.xjy[j];
     const int pt = s.ptype[j];
-    const long long runs = MODEL == M_CLR ? (long long)(d.segPtr[j + 1] - d.segPtr[j]) : nj;
-    const bool staged = MODEL == M_CLR && runs < nj;
+    const long long runs = MODEL == M_CLR ? (long long)(d.segPtr[j + 1] - d.segPtr[j]) : njAll;
+    const bool staged = MODEL == M_CLR && runs < njAll;
     if (threadIdx.x == 0) S.cValid = 0;
     const long long t0 = wall_clock64();
-    double delta;
-    if (nj <= CH) {
-      Lanes<kIT, R> L;
-      passLoad<MODEL, kIT, R>(s, fmt, b, e, b, useW, L);
-      double v[2] = {0.0, 0.0};
-      passGather<MODEL, kIT, R>(fmt, S, (int)nj, staged, L, useW, v);
+    double v[2] = {0.0, 0.0};
+    bool stop = false;
+    // the exchange also carries the lead rank's end-of-quantum decision
+    auto reduce = [&]() -> bool {
       blockSum<2>(v);
-      double g = nj > 0 ? v[0] - xy : 0.0, h = nj > 0 ? v[1] : 0.0;
+      if (!multi) return true;
+      double w[3] = {v[0], v[1],
+                     (lead && threadIdx.x == 0 && deadline > 0 && wall_clock64() >= deadline) ? 1.0 : 0.0};
+      if (!gangSum<3>(gg, w)) return false;
+      v[0] = w[0];
+      v[1] = w[1];
+      stop = w[2] != 0.0;
+      return true;
+    };
+    auto step = [&]() -> double {
+      double g = njAll > 0 ? v[0] - xy : 0.0, h = njAll > 0 ? v[1] : 0.0;
       if (h < 0.0) {
         g = 0.0;
         h = 0.0;
       }
-      delta = applyBounds(priorDelta(pt, pv, g, h, bj), &bnd);
-      if (threadIdx.x == 0) {
-        s.bound[j] = bnd;
-        if (delta != 0.0) s.beta[j] = bj + delta;
+      const double dl = applyBounds(priorDelta(pt, pv, g, h, bj), &bnd);
+      if (threadIdx.x == 0 && lead) {
+        if (multi) {
+          storeShared(&s.bound[j], bnd);
+          if (dl != 0.0) storeShared(&s.beta[j], bj + dl);
+        } else {
+          s.bound[j] = bnd;
+          if (dl != 0.0) s.beta[j] = bj + dl;
+        }
       }
+      return dl;
+    };
+    double delta;
+    if (nj <= CH) {
+      Lanes<kIT, R> L;
+      passLoad<MODEL, kIT, R>(s, fmt, cb, e, b, useW, L);
+      passGather<MODEL, kIT, R>(fmt, S, (int)nj, staged, L, useW, v);
+      if (!reduce()) return -1;
+      delta = step();
       if (delta != 0.0 && nj > 0) {
         passUpdate<MODEL, kIT, R>(s, S, (int)nj, staged, L, delta);
         __syncthreads();
       }
     } else {
-      double v[2] = {0.0, 0.0};
       for (int64_t c0 = b; c0 < e; c0 += CHL) {
         Lanes<kITL, R> L;
-        passLoad<MODEL, kITL, R>(s, fmt, b, e, c0, useW, L);
-        storeRecords<MODEL, kITL, R>(s.rec, c0 - b, L);
+        passLoad<MODEL, kITL, R>(s, fmt, cb, e, c0, useW, L);
+        storeRecords<MODEL, kITL, R>(s.rec, c0 - cb, L);
         passGather<MODEL, kITL, R>(fmt, S, (int)min(CHL, e - c0), staged, L, useW, v);
       }
-      blockSum<2>(v);
-      double g = v[0] - xy, h = v[1];
-      if (h < 0.0) {
-        g = 0.0;
-        h = 0.0;
-      }
-      delta = applyBounds(priorDelta(pt, pv, g, h, bj), &bnd);
-      if (threadIdx.x == 0) {
-        s.bound[j] = bnd;
-        if (delta != 0.0) s.beta[j] = bj + delta;
-        S.cValid = 0;
-      }
+      if (!reduce()) return -1;
+      delta = step();
+      if (threadIdx.x == 0) S.cValid = 0;
       if (delta != 0.0) {
         __syncthreads();  // carry reset visible; records were written by these lanes
         for (int64_t c0 = b; c0 < e; c0 += CHL) {
           Lanes<kITL, R> L;
-          loadRecords<MODEL, kITL, R>(d, fmt, e, c0, c0 - b, s.rec, L);
+          loadRecords<MODEL, kITL, R>(d, fmt, e, c0, c0 - cb, s.rec, L);
           passUpdate<MODEL, kITL, R>(s, S, (int)min(CHL, e - c0), staged, L, delta);
         }
         __syncthreads();
@@ -545,34 +644,48 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus& cs, LDS& S,
       tS += t1 - t0;
     }
     cG += nj;
-    cGR += MODEL == M_CLR ? runs : 0;
+    cGR += MODEL == M_CLR && lead ? runs : 0;
     if (delta != 0.0) {
       cU += nj;
-      cUR += MODEL == M_CLR ? runs : 0;
+      cUR += MODEL == M_CLR && lead ? runs : 0;
+    }
+    if (stop) {
+      jEnd = j + 1;
+      break;
     }
   }
   if (threadIdx.x == 0) {
-    cs.tSmall += tS;
-    cs.tLarge += tL;
-    cs.nLarge += nL;
-    cs.gatherNnz += cG;
-    cs.gatherRuns += cGR;
-    cs.updateNnz += cU;
-    cs.updateRuns += cUR;
+    atomicAdd((unsigned long long*)&cs.tSmall, (unsigned long long)tS);
+    atomicAdd((unsigned long long*)&cs.tLarge, (unsigned long long)tL);
+    atomicAdd((unsigned long long*)&cs.nLarge, (unsigned long long)nL);
+    atomicAdd((unsigned long long*)&cs.gatherNnz, (unsigned long long)cG);
+    atomicAdd((unsigned long long*)&cs.gatherRuns, (unsigned long long)cGR);
+    atomicAdd((unsigned long long*)&cs.updateNnz, (unsigned long long)cU);
+    atomicAdd((unsigned long long*)&cs.updateRuns, (unsigned long long)cUR);
   }
   return jEnd;
 }
 
 // ------------------------------------------------------------ whole-fit pieces
-// log likelihood of one fit (ModelSpecifics.hpp:605-650); accumulated
-// denominators for Cox come from an in-block scan
+// Rows [rowLo, rowHi) and groups [grpLo, grpHi) a workgroup sums over.
+struct Part {
+  int rowLo, rowHi, grpLo, grpHi;
+};
+
+template <typename R>
+__device__ __forceinline__ Part wholeFit(const Slot<R>& s) {
+  return Part{0, s.K, 0, s.N};
+}
+
+// log likelihood terms of one fit over a part (ModelSpecifics.hpp:605-650);
+// accumulated denominators for Cox come from an in-block scan (whole fit only)
 template <int MODEL, typename R>
-__device__ __forceinline__ double fitLogLik(const Slot<R>& s) {
+__device__ __forceinline__ double fitLogLik(const Slot<R>& s, const Part& pt) {
   const DevX& d = s.d;
   const int useW = s.ctl.useW;
   const int N = s.N;
   double v[2] = {0.0, 0.0};
-  for (int k = threadIdx.x; k < s.K; k += blockDim.x) {
+  for (int k = pt.rowLo + threadIdx.x; k < pt.rowHi; k += blockDim.x) {
     const double x = (double)s.xb[k];
     const int yi = (int)d.y[k];
     const double w = useW ? s.kw[k] : 1.0;
@@ -582,7 +695,8 @@ __device__ __forceinline__ double fitLogLik(const Slot<R>& s) {
     if (MODEL == M_POISSON) v[1] += w * (double)s.oeb[k];
   }
   if (MODEL == M_CLR) {
-    for (int i = threadIdx.x; i < N; i += blockDim.x) v[1] += s.nw[i] * log(s.denom[i]);
+    for (int i = pt.grpLo + threadIdx.x; i < pt.grpHi; i += blockDim.x)
+      if (s.nw[i] != 0.0) v[1] += s.nw[i] * log(s.denom[i]);
   }
   if (MODEL == M_COX) {
     Seg<3> carry;
@@ -625,10 +739,10 @@ __device__ __forceinline__ double fitLogLik(const Slot<R>& s) {
 }
 
 template <typename R>
-__device__ __forceinline__ double fitGradObjective(const Slot<R>& s) {
+__device__ __forceinline__ double fitGradObjective(const Slot<R>& s, const Part& pt) {
   const int useW = s.ctl.useW;
   double v[1] = {0.0};
-  for (int k = threadIdx.x; k < s.K; k += blockDim.x) {
+  for (int k = pt.rowLo + threadIdx.x; k < pt.rowHi; k += blockDim.x) {
     const double x = (double)s.xb[k];
     v[0] += useW ? x * s.d.y[k] * s.kw[k] : x * s.d.y[k];
   }
@@ -636,18 +750,51 @@ __device__ __forceinline__ double fitGradObjective(const Slot<R>& s) {
   return v[0];
 }
 
+__device__ __forceinline__ double priorPart(const double* beta, const double* pvar,
+                                            const int8_t* ptype, int J, bool shared) {
+  double v[1] = {0.0};
+  for (int j = threadIdx.x; j < J; j += blockDim.x)
+    v[0] += priorLogDensity(ptype[j], pvar[j], shared ? loadShared(&beta[j]) : beta[j]);
+  blockSum<1>(v);
+  return v[0];
+}
+
+// Convergence objective (getObjectiveFunction, CyclicCoordinateDescent.cpp:541-558)
+// of the whole fit (single workgroup).
 template <int MODEL, typename R>
 __device__ __forceinline__ double fitObjective(const Slot<R>& s) {
   const int ct = s.ctl.convergenceType;
-  if (ct == 0) return fitGradObjective<R>(s);
+  const Part all = wholeFit(s);
+  if (ct == 0) return fitGradObjective<R>(s, all);
   if (ct == 3) return 0.0;
-  const double ll = fitLogLik<MODEL, R>(s) + s.fixedLL;
+  const double ll = fitLogLik<MODEL, R>(s, all) + s.fixedLL;
   if (ct == 2) return ll;
-  double v[1] = {0.0};
-  for (int j = threadIdx.x; j < s.J; j += blockDim.x)
-    v[0] += priorLogDensity(s.ptype[j], s.pvar[j], s.beta[j]);
-  blockSum<1>(v);
-  return ll + v[0];
+  return ll + priorPart(s.beta, s.pvar, s.ptype, s.J, false);
+}
+
+// The same objective summed over a gang; `stop` carries the lead rank's
+// end-of-quantum decision.  Returns false when the exchange failed.
+template <int MODEL, typename R>
+__device__ __forceinline__ bool gangObjective(const Slot<R>& s, Gang& gg, long long deadline,
+                                              double* obj, bool* stop) {
+  const int ct = s.ctl.convergenceType;
+  Part pt{gg.rowLo, gg.rowHi, 0, 0};
+  if (MODEL == M_CLR && gg.rowHi > gg.rowLo) {
+    pt.grpLo = s.d.strata[gg.rowLo];
+    pt.grpHi = s.d.strata[gg.rowHi - 1] + 1;
+  }
+  double v[3] = {0.0, 0.0, 0.0};
+  if (ct == 0) v[0] = fitGradObjective<R>(s, pt);
+  if (ct == 1 || ct == 2) v[0] = fitLogLik<MODEL, R>(s, pt);
+  if (ct == 1 && gg.rank == 0) v[1] = priorPart(s.beta, s.pvar, s.ptype, s.J, true);
+  v[2] = (gg.rank == 0 && threadIdx.x == 0 && deadline > 0 && wall_clock64() >= deadline) ? 1.0 : 0.0;
+  if (!gangSum<3>(gg, v)) return false;
+  *stop = v[2] != 0.0;
+  if (ct == 0) *obj = v[0];
+  else if (ct == 3) *obj = 0.0;
+  else if (ct == 2) *obj = v[0] + s.fixedLL;
+  else *obj = v[0] + s.fixedLL + v[1];
+  return true;
 }
 
 // Per-fit set-up: event weights, XjY, xBeta from the starting coefficients,
@@ -742,52 +889,73 @@ __device__ __forceinline__ void initSlot(const Slot<R>& s, FitStatus& st) {
 
 // Up to `sweeps` complete CCD cycles plus the convergence check after each;
 // with a deadline, the slot stops at a coordinate boundary once it has passed
-// and the next launch resumes the sweep there.
+// and the next launch resumes the sweep there.  Every rank of a gang computes
+// the same objective and convergence decision; the lead rank records them.
 template <int MODEL, typename R, class LDS>
-__device__ __forceinline__ void sweepSlot(const Slot<R>& s, FitStatus& st, LDS& S, int sweeps, long long deadline) {
+__device__ __forceinline__ void sweepSlot(const Slot<R>& s, FitStatus& st, LDS& S, int sweeps,
+                                          long long deadline, Gang& gg) {
   const FitCtl c = s.ctl;
-  __shared__ int sDone;
+  const bool multi = gg.G > 1;
+  const bool rec = threadIdx.x == 0 && gg.rank == 0;
   int j0 = st.jNext;
+  int iteration = st.iteration;
+  double lastObj = st.lastObj;
   for (int sw = 0; sw < sweeps; ++sw) {
     const int jEnd = MODEL == M_COX ? cycleCox<R>(s, st, j0, deadline)
-                                    : cycleFit<MODEL, R>(s, st, S, j0, deadline);
+                                    : cycleFit<MODEL, R>(s, st, S, j0, deadline, gg);
     __syncthreads();
+    if (jEnd < 0) {
+      if (threadIdx.x == 0) st.fault = 1;
+      return;
+    }
     if (jEnd < s.J) {
-      if (threadIdx.x == 0) st.jNext = jEnd;
+      if (rec) st.jNext = jEnd;
       return;
     }
     j0 = 0;
-    if (threadIdx.x == 0) st.jNext = 0;
     // performCheckConvergence
     const long long tq0 = wall_clock64();
-    const double obj = fitObjective<MODEL, R>(s);
-    if (threadIdx.x == 0) {
-      st.tObj += wall_clock64() - tq0;
-      st.iteration += 1;
-      st.objRows += (c.convergenceType == 0 ? s.K : (c.convergenceType == 3 ? 0 : s.K + s.N));
-      double conv;
-      bool ill = false;
-      if (obj != obj) {
-        conv = 0.0;
-        ill = true;
-      } else {
-        conv = fabs(obj - st.lastObj) / (fabs(obj) + 1.0);
+    double obj;
+    bool stop = false;
+    if (multi) {
+      if (!gangObjective<MODEL, R>(s, gg, deadline, &obj, &stop)) {
+        if (threadIdx.x == 0) st.fault = 1;
+        return;
       }
-      st.lastObj = obj;
-      int done = 0;
-      if (c.tolerance > 0 && conv < c.tolerance) {
-        st.flag = ill ? CCDH_ILLCONDITIONED : CCDH_SUCCESS;
-        done = 1;
-      } else if (st.iteration == c.maxIterations) {
-        st.flag = CCDH_MAX_ITERATIONS;
-        done = 1;
-      }
-      st.done = done;
-      sDone = done;
+    } else {
+      obj = fitObjective<MODEL, R>(s);
     }
-    __syncthreads();
-    if (sDone) return;
-    if (quantumOver(deadline)) return;
+    iteration += 1;
+    double conv;
+    bool ill = false;
+    if (obj != obj) {
+      conv = 0.0;
+      ill = true;
+    } else {
+      conv = fabs(obj - lastObj) / (fabs(obj) + 1.0);
+    }
+    lastObj = obj;
+    int done = 0, flag = CCDH_SUCCESS;
+    if (c.tolerance > 0 && conv < c.tolerance) {
+      flag = ill ? CCDH_ILLCONDITIONED : CCDH_SUCCESS;
+      done = 1;
+    } else if (iteration == c.maxIterations) {
+      flag = CCDH_MAX_ITERATIONS;
+      done = 1;
+    }
+    if (rec) {
+      st.tObj += wall_clock64() - tq0;
+      st.objRows += (c.convergenceType == 0 ? s.K : (c.convergenceType == 3 ? 0 : s.K + s.N));
+      st.iteration = iteration;
+      st.lastObj = obj;
+      st.jNext = 0;
+      if (done) {
+        st.flag = flag;
+        st.done = 1;
+      }
+    }
+    if (done) return;
+    if (multi ? stop : quantumOver(deadline)) return;
   }
 }
 
@@ -795,7 +963,7 @@ __device__ __forceinline__ void sweepSlot(const Slot<R>& s, FitStatus& st, LDS& 
 // log likelihood (ModelSpecifics.hpp:652-690; LR / Poisson / CLR forms).
 template <int MODEL, typename R>
 __device__ __forceinline__ void finishSlot(const Slot<R>& s, FitStatus& st) {
-  const double ll = fitLogLik<MODEL, R>(s);
+  const double ll = fitLogLik<MODEL, R>(s, wholeFit(s));
   double v[1] = {0.0};
   if (s.heldout && MODEL != M_COX) {
     const double* w2 = s.heldout;
@@ -832,22 +1000,47 @@ __global__ void k_slot_init(const Slot<R>* slots, FitStatus* st, const int32_t* 
 
 template <typename R, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
-    k_slot_sweep(const Slot<R>* slots, FitStatus* st, const int32_t* active, int sweeps,
-                 long long quantum) {
+    k_slot_sweep(const Slot<R>* slots, FitStatus* st, const GangEntry* map, Xch* xch,
+                 int sweeps, long long quantum) {
   __shared__ PassLds<BLOCK * kITL> S;
   __shared__ long long sStart;
-  const int f = active[blockIdx.x];
+  const GangEntry ge = map[blockIdx.x];
+  if (ge.slot < 0) return;
+  const int f = ge.slot;
   if (st[f].done) return;
   if (threadIdx.x == 0) sStart = wall_clock64();
   __syncthreads();
   const long long deadline = quantum > 0 ? sStart + quantum : 0;
   const Slot<R>& s = slots[f];
-  switch (s.model) {
-    case M_LOGISTIC: sweepSlot<M_LOGISTIC, R>(s, st[f], S, sweeps, deadline); break;
-    case M_POISSON: sweepSlot<M_POISSON, R>(s, st[f], S, sweeps, deadline); break;
-    case M_CLR: sweepSlot<M_CLR, R>(s, st[f], S, sweeps, deadline); break;
-    case M_COX: sweepSlot<M_COX, R>(s, st[f], S, sweeps, deadline); break;
+  Gang gg;
+  gg.rank = ge.rank;
+  gg.G = ge.G;
+  gg.seq = 0;
+  gg.ok = true;
+  gg.x = ge.G > 1 ? xch + f : nullptr;
+  if (ge.G > 1) {
+    gg.s0 = ge.rank * s.d.nShards / ge.G;
+    gg.s1 = (ge.rank + 1) * s.d.nShards / ge.G;
+    gg.rowLo = s.d.shardRow[gg.s0];
+    gg.rowHi = s.d.shardRow[gg.s1];
+  } else {
+    gg.s0 = 0;
+    gg.s1 = s.d.nShards;
+    gg.rowLo = 0;
+    gg.rowHi = s.K;
   }
+  switch (s.model) {
+    case M_LOGISTIC: sweepSlot<M_LOGISTIC, R>(s, st[f], S, sweeps, deadline, gg); break;
+    case M_POISSON: sweepSlot<M_POISSON, R>(s, st[f], S, sweeps, deadline, gg); break;
+    case M_CLR: sweepSlot<M_CLR, R>(s, st[f], S, sweeps, deadline, gg); break;
+    case M_COX: sweepSlot<M_COX, R>(s, st[f], S, sweeps, deadline, gg); break;
+  }
+}
+
+// Zero the exchange counters of the gangs of the next launch.
+__global__ void k_gang_reset(const GangEntry* map, int n, Xch* xch) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && map[i].slot >= 0 && map[i].rank == 0 && map[i].G > 1) xch[map[i].slot].cnt = 0;
 }
 
 template <typename R>
@@ -887,24 +1080,51 @@ int slotLaunchInit(int /*model*/, bool mixed, const void* slots, FitStatus* st,
   return 0;
 }
 
-int slotLaunchSweeps(int /*model*/, bool mixed, const void* slots, FitStatus* st,
-                     const int32_t* active, int n, int threads, int sweeps,
-                     long long quantumTicks, hipStream_t s) {
+int slotLaunchSweeps(bool mixed, const void* slots, FitStatus* st, const GangEntry* map, int n,
+                     Xch* xch, bool gangs, int threads, int sweeps, long long quantumTicks,
+                     hipStream_t s) {
   if (n <= 0) return 0;
-  const long long q = quantumTicks;
-  if (clampThreads(threads) == 256) {
-    if (mixed)
-      k_slot_sweep<float, 256><<<n, 256, 0, s>>>((const Slot<float>*)slots, st, active, sweeps, q);
-    else
-      k_slot_sweep<double, 256><<<n, 256, 0, s>>>((const Slot<double>*)slots, st, active, sweeps, q);
-  } else {
-    if (mixed)
-      k_slot_sweep<float, 512><<<n, 512, 0, s>>>((const Slot<float>*)slots, st, active, sweeps, q);
-    else
-      k_slot_sweep<double, 512><<<n, 512, 0, s>>>((const Slot<double>*)slots, st, active, sweeps, q);
+  if (gangs) {
+    k_gang_reset<<<(n + 255) / 256, 256, 0, s>>>(map, n, xch);
+    HIP_TRY(hipGetLastError());
   }
-  HIP_TRY(hipGetLastError());
+  long long q = quantumTicks;
+  const void* fn;
+  const int block = clampThreads(threads);
+  if (block == 256)
+    fn = mixed ? (const void*)k_slot_sweep<float, 256> : (const void*)k_slot_sweep<double, 256>;
+  else
+    fn = mixed ? (const void*)k_slot_sweep<float, 512> : (const void*)k_slot_sweep<double, 512>;
+  void* args[] = {(void*)&slots, (void*)&st, (void*)&map, (void*)&xch, (void*)&sweeps, (void*)&q};
+  if (gangs) {
+    // every workgroup of a gang must be resident while its peers wait for it
+    HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(n), dim3(block), args, 0, s));
+  } else {
+    HIP_TRY(hipLaunchKernel(fn, dim3(n), dim3(block), args, 0, s));
+  }
   return 0;
+}
+
+int gangSize(int n, int capacity, int64_t nnz, int model, int maxGang, int64_t minNnzPerRank) {
+  if (model == CCDH_COX || n <= 0) return 1;
+  int G = 1;
+  const int cap = std::min(kShards, std::max(1, maxGang));
+  while (G * 2 <= cap && (int64_t)n * G * 2 <= capacity && nnz / (G * 2) >= minNnzPerRank) G *= 2;
+  return G;
+}
+
+int slotGangCapacity(bool mixed, int threads) {
+  const int block = clampThreads(threads);
+  const void* fn;
+  if (block == 256)
+    fn = mixed ? (const void*)k_slot_sweep<float, 256> : (const void*)k_slot_sweep<double, 256>;
+  else
+    fn = mixed ? (const void*)k_slot_sweep<float, 512> : (const void*)k_slot_sweep<double, 512>;
+  int perCU = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fn, block, 0) != hipSuccess) return 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  return perCU * cus;
 }
 
 int slotLaunchFinish(int /*model*/, bool mixed, const void* slots, FitStatus* st,
@@ -1113,7 +1333,10 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
   int threads = (opt && opt->threads) ? opt->threads : 256;
   threads = std::max(64, std::min(1024, threads / 64 * 64));
   const int sweepsPerSync = (opt && opt->sweeps_per_sync > 0) ? opt->sweeps_per_sync : 4;
-  const bool useGraph = opt ? opt->use_graph != 0 : true;
+  const int maxGang = (opt && opt->max_gang > 0) ? opt->max_gang : kShards;
+  const int64_t minNnz = (opt && opt->gang_min_nnz > 0) ? opt->gang_min_nnz : 32768;
+  // graph replay of a fixed launch when gangs are off
+  const bool useGraph = (opt ? opt->use_graph != 0 : true) && maxGang == 1;
 
   Owned o;
   HIP_TRY(hipStreamCreateWithFlags(&o.stream, hipStreamNonBlocking));
@@ -1127,15 +1350,22 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
     anyHeld |= specs[f].heldout != nullptr;
   }
   const SlotLayout L = slotLayout(model, d->K, J, maxN, maxColumnLength(d), anyHeld, sizeof(R));
+  const int capacity = slotGangCapacity(mixed, threads);
+  if (capacity <= 0) return setError(CCDH_E_DEVICE, "sweep kernel occupancy query failed");
   char* arena = nullptr;
   Slot<R>* dslots = nullptr;
   FitStatus* dst = nullptr;
   int32_t* dactive = nullptr;
+  GangEntry* dmap = nullptr;
+  Xch* dxch = nullptr;
+  const int mapCap = std::max(F, capacity);
   int rc = 0;
   rc = rc ? rc : devAllocOwned(&arena, (size_t)F * L.bytes, o.ptrs);
   rc = rc ? rc : devAllocOwned(&dslots, F, o.ptrs);
   rc = rc ? rc : devAllocOwned(&dst, F, o.ptrs);
   rc = rc ? rc : devAllocOwned(&dactive, F, o.ptrs);
+  rc = rc ? rc : devAllocOwned(&dmap, mapCap, o.ptrs);
+  rc = rc ? rc : devAllocOwned(&dxch, F, o.ptrs);
   if (rc) return rc;
   HIP_TRY(hipHostMalloc((void**)&o.hst, F * sizeof(FitStatus), hipHostMallocDefault));
 
@@ -1154,13 +1384,35 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
   rc = slotLaunchInit(model, mixed, dslots, dst, dactive, F, threads, s);
   if (rc) return rc;
 
+  // Launch map: every unfinished fit, each on a gang sized by how many fits are
+  // left (the tail of slow fits spreads over the idle CUs).
+  std::vector<GangEntry> map;
+  int mapN = 0;
+  bool mapGangs = false;
+  std::vector<char> running(F, 1);
+  auto buildMap = [&]() -> int {
+    int nAct = 0;
+    for (int f = 0; f < F; ++f) nAct += running[f];
+    const int G = gangSize(nAct, capacity, d->nnz, model, maxGang, minNnz);
+    map.clear();
+    for (int f = 0; f < F; ++f)
+      if (running[f] || useGraph)
+        for (int r = 0; r < G; ++r) map.push_back(GangEntry{f, r, G, 0});
+    mapN = (int)map.size();
+    mapGangs = G > 1;
+    HIP_TRY(hipMemcpyAsync(dmap, map.data(), mapN * sizeof(GangEntry), hipMemcpyHostToDevice, s));
+    return 0;
+  };
   // sweep launches, optionally replayed from a captured graph
   auto enqueueSweeps = [&]() -> int {
-    int r = slotLaunchSweeps(model, mixed, dslots, dst, dactive, F, threads, sweepsPerSync, 0, s);
+    int r = slotLaunchSweeps(mixed, dslots, dst, dmap, mapN, dxch, mapGangs, threads,
+                             sweepsPerSync, 0, s);
     if (r) return r;
     HIP_TRY(hipMemcpyAsync(o.hst, dst, F * sizeof(FitStatus), hipMemcpyDeviceToHost, s));
     return 0;
   };
+  rc = buildMap();
+  if (rc) return rc;
   if (useGraph) {
     HIP_TRY(hipStreamSynchronize(s));
     HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1179,6 +1431,10 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
   HIP_TRY(hipEventCreate(&o.ev1));
   int guard = 0;
   for (;;) {
+    if (!useGraph) {
+      rc = buildMap();
+      if (rc) return rc;
+    }
     HIP_TRY(hipEventRecord(o.ev0, s));
     if (useGraph) {
       HIP_TRY(hipGraphLaunch(o.exec, s));
@@ -1202,8 +1458,11 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
     }
     const double now = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     bool allDone = true, retry = false;
+    for (int f = 0; f < F; ++f)
+      if (o.hst[f].fault) return setError(CCDH_E_DEVICE, "gang exchange timed out");
     for (int f = 0; f < F; ++f) {
       FitStatus& h = o.hst[f];
+      running[f] = !h.done;
       if (!h.done) {
         allDone = false;
         continue;
@@ -1225,6 +1484,7 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
         rc = slotLaunchInit(model, mixed, dslots, dst, dactive, 1, threads, s);
         if (rc) return rc;
         HIP_TRY(hipStreamSynchronize(s));
+        running[f] = 1;
       }
     }
     if (retry) {

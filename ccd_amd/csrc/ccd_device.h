@@ -37,7 +37,15 @@ struct DevX {
   const int32_t* segKey;    // [nseg] stratum of the run
   const int32_t* stratStart;// [nStrata+1] first row of each stratum
   const int32_t* nzKey;     // [nnz] stratum of each nonzero's row (grouped models)
+  // row shards for workgroup gangs: shard s holds rows [shardRow[s], shardRow[s+1]),
+  // boundaries on stratum (CLR) / tie-group (Cox) starts; column j's nonzeros of
+  // shard s start at colPtr[j] + colShard[j * (nShards + 1) + s]
+  int nShards;
+  const int32_t* shardRow;  // [nShards+1]
+  const int32_t* colShard;  // [J * (nShards+1)]
 };
+
+constexpr int kShards = 64;  // row shards per data set (largest gang)
 
 template <typename R>
 __device__ __forceinline__ R colValue(const DevX& d, int fmt, int64_t p);

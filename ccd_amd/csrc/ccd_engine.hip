@@ -538,6 +538,9 @@ DevX ccdh_data::view() const {
   x.segKey = segKey;
   x.stratStart = stratStart;
   x.nzKey = nzKey;
+  x.nShards = kShards;
+  x.shardRow = shardRow;
+  x.colShard = colShard;
   return x;
 }
 
@@ -583,7 +586,47 @@ int64_t dataFootprint(const ccdh_data_desc* desc) {
   if (needVals) t += r(nnz * 8) + (desc->precision == CCDH_MIXED ? r(nnz * 4) : 0);
   if (desc->model == CCDH_CONDITIONAL_LOGISTIC)
     t += r(nnz * 4) + r((J + 1) * 8) + 2 * r(nnz * 4) + r((nStrata + 1) * 4);
+  t += r((kShards + 1) * 4) + r(J * (int64_t)(kShards + 1) * 4);
   return t;
+}
+
+// Row shards for workgroup gangs: kShards near-equal row ranges whose
+// boundaries start a stratum (CLR) or a tie group of events (Cox: consecutive
+// events of one stratum at one time share a risk-set group under any weights),
+// and per column the offset of each shard's first nonzero.
+static void buildShards(const ccdh_data_desc* desc, const std::vector<int32_t>& strata,
+                        std::vector<int32_t>& shardRow, std::vector<int32_t>& colShard) {
+  const int K = desc->K, J = desc->J, S = kShards;
+  auto startsGroup = [&](int k) {
+    if (k <= 0 || k >= K) return true;
+    if (desc->model == CCDH_CONDITIONAL_LOGISTIC) return strata[k] != strata[k - 1];
+    if (desc->model == CCDH_COX)
+      return strata[k] != strata[k - 1] ||
+             !(desc->y[k] == 1.0 && desc->y[k - 1] == 1.0 && desc->offs[k] == desc->offs[k - 1]);
+    return true;
+  };
+  shardRow.assign(S + 1, K);
+  shardRow[0] = 0;
+  for (int s = 1; s < S; ++s) {
+    int k = std::max((int)((int64_t)K * s / S), shardRow[s - 1]);
+    while (k < K && !startsGroup(k)) ++k;
+    shardRow[s] = k;
+  }
+  colShard.assign((size_t)J * (S + 1), 0);
+  for (int j = 0; j < J; ++j) {
+    const int64_t b = desc->col_ptr[j], e = desc->col_ptr[j + 1];
+    int32_t* out = &colShard[(size_t)j * (S + 1)];
+    const int f = desc->fmt[j];
+    if (f == CCDH_DENSE || f == CCDH_INTERCEPT) {
+      for (int s = 0; s <= S; ++s) out[s] = shardRow[s];
+      continue;
+    }
+    int64_t p = b;
+    for (int s = 0; s <= S; ++s) {
+      while (p < e && desc->rows[p] < shardRow[s]) ++p;
+      out[s] = (int32_t)(p - b);
+    }
+  }
 }
 
 int dataCreate(const ccdh_data_desc* desc, ccdh_data** out, Bump* place) {
@@ -725,6 +768,12 @@ int dataCreate(const ccdh_data_desc* desc, ccdh_data** out, Bump* place) {
     rc = rc ? rc : devUpload(&d->stratStart, ss.data(), ss.size(), &acc, place);
     d->nSeg = (int64_t)segOff.size();
   }
+  if (!rc) {
+    std::vector<int32_t> colShard;
+    buildShards(desc, d->hStrata, d->hShardRow, colShard);
+    rc = devUpload(&d->shardRow, d->hShardRow.data(), d->hShardRow.size(), &acc, place);
+    rc = rc ? rc : devUpload(&d->colShard, colShard.data(), colShard.size(), &acc, place);
+  }
   if (rc) {
     dataDestroy(d);
     return rc;
@@ -736,7 +785,8 @@ int dataCreate(const ccdh_data_desc* desc, ccdh_data** out, Bump* place) {
 void dataDestroy(ccdh_data* d) {
   if (!d) return;
   void* ptrs[] = {d->colPtr, d->rows, d->vals, d->valsF, d->fmt, d->y, d->strata,
-                  d->segPtr, d->segOff, d->segKey, d->stratStart, d->nzKey};
+                  d->segPtr, d->segOff, d->segKey, d->stratStart, d->nzKey, d->shardRow,
+                  d->colShard};
   if (!d->placed)
     for (void* p : ptrs)
       if (p) (void)hipFree(p);

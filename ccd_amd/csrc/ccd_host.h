@@ -28,12 +28,15 @@ struct ccdh_data {
   int32_t* segKey = nullptr;
   int32_t* stratStart = nullptr;
   int32_t* nzKey = nullptr;
+  int32_t* shardRow = nullptr;
+  int32_t* colShard = nullptr;
   bool placed = false;  // device arrays carved from a caller's block (not freed here)
   // host copies of the small per-column / per-row vectors
   std::vector<int64_t> hColPtr, hSegPtr;
   std::vector<int8_t> hFmt;
   std::vector<double> hY, hOffs, hLogFact;
   std::vector<int32_t> hStrata;
+  std::vector<int32_t> hShardRow;
 
   ccdamd::DevX view() const;
   int64_t hSegCount(int j) const { return hSegPtr.empty() ? 0 : hSegPtr[j + 1] - hSegPtr[j]; }

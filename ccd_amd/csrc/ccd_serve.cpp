@@ -96,6 +96,9 @@ class Server final : public ccdh_server {
     precision = o.precision;
     maxSlots_ = o.max_slots > 0 ? o.max_slots : 1024;
     threads_ = o.threads > 0 ? o.threads : 256;
+    maxGang_ = o.max_gang > 0 ? o.max_gang : kShards;
+    capacity_ = slotGangCapacity(kMixed, threads_);
+    if (capacity_ <= 0) return setError(CCDH_E_DEVICE, "sweep kernel occupancy query failed");
     const double qms = o.quantum_ms > 0 ? o.quantum_ms : 20.0;
     quantumTicks_ = (long long)(qms * 1e5);  // device wall clock: 100 MHz
     size_t budget = (size_t)std::max<int64_t>(o.hbm_budget_bytes, 0);
@@ -109,7 +112,8 @@ class Server final : public ccdh_server {
     arena_.reset(arenaBase_, budget);
     if (hipMalloc((void**)&dSlots_, maxSlots_ * sizeof(Slot<R>)) != hipSuccess ||
         hipMalloc((void**)&dSt_, maxSlots_ * sizeof(FitStatus)) != hipSuccess ||
-        hipMalloc((void**)&dActive_, maxSlots_ * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void**)&dMap_, std::max(maxSlots_, capacity_) * sizeof(GangEntry)) != hipSuccess ||
+        hipMalloc((void**)&dXch_, maxSlots_ * sizeof(Xch)) != hipSuccess ||
         hipMalloc((void**)&dFinish_, maxSlots_ * sizeof(int32_t)) != hipSuccess ||
         hipMalloc((void**)&dOne_, sizeof(int32_t)) != hipSuccess)
       return setError(CCDH_E_NOMEM, "server tables");
@@ -237,13 +241,15 @@ class Server final : public ccdh_server {
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     ev0_ = ev1_ = nullptr;
-    void* ptrs[] = {arenaBase_, dSlots_, dSt_, dActive_, dFinish_, dOne_};
+    void* ptrs[] = {arenaBase_, dSlots_, dSt_, dMap_, dXch_, dFinish_, dOne_};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     arenaBase_ = nullptr;
     dSlots_ = nullptr;
     dSt_ = nullptr;
-    dActive_ = dFinish_ = dOne_ = nullptr;
+    dMap_ = nullptr;
+    dXch_ = nullptr;
+    dFinish_ = dOne_ = nullptr;
     if (hSt_) (void)hipHostFree(hSt_);
     hSt_ = nullptr;
     if (schedStream_) (void)hipStreamDestroy(schedStream_);
@@ -384,13 +390,24 @@ class Server final : public ccdh_server {
     }
   }
 
+  // One launch over every fit in flight; each fit gets a gang sized by how many
+  // fits share the device and by its own nonzeros.
   int runLaunch(const std::vector<int32_t>& active) {
-    const int n = (int)active.size();
-    if (hipMemcpyAsync(dActive_, active.data(), n * sizeof(int32_t), hipMemcpyHostToDevice,
+    const int nAct = (int)active.size();
+    map_.clear();
+    bool gangs = false;
+    for (int32_t slot : active) {
+      const Request* r = slotReq_[slot];
+      const int G = gangSize(nAct, capacity_, r->data->nnz, r->model, maxGang_, kGangMinNnz);
+      gangs |= G > 1;
+      for (int k = 0; k < G; ++k) map_.push_back(GangEntry{slot, k, G, 0});
+    }
+    const int n = (int)map_.size();
+    if (hipMemcpyAsync(dMap_, map_.data(), n * sizeof(GangEntry), hipMemcpyHostToDevice,
                        schedStream_) != hipSuccess ||
         hipEventRecord(ev0_, schedStream_) != hipSuccess)
       return setError(CCDH_E_DEVICE, "launch set-up");
-    int rc = slotLaunchSweeps(M_ANY, kMixed, dSlots_, dSt_, dActive_, n, threads_, 1 << 30,
+    int rc = slotLaunchSweeps(kMixed, dSlots_, dSt_, dMap_, n, dXch_, gangs, threads_, 1 << 30,
                               quantumTicks_, schedStream_);
     if (rc) return rc;
     if (hipEventRecord(ev1_, schedStream_) != hipSuccess ||
@@ -399,6 +416,8 @@ class Server final : public ccdh_server {
       return setError(CCDH_E_DEVICE, "launch status copy");
     const hipError_t e = hipStreamSynchronize(schedStream_);
     if (e != hipSuccess) return setError(CCDH_E_DEVICE, std::string("sweep launch: ") + hipGetErrorString(e));
+    for (int32_t slot : active)
+      if (hSt_[slot].fault) return setError(CCDH_E_DEVICE, "gang exchange timed out");
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ev0_, ev1_);
     std::lock_guard<std::mutex> g(mu_);
@@ -539,15 +558,19 @@ class Server final : public ccdh_server {
     cvQueue_.notify_all();
   }
 
+  static constexpr int64_t kGangMinNnz = 32768;
   ccdh_server_options opt_{};
-  int maxSlots_ = 0, threads_ = 0;
+  int maxSlots_ = 0, threads_ = 0, maxGang_ = 0, capacity_ = 0;
+  std::vector<GangEntry> map_;
+  GangEntry* dMap_ = nullptr;
+  Xch* dXch_ = nullptr;
   long long quantumTicks_ = 0;
   Arena arena_;
   char* arenaBase_ = nullptr;
   Slot<R>* dSlots_ = nullptr;
   FitStatus* dSt_ = nullptr;
   FitStatus* hSt_ = nullptr;
-  int32_t *dActive_ = nullptr, *dFinish_ = nullptr, *dOne_ = nullptr;
+  int32_t *dFinish_ = nullptr, *dOne_ = nullptr;
   hipStream_t schedStream_ = nullptr, admitStream_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   std::vector<Slot<R>> hSlots_;

@@ -42,7 +42,7 @@ struct FitStatus {
   int done;
   int N;      // groups of this fit (Cox: tie groups under its weights)
   int jNext;  // coordinate the current sweep resumes at (a launch's time quantum ran out)
-  int pad;
+  int fault;  // a gang exchange gave up waiting (the launch's results are void)
   // work counters (thread 0), read by the host for the bandwidth figure
   long long gatherNnz, gatherRuns, updateNnz, updateRuns, scanPids, objRows;
   // wall-clock ticks (thread 0) spent per phase: single-pass columns, long columns, objective
@@ -86,6 +86,19 @@ struct Slot {
   double fixedLL;
 };
 
+// Cross-workgroup exchange of one slot's gang: an arrival counter (zeroed
+// before each launch) and two alternating rounds of up to 4 values per rank.
+struct Xch {
+  unsigned cnt;
+  unsigned pad[63];
+  double v[2][kShards][4];
+};
+
+// Launch map entry: workgroup -> (slot, rank in the slot's gang, gang size).
+struct GangEntry {
+  int slot, rank, G, pad;
+};
+
 // Byte offsets of every array of one slot inside its block (256-byte aligned).
 struct SlotLayout {
   size_t xb, oeb, kw, heldout, denom, nw, numer, numer2, xjy, beta, bound, pvar, ptype, fixed,
@@ -114,11 +127,20 @@ int64_t maxColumnLength(const ccdh_data* d);
 // indices into `slots`).  model is a CCDH model code or M_ANY.
 int slotLaunchInit(int model, bool mixed, const void* slots, FitStatus* st,
                    const int32_t* active, int n, int threads, hipStream_t s);
+// One sweep launch over `n` workgroups described by `map` (device array).
 // quantumTicks > 0: a slot stops at the first coordinate boundary after that
 // many device clock ticks (100 MHz) and resumes there in the next launch.
-int slotLaunchSweeps(int model, bool mixed, const void* slots, FitStatus* st,
-                     const int32_t* active, int n, int threads, int sweeps,
-                     long long quantumTicks, hipStream_t s);
+// gangs: some slot runs on more than one workgroup (cooperative launch; the
+// exchange counters of `xch`, indexed by slot, are reset first).
+int slotLaunchSweeps(bool mixed, const void* slots, FitStatus* st, const GangEntry* map, int n,
+                     Xch* xch, bool gangs, int threads, int sweeps, long long quantumTicks,
+                     hipStream_t s);
+// Workgroups a cooperative sweep launch can hold at once.
+int slotGangCapacity(bool mixed, int threads);
+// Gang size for one of n slots sharing `capacity` workgroups: the largest
+// power of two <= min(kShards, maxGang) with n * G <= capacity and at least
+// minNnzPerRank of the slot's nonzeros per rank; Cox slots run on one workgroup.
+int gangSize(int n, int capacity, int64_t nnz, int model, int maxGang, int64_t minNnzPerRank);
 int slotLaunchFinish(int model, bool mixed, const void* slots, FitStatus* st,
                      const int32_t* active, int n, int threads, hipStream_t s);
 

@@ -174,10 +174,11 @@ int ccdh_fit_plain(ccdh_data_t data, const ccdh_fit_spec* spec, double* beta_out
 
 /* Batched engine options. tile_fits / threads = 0 picks tuned defaults. */
 typedef struct {
-  int threads;           /* workgroup size (64..1024) */
+  int threads;           /* workgroup size: 256 (<= 256) or 512 */
   int sweeps_per_sync;   /* sweeps launched between host convergence checks */
-  int use_graph;         /* capture the sweep launches in a hipGraph and replay it */
-  int _pad;
+  int use_graph;         /* replay the sweep launch from a hipGraph (only with max_gang 1) */
+  int max_gang;          /* workgroups one fit may use (1: one each; 0: 64) */
+  int64_t gang_min_nnz;  /* nonzeros per workgroup below which a fit's gang stops growing (0: 32768) */
 } ccdh_batch_options;
 
 /* Batched engine: n_fits fits over one data handle, device-resident sweeps.
@@ -229,8 +230,8 @@ double ccdh_cv_grid_point(double lower, double upper, int size, int step);
 typedef struct {
   int precision;            /* CCDH_FP64 or CCDH_MIXED, for every request */
   int max_slots;            /* fits in flight at once (0: 1024) */
-  int threads;              /* workgroup size per fit (0: 256) */
-  int _pad;
+  int threads;              /* workgroup size (0: 256) */
+  int max_gang;             /* workgroups one fit may use when the device has room (0: 64; 1: one) */
   double quantum_ms;        /* device time per fit per launch (0: 20 ms) */
   int64_t hbm_budget_bytes; /* arena size (0: half of the free device memory) */
 } ccdh_server_options;

@@ -95,3 +95,30 @@ def test_batch_mixed_precision(name):
     bc, ic = orc.fit(p, spec)
     bg, ig = api.fit_batch(api.Data(p, precision=api.MIXED), [spec])
     check_fit(bg[0], ig[0], bc, ic, rtol=2e-4, atol=2e-5, ll_rtol=1e-6, same_path=False)
+
+
+@pytest.mark.parametrize("name", ["lr", "poisson", "clr"])
+@pytest.mark.parametrize("gang", [2, 8, 64])
+def test_batch_gangs_match_oracle(name, gang):
+    """Fits run by gangs of workgroups (row shards, one exchange per coordinate)
+    match the oracle to the FP64 tolerance with the same iterations."""
+    p, prior, var, maxit = SYNTH[name]()
+    specs, held = cv_specs(p, prior, var, maxit, n_fold=2, grid=(1.0,))
+    d = api.Data(p)
+    bg, ig = api.fit_batch(d, specs, heldouts=held, max_gang=gang, gang_min_nnz=1)
+    for i, sp in enumerate(specs):
+        bc, ic = orc.fit(p, sp, heldout=held[i])
+        check_fit(bg[i], ig[i], bc, ic)
+        np.testing.assert_allclose(ig[i]["pred_log_likelihood"], ic["pred_log_likelihood"],
+                                   rtol=1e-8)
+
+
+def test_batch_gang_size_changes_only_rounding():
+    """The gang size changes only the order of the per-coordinate sums."""
+    p, prior, var, maxit = SYNTH["clr"]()
+    specs, _ = cv_specs(p, prior, var, maxit, n_fold=3, grid=(0.5,))
+    d = api.Data(p)
+    b1, i1 = api.fit_batch(d, specs, max_gang=1)
+    b16, i16 = api.fit_batch(d, specs, max_gang=16, gang_min_nnz=1)
+    np.testing.assert_allclose(b16, b1, rtol=1e-10, atol=1e-13)
+    assert [r["iterations"] for r in i16] == [r["iterations"] for r in i1]

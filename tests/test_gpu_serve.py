@@ -117,3 +117,14 @@ def test_serve_mixed_precision():
     for (p, sp), (b, r, _) in zip(reqs, got):
         bc, ic = orc.fit(p, sp)
         check_fit(b, r, bc, ic, rtol=2e-4, atol=2e-5, ll_rtol=1e-6, same_path=False)
+
+
+def test_serve_gangs_match_oracle():
+    """Served fits spread over gangs of workgroups (forced by a small gang
+    threshold is not available to the server, so use a large request)."""
+    p, prior, var, maxit = SYNTH["poisson"]()
+    sp = default_spec(p, prior=prior, variance=var, max_iterations=maxit)
+    with api.Server(quantum_ms=1.0) as srv:
+        b, r, info = srv.wait(srv.submit(p, sp))
+    bc, ic = orc.fit(p, sp)
+    check_fit(b, r, bc, ic)
