@@ -457,16 +457,30 @@ __device__ __forceinline__ void loadRecords(const DevX& d, int fmt, int64_t e, i
 }
 
 // ------------------------------------------------------------ gangs
+// XCD (accelerator complex die) this wave runs on.
+__device__ __forceinline__ int xccId() {
+  unsigned v;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+  return (int)(v & 0xfu);
+}
+
+__global__ void k_xcc_probe(int* out) {
+  if (threadIdx.x == 0) out[blockIdx.x] = xccId();
+}
+
 // A slot may be run by a gang of G workgroups (G a power of two <= kShards).
 // Rank r owns the rows of shards [r*S/G, (r+1)*S/G) -- their xBeta, exp(xBeta),
 // weights and (CLR) strata -- and walks only its part of every column.  Per
 // coordinate the ranks exchange their block-reduced (gradient, Hessian) through
 // the slot's exchange buffer and all sum them in rank order, so every rank takes
 // the same prior step and updates its own rows; no rank reads another's rows.
-// Exchanged values travel as agent-scope atomics (coherent across XCDs without
-// L2 write-back or invalidation); a rank's arrival counter increment follows
-// the completion of its stores.  Spins are bounded: a rank that waits longer
-// than kSpinLimit polls gives up and the launch reports the failure.
+// The coefficient, its bound and the end-of-quantum decision belong to the lead
+// rank (rank 0): only it reads and writes them in memory during a gang launch,
+// and it publishes them in the same exchange.  Exchanged values travel as
+// agent-scope atomics (coherent across XCDs without L2 write-back or
+// invalidation); a rank's arrival counter increment follows the completion of
+// its stores.  Spins are bounded: a rank that waits longer than kSpinLimit polls
+// gives up and the launch reports the failure.
 struct Gang {
   int rank, G;
   int rowLo, rowHi;  // rows owned by this rank
@@ -485,17 +499,24 @@ __device__ __forceinline__ void storeShared(double* p, double v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Sum NV block totals over the gang (rank order); every thread gets the sums.
-template <int NV>
-__device__ __forceinline__ bool gangSum(Gang& g, double (&v)[NV]) {
-  static_assert(NV <= 4, "exchange carries at most 4 values");
-  __shared__ double sOut[2][4];
+// Sum NS block totals over the gang (in rank order) and receive NL values from
+// the lead rank; every thread of every rank gets the same results.  `sum` holds
+// this rank's totals (the same in every thread), `lead` the lead rank's values
+// (read in thread 0 of rank 0 only).
+template <int NS, int NL>
+__device__ __forceinline__ bool gangExchange(Gang& g, double (&sum)[NS], double (&lead)[NL]) {
+  static_assert(NS <= kXchSum && NL <= kXchLead, "exchange too wide");
+  __shared__ double sOut[2][kXchSum + kXchLead];
   __shared__ int sOk[2];
   const int par = (int)(g.seq & 1u);
   if (threadIdx.x == 0) {
     double* mine = &g.x->v[par][g.rank][0];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) storeShared(mine + i, v[i]);
+    for (int i = 0; i < NS; ++i) storeShared(mine + i, sum[i]);
+    if (g.rank == 0) {
+#pragma unroll
+      for (int i = 0; i < NL; ++i) storeShared(mine + kXchSum + i, lead[i]);
+    }
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __builtin_amdgcn_s_waitcnt(0);  // the values have reached the coherence point
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -517,15 +538,19 @@ __device__ __forceinline__ bool gangSum(Gang& g, double (&v)[NV]) {
   const bool ok = sOk[par] != 0;
   if (threadIdx.x < kWave) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) {
+    for (int i = 0; i < NS; ++i) {
       double t = (ok && (int)threadIdx.x < g.G) ? loadShared(&g.x->v[par][threadIdx.x][i]) : 0.0;
       t = waveSum(t);
       if (threadIdx.x == 0) sOut[par][i] = t;
     }
+    if ((int)threadIdx.x < NL)
+      sOut[par][kXchSum + threadIdx.x] = ok ? loadShared(&g.x->v[par][0][kXchSum + threadIdx.x]) : 0.0;
   }
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < NV; ++i) v[i] = sOut[par][i];
+  for (int i = 0; i < NS; ++i) sum[i] = sOut[par][i];
+#pragma unroll
+  for (int i = 0; i < NL; ++i) lead[i] = sOut[par][kXchSum + i];
   g.seq++;
   g.ok = g.ok && ok;
   return ok;
@@ -565,8 +590,9 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus& cs, LDS& S,
     const int64_t nj = e - b;      // this rank's nonzeros
     const int64_t njAll = ce - cb;  // the column's
     const int fmt = d.fmt[j];
-    const double bj = multi ? loadShared(&s.beta[j]) : s.beta[j];
-    double bnd = multi ? loadShared(&s.bound[j]) : s.bound[j];
+    // in a gang, only the lead rank's reads count (it publishes them)
+    double bj = s.beta[j];
+    double bnd = s.bound[j];
     const double pv = s.pvar[j], xy = s.xjy[j];
     const int pt = s.ptype[j];
     const long long runs = MODEL == M_CLR ? (long long)(d.segPtr[j + 1] - d.segPtr[j]) : njAll;
@@ -579,12 +605,12 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus& cs, LDS& S,
     auto reduce = [&]() -> bool {
       blockSum<2>(v);
       if (!multi) return true;
-      double w[3] = {v[0], v[1],
-                     (lead && threadIdx.x == 0 && deadline > 0 && wall_clock64() >= deadline) ? 1.0 : 0.0};
-      if (!gangSum<3>(gg, w)) return false;
-      v[0] = w[0];
-      v[1] = w[1];
-      stop = w[2] != 0.0;
+      double ld[3] = {bj, bnd,
+                      (deadline > 0 && wall_clock64() >= deadline) ? 1.0 : 0.0};
+      if (!gangExchange<2, 3>(gg, v, ld)) return false;
+      bj = ld[0];
+      bnd = ld[1];
+      stop = ld[2] != 0.0;
       return true;
     };
     auto step = [&]() -> double {
@@ -595,13 +621,8 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus& cs, LDS& S,
       }
       const double dl = applyBounds(priorDelta(pt, pv, g, h, bj), &bnd);
       if (threadIdx.x == 0 && lead) {
-        if (multi) {
-          storeShared(&s.bound[j], bnd);
-          if (dl != 0.0) storeShared(&s.beta[j], bj + dl);
-        } else {
-          s.bound[j] = bnd;
-          if (dl != 0.0) s.beta[j] = bj + dl;
-        }
+        s.bound[j] = bnd;
+        if (dl != 0.0) s.beta[j] = bj + dl;
       }
       return dl;
     };
@@ -751,10 +772,10 @@ __device__ __forceinline__ double fitGradObjective(const Slot<R>& s, const Part&
 }
 
 __device__ __forceinline__ double priorPart(const double* beta, const double* pvar,
-                                            const int8_t* ptype, int J, bool shared) {
+                                            const int8_t* ptype, int J) {
   double v[1] = {0.0};
   for (int j = threadIdx.x; j < J; j += blockDim.x)
-    v[0] += priorLogDensity(ptype[j], pvar[j], shared ? loadShared(&beta[j]) : beta[j]);
+    v[0] += priorLogDensity(ptype[j], pvar[j], beta[j]);
   blockSum<1>(v);
   return v[0];
 }
@@ -769,7 +790,7 @@ __device__ __forceinline__ double fitObjective(const Slot<R>& s) {
   if (ct == 3) return 0.0;
   const double ll = fitLogLik<MODEL, R>(s, all) + s.fixedLL;
   if (ct == 2) return ll;
-  return ll + priorPart(s.beta, s.pvar, s.ptype, s.J, false);
+  return ll + priorPart(s.beta, s.pvar, s.ptype, s.J);
 }
 
 // The same objective summed over a gang; `stop` carries the lead rank's
@@ -783,17 +804,18 @@ __device__ __forceinline__ bool gangObjective(const Slot<R>& s, Gang& gg, long l
     pt.grpLo = s.d.strata[gg.rowLo];
     pt.grpHi = s.d.strata[gg.rowHi - 1] + 1;
   }
-  double v[3] = {0.0, 0.0, 0.0};
+  double v[1] = {0.0};
   if (ct == 0) v[0] = fitGradObjective<R>(s, pt);
   if (ct == 1 || ct == 2) v[0] = fitLogLik<MODEL, R>(s, pt);
-  if (ct == 1 && gg.rank == 0) v[1] = priorPart(s.beta, s.pvar, s.ptype, s.J, true);
-  v[2] = (gg.rank == 0 && threadIdx.x == 0 && deadline > 0 && wall_clock64() >= deadline) ? 1.0 : 0.0;
-  if (!gangSum<3>(gg, v)) return false;
-  *stop = v[2] != 0.0;
+  // the coefficients belong to the lead rank: it adds the log prior
+  double ld[2] = {0.0, (deadline > 0 && wall_clock64() >= deadline) ? 1.0 : 0.0};
+  if (ct == 1 && gg.rank == 0) ld[0] = priorPart(s.beta, s.pvar, s.ptype, s.J);
+  if (!gangExchange<1, 2>(gg, v, ld)) return false;
+  *stop = ld[1] != 0.0;
   if (ct == 0) *obj = v[0];
   else if (ct == 3) *obj = 0.0;
   else if (ct == 2) *obj = v[0] + s.fixedLL;
-  else *obj = v[0] + s.fixedLL + v[1];
+  else *obj = v[0] + s.fixedLL + ld[0];
   return true;
 }
 
@@ -1019,6 +1041,20 @@ __global__ void __launch_bounds__(BLOCK)
   gg.ok = true;
   gg.x = ge.G > 1 ? xch + f : nullptr;
   if (ge.G > 1) {
+    // a gang shares one XCD (one L2): check before touching any state
+    const double x = (double)xccId();
+    double v[2] = {x, x * x};
+    double ld[1] = {0.0};
+    if (!gangExchange<2, 1>(gg, v, ld)) {
+      if (threadIdx.x == 0) st[f].fault = 1;
+      return;
+    }
+    if (v[0] * v[0] != v[1] * ge.G) {
+      if (threadIdx.x == 0) st[f].fault = 2;
+      return;
+    }
+  }
+  if (ge.G > 1) {
     gg.s0 = ge.rank * s.d.nShards / ge.G;
     gg.s1 = (ge.rank + 1) * s.d.nShards / ge.G;
     gg.rowLo = s.d.shardRow[gg.s0];
@@ -1105,10 +1141,66 @@ int slotLaunchSweeps(bool mixed, const void* slots, FitStatus* st, const GangEnt
   return 0;
 }
 
+static int probeXcds() {
+  const int n = 512;
+  int* d = nullptr;
+  if (hipMalloc((void**)&d, n * sizeof(int)) != hipSuccess) return 0;
+  std::vector<int> h(n, -1);
+  hipStream_t s = nullptr;
+  int nx = 0;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess) {
+    k_xcc_probe<<<n, 64, 0, s>>>(d);
+    if (hipMemcpyAsync(h.data(), d, n * sizeof(int), hipMemcpyDeviceToHost, s) == hipSuccess &&
+        hipStreamSynchronize(s) == hipSuccess) {
+      for (int v : h) nx = std::max(nx, v + 1);
+      bool roundRobin = nx > 0;
+      for (int i = 0; i < n && roundRobin; ++i) roundRobin = h[i] == i % nx;
+      if (!roundRobin) nx = 0;
+    }
+    (void)hipStreamDestroy(s);
+  }
+  (void)hipFree(d);
+  return nx;
+}
+
+int xcdCount() {
+  static const int n = probeXcds();  // once per process (thread-safe initialisation)
+  return n;
+}
+
+bool layoutGangs(const std::vector<GangEntry>& gangs, int capacity, int nX,
+                 std::vector<GangEntry>& map) {
+  map.clear();
+  if (nX <= 0) return false;
+  const int perBin = capacity / nX;
+  std::vector<int> order(gangs.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return gangs[a].G > gangs[b].G; });
+  std::vector<int> fill(nX, 0);
+  std::vector<std::vector<GangEntry>> bins(nX);
+  for (int i : order) {
+    const GangEntry& g = gangs[i];
+    int best = -1;
+    for (int b = 0; b < nX; ++b)
+      if (fill[b] + g.G <= perBin && (best < 0 || fill[b] < fill[best])) best = b;
+    if (best < 0) return false;
+    for (int r = 0; r < g.G; ++r) bins[best].push_back(GangEntry{g.slot, r, g.G, 0});
+    fill[best] += g.G;
+  }
+  const int depth = *std::max_element(fill.begin(), fill.end());
+  map.assign((size_t)depth * nX, GangEntry{-1, 0, 1, 0});
+  for (int b = 0; b < nX; ++b)
+    for (int k = 0; k < fill[b]; ++k) map[(size_t)k * nX + b] = bins[b][k];
+  return true;
+}
+
 int gangSize(int n, int capacity, int64_t nnz, int model, int maxGang, int64_t minNnzPerRank) {
   if (model == CCDH_COX || n <= 0) return 1;
+  const int nX = xcdCount();
+  if (nX <= 0) return 1;
   int G = 1;
-  const int cap = std::min(kShards, std::max(1, maxGang));
+  const int cap = std::min(std::min(kShards, capacity / nX), std::max(1, maxGang));
   while (G * 2 <= cap && (int64_t)n * G * 2 <= capacity && nnz / (G * 2) >= minNnzPerRank) G *= 2;
   return G;
 }
@@ -1352,6 +1444,7 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
   const SlotLayout L = slotLayout(model, d->K, J, maxN, maxColumnLength(d), anyHeld, sizeof(R));
   const int capacity = slotGangCapacity(mixed, threads);
   if (capacity <= 0) return setError(CCDH_E_DEVICE, "sweep kernel occupancy query failed");
+  (void)xcdCount();
   char* arena = nullptr;
   Slot<R>* dslots = nullptr;
   FitStatus* dst = nullptr;
@@ -1390,14 +1483,19 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
   int mapN = 0;
   bool mapGangs = false;
   std::vector<char> running(F, 1);
+  int gangCap = maxGang;
   auto buildMap = [&]() -> int {
     int nAct = 0;
     for (int f = 0; f < F; ++f) nAct += running[f];
-    const int G = gangSize(nAct, capacity, d->nnz, model, maxGang, minNnz);
-    map.clear();
+    int G = gangSize(nAct, capacity, d->nnz, model, gangCap, minNnz);
+    std::vector<GangEntry> gangs;
     for (int f = 0; f < F; ++f)
-      if (running[f] || useGraph)
-        for (int r = 0; r < G; ++r) map.push_back(GangEntry{f, r, G, 0});
+      if (running[f] || useGraph) gangs.push_back(GangEntry{f, 0, G, 0});
+    if (G > 1 && !layoutGangs(gangs, capacity, xcdCount(), map)) G = 1;
+    if (G == 1) {
+      map.clear();
+      for (const GangEntry& g : gangs) map.push_back(GangEntry{g.slot, 0, 1, 0});
+    }
     mapN = (int)map.size();
     mapGangs = G > 1;
     HIP_TRY(hipMemcpyAsync(dmap, map.data(), mapN * sizeof(GangEntry), hipMemcpyHostToDevice, s));
@@ -1458,8 +1556,19 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
     }
     const double now = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     bool allDone = true, retry = false;
-    for (int f = 0; f < F; ++f)
-      if (o.hst[f].fault) return setError(CCDH_E_DEVICE, "gang exchange timed out");
+    bool misplaced = false;
+    for (int f = 0; f < F; ++f) {
+      if (o.hst[f].fault == 1) return setError(CCDH_E_DEVICE, "gang exchange timed out");
+      if (o.hst[f].fault == 2) {  // a gang spanned XCDs: it backed off untouched
+        misplaced = true;
+        o.hst[f].fault = 0;
+        HIP_TRY(hipMemcpyAsync(&dst[f].fault, &o.hst[f].fault, sizeof(int), hipMemcpyHostToDevice, s));
+      }
+    }
+    if (misplaced) {
+      gangCap = 1;
+      HIP_TRY(hipStreamSynchronize(s));
+    }
     for (int f = 0; f < F; ++f) {
       FitStatus& h = o.hst[f];
       running[f] = !h.done;

@@ -99,6 +99,7 @@ class Server final : public ccdh_server {
     maxGang_ = o.max_gang > 0 ? o.max_gang : kShards;
     capacity_ = slotGangCapacity(kMixed, threads_);
     if (capacity_ <= 0) return setError(CCDH_E_DEVICE, "sweep kernel occupancy query failed");
+    (void)xcdCount();
     const double qms = o.quantum_ms > 0 ? o.quantum_ms : 20.0;
     quantumTicks_ = (long long)(qms * 1e5);  // device wall clock: 100 MHz
     size_t budget = (size_t)std::max<int64_t>(o.hbm_budget_bytes, 0);
@@ -394,13 +395,18 @@ class Server final : public ccdh_server {
   // fits share the device and by its own nonzeros.
   int runLaunch(const std::vector<int32_t>& active) {
     const int nAct = (int)active.size();
-    map_.clear();
+    std::vector<GangEntry> want;
     bool gangs = false;
     for (int32_t slot : active) {
       const Request* r = slotReq_[slot];
       const int G = gangSize(nAct, capacity_, r->data->nnz, r->model, maxGang_, kGangMinNnz);
       gangs |= G > 1;
-      for (int k = 0; k < G; ++k) map_.push_back(GangEntry{slot, k, G, 0});
+      want.push_back(GangEntry{slot, 0, G, 0});
+    }
+    if (gangs && !layoutGangs(want, capacity_, xcdCount(), map_)) gangs = false;
+    if (!gangs) {
+      map_.clear();
+      for (int32_t slot : active) map_.push_back(GangEntry{slot, 0, 1, 0});
     }
     const int n = (int)map_.size();
     if (hipMemcpyAsync(dMap_, map_.data(), n * sizeof(GangEntry), hipMemcpyHostToDevice,
@@ -416,8 +422,17 @@ class Server final : public ccdh_server {
       return setError(CCDH_E_DEVICE, "launch status copy");
     const hipError_t e = hipStreamSynchronize(schedStream_);
     if (e != hipSuccess) return setError(CCDH_E_DEVICE, std::string("sweep launch: ") + hipGetErrorString(e));
-    for (int32_t slot : active)
-      if (hSt_[slot].fault) return setError(CCDH_E_DEVICE, "gang exchange timed out");
+    for (int32_t slot : active) {
+      if (hSt_[slot].fault == 1) return setError(CCDH_E_DEVICE, "gang exchange timed out");
+      if (hSt_[slot].fault == 2) {  // a gang spanned XCDs and backed off untouched
+        maxGang_ = 1;
+        hSt_[slot].fault = 0;
+        if (hipMemcpyAsync(&dSt_[slot].fault, &hSt_[slot].fault, sizeof(int), hipMemcpyHostToDevice,
+                           schedStream_) != hipSuccess ||
+            hipStreamSynchronize(schedStream_) != hipSuccess)
+          return setError(CCDH_E_DEVICE, "fault reset");
+      }
+    }
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ev0_, ev1_);
     std::lock_guard<std::mutex> g(mu_);

@@ -14,6 +14,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "../../include/ccdhip.h"
 #include "ccd_device.h"
@@ -87,11 +88,13 @@ struct Slot {
 };
 
 // Cross-workgroup exchange of one slot's gang: an arrival counter (zeroed
-// before each launch) and two alternating rounds of up to 4 values per rank.
+// before each launch) and two alternating rounds of values per rank: 4 that
+// are summed over the gang, then 4 that only the lead rank publishes.
+constexpr int kXchSum = 4, kXchLead = 4;
 struct Xch {
   unsigned cnt;
   unsigned pad[63];
-  double v[2][kShards][4];
+  double v[2][kShards][kXchSum + kXchLead];
 };
 
 // Launch map entry: workgroup -> (slot, rank in the slot's gang, gang size).
@@ -137,6 +140,13 @@ int slotLaunchSweeps(bool mixed, const void* slots, FitStatus* st, const GangEnt
                      hipStream_t s);
 // Workgroups a cooperative sweep launch can hold at once.
 int slotGangCapacity(bool mixed, int threads);
+// XCDs of the device when workgroups are dealt to them round-robin by index
+// (measured once with a probe kernel), else 0 (gangs are then not used).
+int xcdCount();
+// Map of one launch with every gang inside one XCD: gangs[i] = {slot, -, G};
+// workgroup k*nX + b runs on XCD b.  False when they do not fit.
+bool layoutGangs(const std::vector<GangEntry>& gangs, int capacity, int nX,
+                 std::vector<GangEntry>& map);
 // Gang size for one of n slots sharing `capacity` workgroups: the largest
 // power of two <= min(kShards, maxGang) with n * G <= capacity and at least
 // minNnzPerRank of the slot's nonzeros per rank; Cox slots run on one workgroup.

@@ -122,3 +122,33 @@ def test_batch_gang_size_changes_only_rounding():
     b16, i16 = api.fit_batch(d, specs, max_gang=16, gang_min_nnz=1)
     np.testing.assert_allclose(b16, b1, rtol=1e-10, atol=1e-13)
     assert [r["iterations"] for r in i16] == [r["iterations"] for r in i1]
+
+
+def test_batch_gangs_with_concurrent_kernels():
+    """Gang fits stay exact while another host thread launches kernels on other
+    streams (every gang runs inside one XCD, so no cache line is shared between
+    XCDs)."""
+    import threading
+    p, prior, var, maxit = SYNTH["poisson"]()
+    specs = [default_spec(p, prior=prior, variance=var * s, max_iterations=maxit)
+             for s in (1.0, 0.1)]
+    refs = [orc.fit(p, sp) for sp in specs]
+    q, qprior, qvar, _ = SYNTH["lr"]()
+    qspec = default_spec(q, prior=qprior, variance=qvar)
+    dp, dq = api.Data(p), api.Data(q)
+    stop = threading.Event()
+
+    def other_work():
+        while not stop.is_set():
+            api.fit_plain(dq, qspec)
+
+    th = threading.Thread(target=other_work)
+    th.start()
+    try:
+        for _ in range(3):
+            bg, ig = api.fit_batch(dp, specs, max_gang=4, gang_min_nnz=1)
+            for i, (bc, ic) in enumerate(refs):
+                check_fit(bg[i], ig[i], bc, ic)
+    finally:
+        stop.set()
+        th.join()

@@ -120,8 +120,8 @@ def test_serve_mixed_precision():
 
 
 def test_serve_gangs_match_oracle():
-    """Served fits spread over gangs of workgroups (forced by a small gang
-    threshold is not available to the server, so use a large request)."""
+    """A request large enough for a gang (about 240k nonzeros: 4 workgroups)
+    matches the oracle when served alone with a short quantum."""
     p, prior, var, maxit = SYNTH["poisson"]()
     sp = default_spec(p, prior=prior, variance=var, max_iterations=maxit)
     with api.Server(quantum_ms=1.0) as srv:
