@@ -476,11 +476,13 @@ __global__ void k_xcc_probe(int* out) {
 // the same prior step and updates its own rows; no rank reads another's rows.
 // The coefficient, its bound and the end-of-quantum decision belong to the lead
 // rank (rank 0): only it reads and writes them in memory during a gang launch,
-// and it publishes them in the same exchange.  Exchanged values travel as
-// agent-scope atomics (coherent across XCDs without L2 write-back or
-// invalidation); a rank's arrival counter increment follows the completion of
-// its stores.  Spins are bounded: a rank that waits longer than kSpinLimit polls
-// gives up and the launch reports the failure.
+// and it publishes them in the same exchange.  A gang runs inside one XCD.
+// Exchanged values travel as agent-scope atomics: a rank publishes its values
+// by returning atomic swaps and increments the slot's arrival counter only
+// after the swaps have returned (a plain or atomic store may be acknowledged
+// before it is performed, and a waiter could then see the arrival first).
+// Spins are bounded: a rank that waits longer than kSpinLimit polls gives up
+// and the launch reports the failure.
 struct Gang {
   int rank, G;
   int rowLo, rowHi;  // rows owned by this rank
@@ -495,8 +497,11 @@ constexpr long long kSpinLimit = 1ll << 24;
 __device__ __forceinline__ double loadShared(const double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void storeShared(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Publish by atomic swap: its return means the value has been performed at the
+// coherence point (a store may be acknowledged before that), so an arrival
+// issued after the returns cannot overtake it.
+__device__ __forceinline__ double publishShared(double* p, double v) {
+  return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Sum NS block totals over the gang (in rank order) and receive NL values from
@@ -511,14 +516,17 @@ __device__ __forceinline__ bool gangExchange(Gang& g, double (&sum)[NS], double 
   const int par = (int)(g.seq & 1u);
   if (threadIdx.x == 0) {
     double* mine = &g.x->v[par][g.rank][0];
+    double chk = 0.0;
 #pragma unroll
-    for (int i = 0; i < NS; ++i) storeShared(mine + i, sum[i]);
+    for (int i = 0; i < NS; ++i) chk += publishShared(mine + i, sum[i]);
     if (g.rank == 0) {
 #pragma unroll
-      for (int i = 0; i < NL; ++i) storeShared(mine + kXchSum + i, lead[i]);
+      for (int i = 0; i < NL; ++i) chk += publishShared(mine + kXchSum + i, lead[i]);
     }
+    // consume the swaps' results, so they are returning atomics
+    asm volatile("; swap results %0" : : "v"(chk));
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0);  // the values have reached the coherence point
+    __builtin_amdgcn_s_waitcnt(0);  // ... and have returned
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __hip_atomic_fetch_add(&g.x->cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned target = (unsigned)g.G * (g.seq + 1u);
@@ -574,12 +582,28 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus& cs, LDS& S,
   const bool lead = gg.rank == 0;
   long long cG = 0, cGR = 0, cU = 0, cUR = 0, tS = 0, tL = 0, nL = 0;
   int jEnd = J;
+  // Records are indexed by position in the column, and a rank's positions move
+  // from column to column: a rank may already store coordinate k+1's records
+  // while another still reads coordinate k's.  A gang alternates two buffers
+  // (no rank reaches coordinate k+2 before every rank has finished k).
+  unsigned recSel = 0;
   for (int j = j0; j < J; ++j) {
     if (!multi && j > j0 && (j & 3) == 0 && quantumOver(deadline)) {
       jEnd = j;
       break;
     }
     if (s.fixed[j]) continue;
+    Records<R> rec = s.rec;
+    if (multi && (recSel++ & 1u)) {
+      rec.row += s.recStride;
+      rec.o += s.recStride;
+      rec.xo += s.recStride;
+      if (MODEL == M_CLR) {
+        rec.key += s.recStride;
+        rec.flags += s.recStride;
+        rec.den += s.recStride;
+      }
+    }
     const int64_t cb = d.colPtr[j], ce = d.colPtr[j + 1];
     int64_t b = cb, e = ce;
     if (multi) {
@@ -641,7 +665,7 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus& cs, LDS& S,
       for (int64_t c0 = b; c0 < e; c0 += CHL) {
         Lanes<kITL, R> L;
         passLoad<MODEL, kITL, R>(s, fmt, cb, e, c0, useW, L);
-        storeRecords<MODEL, kITL, R>(s.rec, c0 - cb, L);
+        storeRecords<MODEL, kITL, R>(rec, c0 - cb, L);
         passGather<MODEL, kITL, R>(fmt, S, (int)min(CHL, e - c0), staged, L, useW, v);
       }
       if (!reduce()) return -1;
@@ -651,7 +675,7 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus& cs, LDS& S,
         __syncthreads();  // carry reset visible; records were written by these lanes
         for (int64_t c0 = b; c0 < e; c0 += CHL) {
           Lanes<kITL, R> L;
-          loadRecords<MODEL, kITL, R>(d, fmt, e, c0, c0 - cb, s.rec, L);
+          loadRecords<MODEL, kITL, R>(d, fmt, e, c0, c0 - cb, rec, L);
           passUpdate<MODEL, kITL, R>(s, S, (int)min(CHL, e - c0), staged, L, delta);
         }
         __syncthreads();
@@ -1265,13 +1289,15 @@ SlotLayout slotLayout(int model, int K, int J, int nCap, int64_t maxLen, bool he
   L.pvar = take((size_t)J * 8);
   L.ptype = take((size_t)J);
   L.fixed = take((size_t)J);
-  L.recRow = cox ? kNone : take(nR * 4);
-  L.recO = cox ? kNone : take(nR * rs);
-  L.recXo = cox ? kNone : take(nR * rs);
+  // two record buffers: a gang alternates them by coordinate (see cycleFit)
+  L.recStride = (int64_t)nR;
+  L.recRow = cox ? kNone : take(2 * nR * 4);
+  L.recO = cox ? kNone : take(2 * nR * rs);
+  L.recXo = cox ? kNone : take(2 * nR * rs);
   const bool clr = model == CCDH_CONDITIONAL_LOGISTIC;
-  L.recKey = clr ? take(nR * 4) : kNone;
-  L.recFlags = clr ? take(nR) : kNone;
-  L.recDen = clr ? take(nR * 8) : kNone;
+  L.recKey = clr ? take(2 * nR * 4) : kNone;
+  L.recFlags = clr ? take(2 * nR) : kNone;
+  L.recDen = clr ? take(2 * nR * 8) : kNone;
   L.bytes = off;
   return L;
 }
@@ -1330,6 +1356,7 @@ int slotPrepare(const ccdh_data* d, const ccdh_fit_spec& sp, char* block, const 
   x.rec.o = at<R>(block, L.recO);
   x.rec.xo = at<R>(block, L.recXo);
   x.rec.den = at<double>(block, L.recDen);
+  x.recStride = L.recStride;
   x.ctl = FitCtl{sp.tolerance, sp.initial_bound, sp.max_iterations, sp.convergence_type,
                  sp.weights != nullptr, 0};
   // groups

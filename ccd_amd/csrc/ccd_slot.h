@@ -83,6 +83,7 @@ struct Slot {
   int32_t* pidFirst;
   uint8_t* reset;
   Records<R> rec;
+  int64_t recStride;  // records per buffer; the second buffer follows the first
   FitCtl ctl;
   double fixedLL;
 };
@@ -106,6 +107,7 @@ struct GangEntry {
 struct SlotLayout {
   size_t xb, oeb, kw, heldout, denom, nw, numer, numer2, xjy, beta, bound, pvar, ptype, fixed,
       pid, pidFirst, reset, recRow, recKey, recFlags, recO, recXo, recDen;
+  int64_t recStride;
   size_t bytes;
 };
 

@@ -152,3 +152,19 @@ def test_batch_gangs_with_concurrent_kernels():
     finally:
         stop.set()
         th.join()
+
+
+def test_batch_gangs_long_columns_many_fits():
+    """Gangs on heavy Zipf columns (several passes per rank, rank positions that
+    move from column to column) with many fits in flight agree with one
+    workgroup per fit."""
+    from ccd_amd import synth
+    p, _ = synth.lr_zipf(K=120_000, P=600, per_row=20, seed=9)
+    specs = [default_spec(p, prior=PRIOR_LAPLACE, variance=v, max_iterations=3)
+             for v in np.geomspace(0.01, 1.0, 48)]
+    d = api.Data(p)
+    b1, i1 = api.fit_batch(d, specs, max_gang=1)
+    for _ in range(2):
+        b2, i2 = api.fit_batch(d, specs, max_gang=2, gang_min_nnz=1)
+        np.testing.assert_allclose(b2, b1, rtol=1e-10, atol=1e-13)
+        assert [r["iterations"] for r in i2] == [r["iterations"] for r in i1]
