@@ -252,6 +252,7 @@ def main():
         f"{stats['ticks_single'] * tick:.1f}s, long columns {stats['ticks_long'] * tick:.1f}s "
         f"({stats['long_columns']} visits), objective {stats['ticks_objective'] * tick:.1f}s; "
         f"sweeps {stats['sweeps']}, gathered nnz {stats['gather_nnz']}, updated nnz {stats['update_nnz']}")
+    log("launch timeline (ms, fits in flight, workgroups per fit):", stats["timeline"])
     mx = int(iters.max())
     log(f"slowest fit: {mx} sweeps in {lat.max():.2f}s -> {lat.max() / mx * 1e3:.1f} ms/sweep")
 

@@ -73,10 +73,16 @@ class BatchStats(ctypes.Structure):
         "launches", "sweeps", "sweep_rounds", "gather_nnz", "gather_runs", "update_nnz",
         "update_runs", "scan_pids", "objective_rows", "ticks_single", "ticks_long",
         "ticks_objective", "long_columns")] + [
-        ("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double)]
+        ("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
+        ("launch_ms", ctypes.c_float * 128), ("launch_fits", ctypes.c_int32 * 128),
+        ("launch_gang", ctypes.c_int32 * 128)]
 
     def as_dict(self):
-        return {n: getattr(self, n) for n, _ in self._fields_}
+        out = {n: getattr(self, n) for n, _ in self._fields_[:15]}
+        k = min(128, self.launches)
+        out["timeline"] = [(round(self.launch_ms[i], 2), self.launch_fits[i], self.launch_gang[i])
+                           for i in range(k)]
+        return out
 
 
 class BatchOptions(ctypes.Structure):

@@ -67,7 +67,6 @@ __device__ __forceinline__ bool coxHead(const DevX& d, int fmt, int64_t b, int64
 template <typename R>
 __device__ __forceinline__ void ghCox(const Slot<R>& s, int j, int fmt, int useW, double (&v)[2]) {
   const DevX& d = s.d;
-  const R* oeb = s.oeb;
   const double* kw = s.kw;
   const int32_t* pid = s.pid;
   const int N = s.N;
@@ -85,7 +84,7 @@ __device__ __forceinline__ void ghCox(const Slot<R>& s, int j, int fmt, int useW
       if (pq >= N) continue;
       if (pq != pp) break;
       const double x = (double)colValue<R>(d, fmt, q);
-      const double o = (double)oeb[k];
+      const double o = (double)s.row[k].e;
       n1 += useW ? kw[k] * (o * x) : o * x;
       if (!ind) n2 += useW ? kw[k] * (o * x * x) : o * x * x;
     }
@@ -111,7 +110,7 @@ __device__ __forceinline__ void ghCox(const Slot<R>& s, int j, int fmt, int useW
       x.v[0] = x.v[1] = x.v[2] = 0.0;
       if (i < N) {
         x.f = s.reset[i];
-        x.v[0] = s.denom[i];
+        x.v[0] = s.grp[i].den;
         x.v[1] = numer[i];
         x.v[2] = ind ? 0.0 : numer2[i];
       }
@@ -127,9 +126,10 @@ __device__ __forceinline__ void ghCox(const Slot<R>& s, int j, int fmt, int useW
       run = segCombine(run, items[it]);
       const double accD = run.v[0];
       const double t = run.v[1] / accD;
-      const double g = s.nw[i] * t;
+      const double wn = s.grp[i].nw;
+      const double g = wn * t;
       v[0] += g;
-      v[1] += ind ? g * (1.0 - t) : s.nw[i] * (run.v[2] / accD - t * t);
+      v[1] += ind ? g * (1.0 - t) : wn * (run.v[2] / accD - t * t);
       if (items[it].v[1] != 0.0) numer[i] = 0.0;
       if (!ind && items[it].v[2] != 0.0) numer2[i] = 0.0;
     }
@@ -140,8 +140,7 @@ __device__ __forceinline__ void ghCox(const Slot<R>& s, int j, int fmt, int useW
 template <typename R>
 __device__ __forceinline__ void updCox(const Slot<R>& s, int j, int fmt, int useW, double delta) {
   const DevX& d = s.d;
-  R* xb = s.xb;
-  R* oeb = s.oeb;
+  RowSt<R>* rowSt = s.row;
   const double* kw = s.kw;
   const int32_t* pid = s.pid;
   const int N = s.N;
@@ -152,28 +151,27 @@ __device__ __forceinline__ void updCox(const Slot<R>& s, int j, int fmt, int use
     if (!coxHead<R>(d, fmt, b, p, pid, N, &pp)) {
       if (pp >= N) {
         const int k = colRow(d, fmt, p, b);
-        const R nx = xb[k] + rd * colValue<R>(d, fmt, p);
-        xb[k] = nx;
-        oeb[k] = rexp(nx);
+        const R nx = rowSt[k].x + rd * colValue<R>(d, fmt, p);
+        rowSt[k] = RowSt<R>{nx, rexp(nx)};
       }
       continue;
     }
-    double den = s.denom[pp];
+    double den = s.grp[pp].den;
     for (int64_t q = p; q < e; ++q) {
       const int k = colRow(d, fmt, q, b);
       const int pq = pid[k];
       if (pq >= N) continue;
       if (pq != pp) break;
-      const R nx = xb[k] + rd * colValue<R>(d, fmt, q);
-      xb[k] = nx;
+      const RowSt<R> old = rowSt[k];
+      const R nx = old.x + rd * colValue<R>(d, fmt, q);
       const double w = useW ? kw[k] : 1.0;
-      const double oldE = useW ? w * (double)oeb[k] : (double)oeb[k];
+      const double oldE = useW ? w * (double)old.e : (double)old.e;
       const R o = rexp(nx);
-      oeb[k] = o;
+      rowSt[k] = RowSt<R>{nx, o};
       const double newE = useW ? w * (double)o : (double)o;
       den += newE - oldE;
     }
-    s.denom[pp] = den;
+    s.grp[pp].den = den;
   }
 }
 
@@ -287,11 +285,13 @@ __device__ __forceinline__ void passLoad(const Slot<R>& s, int fmt, int64_t b, i
     const bool valid = r >= 0;
     L.own[it] = MODEL == M_CLR ? (valid && L.key[it] != kn[it]) : valid;
     L.start[it] = MODEL == M_CLR ? (valid && L.key[it] != kp[it]) : valid;
-    L.o[it] = valid ? s.oeb[r] : (R)0;
-    L.xo[it] = valid ? s.xb[r] : (R)0;
+    const RowSt<R> rs = valid ? s.row[r] : RowSt<R>{(R)0, (R)0};
+    L.o[it] = rs.e;
+    L.xo[it] = rs.x;
     L.w[it] = (MODEL != M_CLR && useW && valid) ? s.kw[r] : 1.0;
-    L.den[it] = (MODEL == M_CLR && L.own[it]) ? s.denom[L.key[it]] : 1.0;
-    L.wn[it] = (MODEL == M_CLR && L.own[it]) ? s.nw[L.key[it]] : 0.0;
+    const GrpSt gs = (MODEL == M_CLR && L.own[it]) ? s.grp[L.key[it]] : GrpSt{1.0, 0.0};
+    L.den[it] = gs.den;
+    L.wn[it] = gs.nw;
   }
 }
 
@@ -401,15 +401,14 @@ __device__ __forceinline__ void passUpdate(const Slot<R>& s, LDS& S, int nLocal,
     if (r < 0) continue;
     const R nx = L.xo[it] + rd * L.x[it];
     const R no = rexp(nx);
-    s.xb[r] = nx;
-    s.oeb[r] = no;
+    s.row[r] = RowSt<R>{nx, no};
     val[it][0] = (double)no - (double)L.o[it];
   }
   if (MODEL == M_CLR) {
     runSums<IT, 1>(S, nLocal, L.own, L.start, val, staged);
 #pragma unroll
     for (int it = 0; it < IT; ++it)
-      if (L.own[it]) s.denom[L.key[it]] = L.den[it] + val[it][0];
+      if (L.own[it]) s.grp[L.key[it]].den = L.den[it] + val[it][0];
   }
 }
 
@@ -731,17 +730,20 @@ __device__ __forceinline__ double fitLogLik(const Slot<R>& s, const Part& pt) {
   const int N = s.N;
   double v[2] = {0.0, 0.0};
   for (int k = pt.rowLo + threadIdx.x; k < pt.rowHi; k += blockDim.x) {
-    const double x = (double)s.xb[k];
+    const RowSt<R> rs = s.row[k];
+    const double x = (double)rs.x;
     const int yi = (int)d.y[k];
     const double w = useW ? s.kw[k] : 1.0;
     double c = (MODEL == M_COX) ? (yi != 1 ? 0.0 : x) : yi * x;
     v[0] += useW ? c * w : c;
-    if (MODEL == M_LOGISTIC) v[1] += w * log(1.0 + (double)s.oeb[k]);
-    if (MODEL == M_POISSON) v[1] += w * (double)s.oeb[k];
+    if (MODEL == M_LOGISTIC) v[1] += w * log(1.0 + (double)rs.e);
+    if (MODEL == M_POISSON) v[1] += w * (double)rs.e;
   }
   if (MODEL == M_CLR) {
-    for (int i = pt.grpLo + threadIdx.x; i < pt.grpHi; i += blockDim.x)
-      if (s.nw[i] != 0.0) v[1] += s.nw[i] * log(s.denom[i]);
+    for (int i = pt.grpLo + threadIdx.x; i < pt.grpHi; i += blockDim.x) {
+      const GrpSt gs = s.grp[i];
+      if (gs.nw != 0.0) v[1] += gs.nw * log(gs.den);
+    }
   }
   if (MODEL == M_COX) {
     Seg<3> carry;
@@ -762,7 +764,7 @@ __device__ __forceinline__ double fitLogLik(const Slot<R>& s, const Part& pt) {
         x.v[0] = x.v[1] = x.v[2] = 0.0;
         if (i < N) {
           x.f = s.reset[i];
-          x.v[0] = s.denom[i];
+          x.v[0] = s.grp[i].den;
         }
         items[it] = x;
         agg = segCombine(agg, x);
@@ -774,7 +776,7 @@ __device__ __forceinline__ double fitLogLik(const Slot<R>& s, const Part& pt) {
         const int i = base + it;
         if (i >= N) break;
         run = segCombine(run, items[it]);
-        v[1] += s.nw[i] * log(run.v[0]);
+        v[1] += s.grp[i].nw * log(run.v[0]);
       }
       carry = segCombine(carry, total);
     }
@@ -788,7 +790,7 @@ __device__ __forceinline__ double fitGradObjective(const Slot<R>& s, const Part&
   const int useW = s.ctl.useW;
   double v[1] = {0.0};
   for (int k = pt.rowLo + threadIdx.x; k < pt.rowHi; k += blockDim.x) {
-    const double x = (double)s.xb[k];
+    const double x = (double)s.row[k].x;
     v[0] += useW ? x * s.d.y[k] * s.kw[k] : x * s.d.y[k];
   }
   blockSum<1>(v);
@@ -850,22 +852,21 @@ __device__ __forceinline__ void initSlot(const Slot<R>& s, FitStatus& st) {
   const DevX& d = s.d;
   const int useW = s.ctl.useW;
   const int N = s.N;
-  R* xb = s.xb;
-  R* oeb = s.oeb;
+  RowSt<R>* rowSt = s.row;
   const double* kw = s.kw;
   // nW per group
   if (MODEL == M_CLR) {
     for (int g = threadIdx.x; g < d.nStrata; g += blockDim.x) {
       double acc = 0.0;
       for (int k = d.stratStart[g]; k < d.stratStart[g + 1]; ++k) acc += d.y[k] * kw[k];
-      s.nw[g] = acc;
+      s.grp[g].nw = acc;
     }
   } else if (MODEL == M_COX) {
     for (int p = threadIdx.x; p < N; p += blockDim.x) {
       double acc = 0.0;
       for (int k = s.pidFirst[p]; k < s.pidFirst[p + 1]; ++k)
         if (s.pid[k] == p) acc += d.y[k] * kw[k];
-      s.nw[p] = acc;
+      s.grp[p].nw = acc;
     }
   }
   // XjY
@@ -882,7 +883,7 @@ __device__ __forceinline__ void initSlot(const Slot<R>& s, FitStatus& st) {
     if (threadIdx.x == 0) s.xjy[j] = t[0];
   }
   // xBeta = sum_j beta_j X_j in column order
-  for (int k = threadIdx.x; k < s.K; k += blockDim.x) xb[k] = (R)0;
+  for (int k = threadIdx.x; k < s.K; k += blockDim.x) rowSt[k].x = (R)0;
   __syncthreads();
   for (int j = 0; j < s.J; ++j) {
     const double bj = s.beta[j];
@@ -892,7 +893,7 @@ __device__ __forceinline__ void initSlot(const Slot<R>& s, FitStatus& st) {
     const R rb = (R)bj;
     for (int64_t p = b + threadIdx.x; p < e; p += blockDim.x) {
       const int k = colRow(d, fmt, p, b);
-      xb[k] += rb * colValue<R>(d, fmt, p);
+      rowSt[k].x += rb * colValue<R>(d, fmt, p);
     }
     __syncthreads();
   }
@@ -901,23 +902,23 @@ __device__ __forceinline__ void initSlot(const Slot<R>& s, FitStatus& st) {
     for (int g = threadIdx.x; g < d.nStrata; g += blockDim.x) {
       double acc = 0.0;
       for (int k = d.stratStart[g]; k < d.stratStart[g + 1]; ++k) {
-        const R o = rexp(xb[k]);
-        oeb[k] = o;
+        const R o = rexp(rowSt[k].x);
+        rowSt[k].e = o;
         acc += (double)o;
       }
-      s.denom[g] = acc;
+      s.grp[g].den = acc;
     }
   } else {
-    for (int k = threadIdx.x; k < s.K; k += blockDim.x) oeb[k] = rexp(xb[k]);
+    for (int k = threadIdx.x; k < s.K; k += blockDim.x) rowSt[k].e = rexp(rowSt[k].x);
     if (MODEL == M_COX) {
       __syncthreads();
       for (int p = threadIdx.x; p < N; p += blockDim.x) {
         double acc = 0.0;
         for (int k = s.pidFirst[p]; k < s.pidFirst[p + 1]; ++k) {
           if (s.pid[k] != p) continue;
-          acc += useW ? kw[k] * (double)oeb[k] : (double)oeb[k];
+          acc += useW ? kw[k] * (double)rowSt[k].e : (double)rowSt[k].e;
         }
-        s.denom[p] = acc;
+        s.grp[p].den = acc;
       }
     }
   }
@@ -1014,11 +1015,12 @@ __device__ __forceinline__ void finishSlot(const Slot<R>& s, FitStatus& st) {
   if (s.heldout && MODEL != M_COX) {
     const double* w2 = s.heldout;
     for (int k = threadIdx.x; k < s.K; k += blockDim.x) {
-      const double x = (double)s.xb[k];
+      const RowSt<R> rs = s.row[k];
+      const double x = (double)rs.x;
       const double y = s.d.y[k];
-      if (MODEL == M_LOGISTIC) v[0] += y * w2[k] * (x - log(1.0 + (double)s.oeb[k]));
+      if (MODEL == M_LOGISTIC) v[0] += y * w2[k] * (x - log(1.0 + (double)rs.e));
       if (MODEL == M_POISSON) v[0] += (y * x - exp(x)) * w2[k];
-      if (MODEL == M_CLR) v[0] += y * w2[k] * (x - log(s.denom[s.d.strata[k]]));
+      if (MODEL == M_CLR) v[0] += y * w2[k] * (x - log(s.grp[s.d.strata[k]].den));
     }
   }
   blockSum<1>(v);
@@ -1219,14 +1221,59 @@ bool layoutGangs(const std::vector<GangEntry>& gangs, int capacity, int nX,
   return true;
 }
 
+static int deviceCUs() {
+  static const int n = [] {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return 0;
+    return cus;
+  }();
+  return n;
+}
+
 int gangSize(int n, int capacity, int64_t nnz, int model, int maxGang, int64_t minNnzPerRank) {
   if (model == CCDH_COX || n <= 0) return 1;
   const int nX = xcdCount();
   if (nX <= 0) return 1;
+  // a rank per CU: two ranks sharing a CU gain less than the exchange costs
+  const int budget = std::min(capacity, deviceCUs() > 0 ? deviceCUs() : capacity);
   int G = 1;
-  const int cap = std::min(std::min(kShards, capacity / nX), std::max(1, maxGang));
-  while (G * 2 <= cap && (int64_t)n * G * 2 <= capacity && nnz / (G * 2) >= minNnzPerRank) G *= 2;
+  const int cap = std::min(std::min(kShards, budget / nX), std::max(1, maxGang));
+  while (G * 2 <= cap && (int64_t)n * G * 2 <= budget && nnz / (G * 2) >= minNnzPerRank) G *= 2;
   return G;
+}
+
+void assignGangs(std::vector<GangEntry>& want, const std::vector<double>& priority,
+                 const std::vector<int64_t>& nnz, const std::vector<int>& model, int capacity,
+                 int maxGang, int64_t minNnzPerRank) {
+  const int n = (int)want.size();
+  if (n == 0) return;
+  const int nX = xcdCount();
+  const int budget = std::min(capacity, deviceCUs() > 0 ? deviceCUs() : capacity);
+  const int cap = nX > 0 ? std::min(std::min(kShards, budget / nX), std::max(1, maxGang)) : 1;
+  int used = 0;
+  for (int i = 0; i < n; ++i) {
+    want[i].G = gangSize(n, capacity, nnz[i], model[i], maxGang, minNnzPerRank);
+    used += want[i].G;
+  }
+  // spare CUs go to the highest-priority fits, one doubling at a time
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return priority[a] > priority[b]; });
+  bool grew = true;
+  while (grew && nX > 0) {
+    grew = false;
+    for (int i : order) {
+      const int G = want[i].G;
+      if (model[i] == CCDH_COX || G * 2 > cap || nnz[i] / (G * 2) < minNnzPerRank) continue;
+      if (used + G > budget) continue;
+      want[i].G = G * 2;
+      used += G;
+      grew = true;
+    }
+  }
 }
 
 int slotGangCapacity(bool mixed, int threads) {
@@ -1272,12 +1319,10 @@ SlotLayout slotLayout(int model, int K, int J, int nCap, int64_t maxLen, bool he
   const bool cox = model == CCDH_COX;
   const size_t nG = grouped ? (size_t)nCap + 1 : 1;
   const size_t nR = cox ? 1 : (size_t)std::max<int64_t>(maxLen, 1);
-  L.xb = take((size_t)K * rs);
-  L.oeb = take((size_t)K * rs);
+  L.row = take((size_t)K * 2 * rs);
   L.kw = take((size_t)K * 8);
   L.heldout = heldout && !cox ? take((size_t)K * 8) : kNone;
-  L.denom = grouped ? take(nG * 8) : kNone;
-  L.nw = grouped ? take(nG * 8) : kNone;
+  L.grp = grouped ? take(nG * sizeof(GrpSt)) : kNone;
   L.numer = cox ? take(nG * 8) : kNone;
   L.numer2 = cox ? take(nG * 8) : kNone;
   L.pid = cox ? take((size_t)K * 4) : kNone;
@@ -1333,12 +1378,10 @@ int slotPrepare(const ccdh_data* d, const ccdh_fit_spec& sp, char* block, const 
   x.model = d->model;
   x.K = K;
   x.J = J;
-  x.xb = at<R>(block, L.xb);
-  x.oeb = at<R>(block, L.oeb);
+  x.row = at<RowSt<R>>(block, L.row);
   x.kw = at<double>(block, L.kw);
   x.heldout = sp.heldout ? at<double>(block, L.heldout) : nullptr;
-  x.denom = at<double>(block, L.denom);
-  x.nw = at<double>(block, L.nw);
+  x.grp = at<GrpSt>(block, L.grp);
   x.numer = at<double>(block, L.numer);
   x.numer2 = at<double>(block, L.numer2);
   x.xjy = at<double>(block, L.xjy);
@@ -1506,18 +1549,38 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
 
   // Launch map: every unfinished fit, each on a gang sized by how many fits are
   // left (the tail of slow fits spreads over the idle CUs).
+  // fits expected to need the most sweeps first: the weakest prior (largest
+  // mean variance over the free coordinates)
+  std::vector<double> priorStrength(F, 0.0);
+  for (int f = 0; f < F; ++f) {
+    double sum = 0.0;
+    int cnt = 0;
+    for (int j = 0; j < J; ++j)
+      if (!specs[f].fixed[j] && specs[f].prior_type[j] != CCDH_PRIOR_NONE) {
+        sum += specs[f].prior_var[j];
+        ++cnt;
+      }
+    priorStrength[f] = cnt ? sum / cnt : 1e300;
+  }
   std::vector<GangEntry> map;
   int mapN = 0;
   bool mapGangs = false;
   std::vector<char> running(F, 1);
   int gangCap = maxGang;
+  int curFits = 0, curGang = 1;
   auto buildMap = [&]() -> int {
-    int nAct = 0;
-    for (int f = 0; f < F; ++f) nAct += running[f];
-    int G = gangSize(nAct, capacity, d->nnz, model, gangCap, minNnz);
     std::vector<GangEntry> gangs;
+    std::vector<double> prio;
     for (int f = 0; f < F; ++f)
-      if (running[f] || useGraph) gangs.push_back(GangEntry{f, 0, G, 0});
+      if (running[f] || useGraph) {
+        gangs.push_back(GangEntry{f, 0, 1, 0});
+        prio.push_back(priorStrength[f]);
+      }
+    if (!useGraph && gangCap > 1)
+      assignGangs(gangs, prio, std::vector<int64_t>(gangs.size(), d->nnz),
+                  std::vector<int>(gangs.size(), model), capacity, gangCap, minNnz);
+    int G = 1;
+    for (const GangEntry& g : gangs) G = std::max(G, g.G);
     if (G > 1 && !layoutGangs(gangs, capacity, xcdCount(), map)) G = 1;
     if (G == 1) {
       map.clear();
@@ -1525,6 +1588,8 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
     }
     mapN = (int)map.size();
     mapGangs = G > 1;
+    curFits = (int)gangs.size();
+    curGang = G;
     HIP_TRY(hipMemcpyAsync(dmap, map.data(), mapN * sizeof(GangEntry), hipMemcpyHostToDevice, s));
     return 0;
   };
@@ -1573,6 +1638,11 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
       float ms = 0.f;
       HIP_TRY(hipEventElapsedTime(&ms, o.ev0, o.ev1));
       g_stats.kernel_ms += ms;
+      if (g_stats.launches < 128) {
+        g_stats.launch_ms[g_stats.launches] = ms;
+        g_stats.launch_fits[g_stats.launches] = curFits;
+        g_stats.launch_gang[g_stats.launches] = curGang;
+      }
       g_stats.launches += 1;
       int rounds = 0;
       for (int f = 0; f < F; ++f) {

@@ -394,15 +394,20 @@ class Server final : public ccdh_server {
   // One launch over every fit in flight; each fit gets a gang sized by how many
   // fits share the device and by its own nonzeros.
   int runLaunch(const std::vector<int32_t>& active) {
-    const int nAct = (int)active.size();
     std::vector<GangEntry> want;
-    bool gangs = false;
+    std::vector<double> prio;
+    std::vector<int64_t> nnz;
+    std::vector<int> models;
     for (int32_t slot : active) {
       const Request* r = slotReq_[slot];
-      const int G = gangSize(nAct, capacity_, r->data->nnz, r->model, maxGang_, kGangMinNnz);
-      gangs |= G > 1;
-      want.push_back(GangEntry{slot, 0, G, 0});
+      want.push_back(GangEntry{slot, 0, 1, 0});
+      nnz.push_back(r->data->nnz);
+      prio.push_back((double)r->data->nnz);  // spare CUs to the largest fits
+      models.push_back(r->model);
     }
+    if (maxGang_ > 1) assignGangs(want, prio, nnz, models, capacity_, maxGang_, kGangMinNnz);
+    bool gangs = false;
+    for (const GangEntry& g : want) gangs |= g.G > 1;
     if (gangs && !layoutGangs(want, capacity_, xcdCount(), map_)) gangs = false;
     if (!gangs) {
       map_.clear();

@@ -61,16 +61,28 @@ struct Records {
   double* den;
 };
 
+// Per-row state: xBeta and exp(xBeta) side by side, so the random gather of
+// one nonzero's row touches one cache line with one load (16 B in FP64).
+template <typename R>
+struct alignas(2 * sizeof(R)) RowSt {
+  R x;  // xBeta
+  R e;  // exp(xBeta) (offsExpXBeta)
+};
+
+// Per-group state (stratum / risk-set group): denominator and event weight nW.
+struct alignas(16) GrpSt {
+  double den;
+  double nw;
+};
+
 template <typename R>
 struct Slot {
   DevX d;
   int model, K, J, N;
-  R* xb;
-  R* oeb;
+  RowSt<R>* row;
   double* kw;
   double* heldout;  // nullptr when the fit has no held-out weights
-  double* denom;
-  double* nw;
+  GrpSt* grp;       // groups (grouped models only)
   double* numer;
   double* numer2;
   double* xjy;
@@ -105,7 +117,7 @@ struct GangEntry {
 
 // Byte offsets of every array of one slot inside its block (256-byte aligned).
 struct SlotLayout {
-  size_t xb, oeb, kw, heldout, denom, nw, numer, numer2, xjy, beta, bound, pvar, ptype, fixed,
+  size_t row, kw, heldout, grp, numer, numer2, xjy, beta, bound, pvar, ptype, fixed,
       pid, pidFirst, reset, recRow, recKey, recFlags, recO, recXo, recDen;
   int64_t recStride;
   size_t bytes;
@@ -153,6 +165,11 @@ bool layoutGangs(const std::vector<GangEntry>& gangs, int capacity, int nX,
 // power of two <= min(kShards, maxGang) with n * G <= capacity and at least
 // minNnzPerRank of the slot's nonzeros per rank; Cox slots run on one workgroup.
 int gangSize(int n, int capacity, int64_t nnz, int model, int maxGang, int64_t minNnzPerRank);
+// Gang sizes for slots sharing the device: the uniform size above, then spare
+// CUs (one rank per CU) to the slots of highest priority, doubling one at a time.
+void assignGangs(std::vector<GangEntry>& want, const std::vector<double>& priority,
+                 const std::vector<int64_t>& nnz, const std::vector<int>& model, int capacity,
+                 int maxGang, int64_t minNnzPerRank);
 int slotLaunchFinish(int model, bool mixed, const void* slots, FitStatus* st,
                      const int32_t* active, int n, int threads, hipStream_t s);
 

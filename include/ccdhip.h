@@ -205,6 +205,10 @@ typedef struct {
   int64_t long_columns;    /* multi-pass column visits */
   double kernel_ms;        /* summed HIP-event time of the sweep launches */
   double wall_ms;          /* host wall time of the whole batch */
+  /* the first 128 launches: HIP-event time, fits in flight, workgroups per fit */
+  float launch_ms[128];
+  int32_t launch_fits[128];
+  int32_t launch_gang[128];
 } ccdh_batch_stats;
 int ccdh_batch_last_stats(ccdh_batch_stats* out);
 
