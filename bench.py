@@ -259,11 +259,11 @@ def main():
     # HBM traffic per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
     # workload (profiles/), used when the workload matches
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_batch_sweep_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "k_slot_sweep_traffic.json")
     if os.path.exists(tpath):
         t = json.load(open(tpath))
         if (t.get("strata") == n_strata and t.get("covariates") == P and t.get("fits") == n_fits
-                and precision == api.FP64):
+                and precision == api.FP64 and args.sweeps_per_sync in (0, t.get("sweeps_per_launch"))):
             traffic = t["traffic_bytes_per_launch"]
 
     # plain engine and CPU oracle on a spread of the same fits
