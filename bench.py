@@ -169,6 +169,9 @@ def main():
     ap.add_argument("--serve-plain-seconds", type=float, default=20.0,
                     help="time budget of the plain sequential sample")
     ap.add_argument("--skip-batch", action="store_true", help="serving leg only (probe)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="tune pass width and gangs on the workload first (inside the timed step)")
+    ap.add_argument("--pass-items", type=int, default=0, help="long-column pass width without tuning")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -221,7 +224,8 @@ def main():
         hip.hipFree(src)
         hip.hipFree(dst)
         log(f"copy calibration: {nbytes} bytes read + {nbytes} bytes written")
-    opt = dict(threads=args.threads, sweeps_per_sync=args.sweeps_per_sync, use_graph=1)
+    opt = dict(threads=args.threads, sweeps_per_sync=args.sweeps_per_sync, use_graph=1,
+               autotune=args.autotune, pass_items=args.pass_items)
 
     # load the code objects and touch the allocator (untimed)
     mini = [default_spec(prob, prior=PRIOR_LAPLACE, variance=1.0, max_iterations=1)]
@@ -253,6 +257,10 @@ def main():
         f"({stats['long_columns']} visits), objective {stats['ticks_objective'] * tick:.1f}s; "
         f"sweeps {stats['sweeps']}, gathered nnz {stats['gather_nnz']}, updated nnz {stats['update_nnz']}")
     log("launch timeline (ms, fits in flight, workgroups per fit):", stats["timeline"])
+    tuning = {"autotune": bool(args.autotune), "pass_items": stats["tuned_pass_items"],
+              "max_gang": stats["tuned_max_gang"], "tune_ms": stats["tune_ms"],
+              "candidate_coords_per_ms": [r for r in stats["tune_rates"] if r > 0]}
+    log("tuning:", tuning)
     mx = int(iters.max())
     log(f"slowest fit: {mx} sweeps in {lat.max():.2f}s -> {lat.max() / mx * 1e3:.1f} ms/sweep")
 
@@ -324,6 +332,7 @@ def main():
                      "kernel": "k_slot_sweep", "avg_launch_ms": per_launch_ms,
                      "launches": stats["launches"]},
         "cpu_baseline": cpu,
+        "tuning": tuning,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

@@ -26,7 +26,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -241,7 +244,7 @@ __device__ __forceinline__ int cycleCox(const Slot<R>& s, FitStatus& cs, int j0,
 // write one record per nonzero in the gather pass and the update pass reads the
 // records back in one coalesced hop.
 constexpr int kIT = 4;
-constexpr int kITL = 8;
+constexpr int kITLDefault = 8;  // long-column pass width (nonzeros per lane); 4 is the alternative
 
 template <int NMAX>
 struct PassLds {
@@ -569,12 +572,12 @@ __device__ __forceinline__ bool gangExchange(Gang& g, double (&sum)[NS], double 
 // barrier follows an update only.  Returns the coordinate to resume at when the
 // quantum ran out (J when the cycle completed), or -1 when a gang exchange
 // failed.
-template <int MODEL, typename R, class LDS>
+template <int MODEL, typename R, int ITL, class LDS>
 __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus& cs, LDS& S, int j0,
                                         long long deadline, Gang& gg) {
   const DevX& d = s.d;
   const int64_t CH = (int64_t)blockDim.x * kIT;
-  const int64_t CHL = (int64_t)blockDim.x * kITL;
+  const int64_t CHL = (int64_t)blockDim.x * ITL;
   const int J = s.J;
   const int useW = s.ctl.useW;
   const bool multi = gg.G > 1;
@@ -662,10 +665,10 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus& cs, LDS& S,
       }
     } else {
       for (int64_t c0 = b; c0 < e; c0 += CHL) {
-        Lanes<kITL, R> L;
-        passLoad<MODEL, kITL, R>(s, fmt, cb, e, c0, useW, L);
-        storeRecords<MODEL, kITL, R>(rec, c0 - cb, L);
-        passGather<MODEL, kITL, R>(fmt, S, (int)min(CHL, e - c0), staged, L, useW, v);
+        Lanes<ITL, R> L;
+        passLoad<MODEL, ITL, R>(s, fmt, cb, e, c0, useW, L);
+        storeRecords<MODEL, ITL, R>(rec, c0 - cb, L);
+        passGather<MODEL, ITL, R>(fmt, S, (int)min(CHL, e - c0), staged, L, useW, v);
       }
       if (!reduce()) return -1;
       delta = step();
@@ -673,9 +676,9 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus& cs, LDS& S,
       if (delta != 0.0) {
         __syncthreads();  // carry reset visible; records were written by these lanes
         for (int64_t c0 = b; c0 < e; c0 += CHL) {
-          Lanes<kITL, R> L;
-          loadRecords<MODEL, kITL, R>(d, fmt, e, c0, c0 - cb, rec, L);
-          passUpdate<MODEL, kITL, R>(s, S, (int)min(CHL, e - c0), staged, L, delta);
+          Lanes<ITL, R> L;
+          loadRecords<MODEL, ITL, R>(d, fmt, e, c0, c0 - cb, rec, L);
+          passUpdate<MODEL, ITL, R>(s, S, (int)min(CHL, e - c0), staged, L, delta);
         }
         __syncthreads();
       }
@@ -848,14 +851,15 @@ __device__ __forceinline__ bool gangObjective(const Slot<R>& s, Gang& gg, long l
 // Per-fit set-up: event weights, XjY, xBeta from the starting coefficients,
 // exp / denominators, bounds and the starting objective (findMode's lazy part).
 template <int MODEL, typename R>
-__device__ __forceinline__ void initSlot(const Slot<R>& s, FitStatus& st) {
+__device__ __forceinline__ void initSlot(const Slot<R>& s, FitStatus& st, bool full) {
   const DevX& d = s.d;
   const int useW = s.ctl.useW;
   const int N = s.N;
   RowSt<R>* rowSt = s.row;
   const double* kw = s.kw;
-  // nW per group
-  if (MODEL == M_CLR) {
+  // nW and XjY depend on the weights only: a restart keeps them
+  if (!full) {
+  } else if (MODEL == M_CLR) {
     for (int g = threadIdx.x; g < d.nStrata; g += blockDim.x) {
       double acc = 0.0;
       for (int k = d.stratStart[g]; k < d.stratStart[g + 1]; ++k) acc += d.y[k] * kw[k];
@@ -870,7 +874,7 @@ __device__ __forceinline__ void initSlot(const Slot<R>& s, FitStatus& st) {
     }
   }
   // XjY
-  for (int j = 0; j < s.J; ++j) {
+  for (int j = 0; full && j < s.J; ++j) {
     const int fmt = d.fmt[j];
     const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
     double t[1] = {0.0};
@@ -938,7 +942,7 @@ __device__ __forceinline__ void initSlot(const Slot<R>& s, FitStatus& st) {
 // with a deadline, the slot stops at a coordinate boundary once it has passed
 // and the next launch resumes the sweep there.  Every rank of a gang computes
 // the same objective and convergence decision; the lead rank records them.
-template <int MODEL, typename R, class LDS>
+template <int MODEL, typename R, int ITL, class LDS>
 __device__ __forceinline__ void sweepSlot(const Slot<R>& s, FitStatus& st, LDS& S, int sweeps,
                                           long long deadline, Gang& gg) {
   const FitCtl c = s.ctl;
@@ -949,7 +953,7 @@ __device__ __forceinline__ void sweepSlot(const Slot<R>& s, FitStatus& st, LDS& 
   double lastObj = st.lastObj;
   for (int sw = 0; sw < sweeps; ++sw) {
     const int jEnd = MODEL == M_COX ? cycleCox<R>(s, st, j0, deadline)
-                                    : cycleFit<MODEL, R>(s, st, S, j0, deadline, gg);
+                                    : cycleFit<MODEL, R, ITL>(s, st, S, j0, deadline, gg);
     __syncthreads();
     if (jEnd < 0) {
       if (threadIdx.x == 0) st.fault = 1;
@@ -1035,22 +1039,22 @@ __device__ __forceinline__ void finishSlot(const Slot<R>& s, FitStatus& st) {
 // model (uniform per workgroup).
 
 template <typename R>
-__global__ void k_slot_init(const Slot<R>* slots, FitStatus* st, const int32_t* active) {
+__global__ void k_slot_init(const Slot<R>* slots, FitStatus* st, const int32_t* active, int full) {
   const int f = active[blockIdx.x];
   const Slot<R>& s = slots[f];
   switch (s.model) {
-    case M_LOGISTIC: initSlot<M_LOGISTIC, R>(s, st[f]); break;
-    case M_POISSON: initSlot<M_POISSON, R>(s, st[f]); break;
-    case M_CLR: initSlot<M_CLR, R>(s, st[f]); break;
-    case M_COX: initSlot<M_COX, R>(s, st[f]); break;
+    case M_LOGISTIC: initSlot<M_LOGISTIC, R>(s, st[f], full != 0); break;
+    case M_POISSON: initSlot<M_POISSON, R>(s, st[f], full != 0); break;
+    case M_CLR: initSlot<M_CLR, R>(s, st[f], full != 0); break;
+    case M_COX: initSlot<M_COX, R>(s, st[f], full != 0); break;
   }
 }
 
-template <typename R, int BLOCK>
+template <typename R, int BLOCK, int ITL>
 __global__ void __launch_bounds__(BLOCK)
     k_slot_sweep(const Slot<R>* slots, FitStatus* st, const GangEntry* map, Xch* xch,
                  int sweeps, long long quantum) {
-  __shared__ PassLds<BLOCK * kITL> S;
+  __shared__ PassLds<BLOCK * ITL> S;
   __shared__ long long sStart;
   const GangEntry ge = map[blockIdx.x];
   if (ge.slot < 0) return;
@@ -1092,10 +1096,10 @@ __global__ void __launch_bounds__(BLOCK)
     gg.rowHi = s.K;
   }
   switch (s.model) {
-    case M_LOGISTIC: sweepSlot<M_LOGISTIC, R>(s, st[f], S, sweeps, deadline, gg); break;
-    case M_POISSON: sweepSlot<M_POISSON, R>(s, st[f], S, sweeps, deadline, gg); break;
-    case M_CLR: sweepSlot<M_CLR, R>(s, st[f], S, sweeps, deadline, gg); break;
-    case M_COX: sweepSlot<M_COX, R>(s, st[f], S, sweeps, deadline, gg); break;
+    case M_LOGISTIC: sweepSlot<M_LOGISTIC, R, ITL>(s, st[f], S, sweeps, deadline, gg); break;
+    case M_POISSON: sweepSlot<M_POISSON, R, ITL>(s, st[f], S, sweeps, deadline, gg); break;
+    case M_CLR: sweepSlot<M_CLR, R, ITL>(s, st[f], S, sweeps, deadline, gg); break;
+    case M_COX: sweepSlot<M_COX, R, ITL>(s, st[f], S, sweeps, deadline, gg); break;
   }
 }
 
@@ -1131,32 +1135,37 @@ int clampThreads(int threads) { return threads <= 256 ? 256 : 512; }
 namespace ccdamd {
 
 int slotLaunchInit(int /*model*/, bool mixed, const void* slots, FitStatus* st,
-                   const int32_t* active, int n, int threads, hipStream_t s) {
+                   const int32_t* active, int n, int threads, hipStream_t s, bool full) {
   if (n <= 0) return 0;
   threads = clampThreads(threads);
   if (mixed)
-    k_slot_init<float><<<n, threads, 0, s>>>((const Slot<float>*)slots, st, active);
+    k_slot_init<float><<<n, threads, 0, s>>>((const Slot<float>*)slots, st, active, full ? 1 : 0);
   else
-    k_slot_init<double><<<n, threads, 0, s>>>((const Slot<double>*)slots, st, active);
+    k_slot_init<double><<<n, threads, 0, s>>>((const Slot<double>*)slots, st, active, full ? 1 : 0);
   HIP_TRY(hipGetLastError());
   return 0;
 }
 
+// Sweep kernel variants: (256 lanes, pass width 8 or 4), (512 lanes, width 8).
+static const void* sweepKernel(bool mixed, int threads, int itl) {
+  if (clampThreads(threads) == 512)
+    return mixed ? (const void*)k_slot_sweep<float, 512, 8> : (const void*)k_slot_sweep<double, 512, 8>;
+  if (itl == 4)
+    return mixed ? (const void*)k_slot_sweep<float, 256, 4> : (const void*)k_slot_sweep<double, 256, 4>;
+  return mixed ? (const void*)k_slot_sweep<float, 256, 8> : (const void*)k_slot_sweep<double, 256, 8>;
+}
+
 int slotLaunchSweeps(bool mixed, const void* slots, FitStatus* st, const GangEntry* map, int n,
-                     Xch* xch, bool gangs, int threads, int sweeps, long long quantumTicks,
-                     hipStream_t s) {
+                     Xch* xch, bool gangs, int threads, int itl, int sweeps,
+                     long long quantumTicks, hipStream_t s) {
   if (n <= 0) return 0;
   if (gangs) {
     k_gang_reset<<<(n + 255) / 256, 256, 0, s>>>(map, n, xch);
     HIP_TRY(hipGetLastError());
   }
   long long q = quantumTicks;
-  const void* fn;
+  const void* fn = sweepKernel(mixed, threads, itl);
   const int block = clampThreads(threads);
-  if (block == 256)
-    fn = mixed ? (const void*)k_slot_sweep<float, 256> : (const void*)k_slot_sweep<double, 256>;
-  else
-    fn = mixed ? (const void*)k_slot_sweep<float, 512> : (const void*)k_slot_sweep<double, 512>;
   void* args[] = {(void*)&slots, (void*)&st, (void*)&map, (void*)&xch, (void*)&sweeps, (void*)&q};
   if (gangs) {
     // every workgroup of a gang must be resident while its peers wait for it
@@ -1166,6 +1175,7 @@ int slotLaunchSweeps(bool mixed, const void* slots, FitStatus* st, const GangEnt
   }
   return 0;
 }
+
 
 static int probeXcds() {
   const int n = 512;
@@ -1276,13 +1286,9 @@ void assignGangs(std::vector<GangEntry>& want, const std::vector<double>& priori
   }
 }
 
-int slotGangCapacity(bool mixed, int threads) {
+int slotGangCapacity(bool mixed, int threads, int itl) {
   const int block = clampThreads(threads);
-  const void* fn;
-  if (block == 256)
-    fn = mixed ? (const void*)k_slot_sweep<float, 256> : (const void*)k_slot_sweep<double, 256>;
-  else
-    fn = mixed ? (const void*)k_slot_sweep<float, 512> : (const void*)k_slot_sweep<double, 512>;
+  const void* fn = sweepKernel(mixed, threads, itl);
   int perCU = 0, dev = 0, cus = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fn, block, 0) != hipSuccess) return 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
@@ -1485,6 +1491,101 @@ int devAllocOwned(T** p, size_t n, std::vector<void*>& owned) {
   return 0;
 }
 
+// ---------------------------------------------------------------- autotuner
+// Candidates for a workload: long-column pass width (8 or 4 nonzeros per lane)
+// and whether fits may run on gangs.  Each candidate runs one launch of a fixed
+// device-time quantum over every fit from its starting point; the rate is the
+// coordinates completed per millisecond of kernel time.  The fits are then put
+// back to their starting state.  The choice is cached per workload signature.
+struct TuneKey {
+  int model, K, J, F, precision, threads, maxGang;
+  int64_t nnz;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(model, K, J, F, precision, threads, maxGang, nnz) <
+           std::tie(o.model, o.K, o.J, o.F, o.precision, o.threads, o.maxGang, o.nnz);
+  }
+};
+std::mutex g_tuneMu;
+std::map<TuneKey, std::pair<int, int>> g_tuned;
+
+constexpr long long kTrialTicks = 5000000;  // 50 ms of device time per candidate
+
+int timeTrial(bool mixed, const void* dslots, FitStatus* dst, const GangEntry* dmap, int mapN,
+              Xch* dxch, bool gangs, int threads, int itl, int F, int J, Owned& o, hipStream_t s,
+              double* rate) {
+  HIP_TRY(hipEventRecord(o.ev0, s));
+  int rc = slotLaunchSweeps(mixed, dslots, dst, dmap, mapN, dxch, gangs, threads, itl, 1 << 30,
+                            kTrialTicks, s);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(o.ev1, s));
+  HIP_TRY(hipMemcpyAsync(o.hst, dst, F * sizeof(FitStatus), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, o.ev0, o.ev1));
+  double coords = 0.0;
+  for (int f = 0; f < F; ++f) {
+    if (o.hst[f].fault) return setError(CCDH_E_DEVICE, "gang exchange failed during tuning");
+    coords += (double)o.hst[f].iteration * J + o.hst[f].jNext;
+  }
+  *rate = ms > 0.f ? coords / ms : 0.0;
+  return 0;
+}
+
+template <typename R, class Trial>
+int tuneBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, int threads, int maxGang,
+              std::vector<Slot<R>>& slots, std::vector<FitStatus>& st, Slot<R>* dslots,
+              FitStatus* dst, int32_t* dactive, hipStream_t s, Owned& o, Trial trial, int* itlOut,
+              int* gangOut) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const TuneKey key{d->model, d->K, d->J, F, d->precision, clampThreads(threads), maxGang, d->nnz};
+  {
+    std::lock_guard<std::mutex> g(g_tuneMu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) {
+      *itlOut = it->second.first;
+      *gangOut = it->second.second;
+      return 0;
+    }
+  }
+  if (!o.ev0) HIP_TRY(hipEventCreate(&o.ev0));
+  if (!o.ev1) HIP_TRY(hipEventCreate(&o.ev1));
+  std::vector<std::pair<int, int>> cands = {{8, maxGang}, {8, 1}};
+  if (clampThreads(threads) == 256) cands.push_back({4, maxGang});
+  if (maxGang == 1) cands.erase(cands.begin() + 1);
+  double best = -1.0;
+  std::pair<int, int> pick = cands[0];
+  for (size_t c = 0; c < cands.size(); ++c) {
+    double rate = 0.0;
+    int rc = trial(cands[c].first, cands[c].second, &rate);
+    if (rc) return rc;
+    g_stats.tune_rates[c] = rate;
+    if (rate > best) {
+      best = rate;
+      pick = cands[c];
+    }
+    // back to the starting point: coefficients, statuses, then the set-up kernel
+    for (int f = 0; f < F; ++f)
+      HIP_TRY(hipMemcpyAsync(slots[f].beta, specs[f].beta0, d->J * sizeof(double),
+                             hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dst, st.data(), F * sizeof(FitStatus), hipMemcpyHostToDevice, s));
+    std::vector<int32_t> all(F);
+    for (int f = 0; f < F; ++f) all[f] = f;
+    HIP_TRY(hipMemcpyAsync(dactive, all.data(), F * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    rc = slotLaunchInit(d->model, std::is_same<R, float>::value, dslots, dst, dactive, F, threads, s,
+                        false);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  *itlOut = pick.first;
+  *gangOut = pick.second;
+  {
+    std::lock_guard<std::mutex> g(g_tuneMu);
+    g_tuned[key] = pick;
+  }
+  g_stats.tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
 template <typename R>
 int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
              ccdh_fit_result* results, const ccdh_batch_options* opt) {
@@ -1497,8 +1598,10 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
   const int sweepsPerSync = (opt && opt->sweeps_per_sync > 0) ? opt->sweeps_per_sync : 4;
   const int maxGang = (opt && opt->max_gang > 0) ? opt->max_gang : kShards;
   const int64_t minNnz = (opt && opt->gang_min_nnz > 0) ? opt->gang_min_nnz : 32768;
+  int itl = (opt && opt->pass_items == 4 && clampThreads(threads) == 256) ? 4 : kITLDefault;
+  const bool autotune = opt && opt->autotune && model != CCDH_COX;
   // graph replay of a fixed launch when gangs are off
-  const bool useGraph = (opt ? opt->use_graph != 0 : true) && maxGang == 1;
+  const bool useGraph = (opt ? opt->use_graph != 0 : true) && maxGang == 1 && !autotune;
 
   Owned o;
   HIP_TRY(hipStreamCreateWithFlags(&o.stream, hipStreamNonBlocking));
@@ -1512,7 +1615,7 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
     anyHeld |= specs[f].heldout != nullptr;
   }
   const SlotLayout L = slotLayout(model, d->K, J, maxN, maxColumnLength(d), anyHeld, sizeof(R));
-  const int capacity = slotGangCapacity(mixed, threads);
+  const int capacity = std::min(slotGangCapacity(mixed, threads, 8), slotGangCapacity(mixed, threads, 4));
   if (capacity <= 0) return setError(CCDH_E_DEVICE, "sweep kernel occupancy query failed");
   (void)xcdCount();
   char* arena = nullptr;
@@ -1595,12 +1698,30 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
   };
   // sweep launches, optionally replayed from a captured graph
   auto enqueueSweeps = [&]() -> int {
-    int r = slotLaunchSweeps(mixed, dslots, dst, dmap, mapN, dxch, mapGangs, threads,
+    int r = slotLaunchSweeps(mixed, dslots, dst, dmap, mapN, dxch, mapGangs, threads, itl,
                              sweepsPerSync, 0, s);
     if (r) return r;
     HIP_TRY(hipMemcpyAsync(o.hst, dst, F * sizeof(FitStatus), hipMemcpyDeviceToHost, s));
     return 0;
   };
+  std::memset(&g_stats, 0, sizeof(g_stats));
+  g_stats.tuned_pass_items = itl;
+  g_stats.tuned_max_gang = gangCap;
+  if (autotune) {
+    rc = tuneBatch<R>(d, F, specs, threads, maxGang, slots, st, dslots, dst, dactive, s, o,
+                      [&](int candItl, int candGang, double* rate) -> int {
+                        itl = candItl;
+                        gangCap = candGang;
+                        int r = buildMap();
+                        if (r) return r;
+                        return timeTrial(mixed, dslots, dst, dmap, mapN, dxch, mapGangs, threads,
+                                         itl, F, J, o, s, rate);
+                      },
+                      &itl, &gangCap);
+    if (rc) return rc;
+    g_stats.tuned_pass_items = itl;
+    g_stats.tuned_max_gang = gangCap;
+  }
   rc = buildMap();
   if (rc) return rc;
   if (useGraph) {
@@ -1616,9 +1737,8 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
   }
   std::vector<double> doneAt(F, 0.0);
   std::vector<int> lastIter(F, 0);
-  std::memset(&g_stats, 0, sizeof(g_stats));
-  HIP_TRY(hipEventCreate(&o.ev0));
-  HIP_TRY(hipEventCreate(&o.ev1));
+  if (!o.ev0) HIP_TRY(hipEventCreate(&o.ev0));
+  if (!o.ev1) HIP_TRY(hipEventCreate(&o.ev1));
   int guard = 0;
   for (;;) {
     if (!useGraph) {
@@ -1687,7 +1807,7 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
         retry = true;
         active[0] = f;
         HIP_TRY(hipMemcpyAsync(dactive, active.data(), sizeof(int32_t), hipMemcpyHostToDevice, s));
-        rc = slotLaunchInit(model, mixed, dslots, dst, dactive, 1, threads, s);
+        rc = slotLaunchInit(model, mixed, dslots, dst, dactive, 1, threads, s, false);
         if (rc) return rc;
         HIP_TRY(hipStreamSynchronize(s));
         running[f] = 1;

@@ -97,7 +97,7 @@ class Server final : public ccdh_server {
     maxSlots_ = o.max_slots > 0 ? o.max_slots : 1024;
     threads_ = o.threads > 0 ? o.threads : 256;
     maxGang_ = o.max_gang > 0 ? o.max_gang : kShards;
-    capacity_ = slotGangCapacity(kMixed, threads_);
+    capacity_ = slotGangCapacity(kMixed, threads_, 8);
     if (capacity_ <= 0) return setError(CCDH_E_DEVICE, "sweep kernel occupancy query failed");
     (void)xcdCount();
     const double qms = o.quantum_ms > 0 ? o.quantum_ms : 20.0;
@@ -418,7 +418,7 @@ class Server final : public ccdh_server {
                        schedStream_) != hipSuccess ||
         hipEventRecord(ev0_, schedStream_) != hipSuccess)
       return setError(CCDH_E_DEVICE, "launch set-up");
-    int rc = slotLaunchSweeps(kMixed, dSlots_, dSt_, dMap_, n, dXch_, gangs, threads_, 1 << 30,
+    int rc = slotLaunchSweeps(kMixed, dSlots_, dSt_, dMap_, n, dXch_, gangs, threads_, 8, 1 << 30,
                               quantumTicks_, schedStream_);
     if (rc) return rc;
     if (hipEventRecord(ev1_, schedStream_) != hipSuccess ||
@@ -465,7 +465,7 @@ class Server final : public ccdh_server {
             hipMemcpyAsync(dSlots_ + slot, &s, sizeof(s), hipMemcpyHostToDevice, schedStream_) != hipSuccess ||
             hipMemcpyAsync(dOne_, &slot, sizeof(int32_t), hipMemcpyHostToDevice, schedStream_) != hipSuccess)
           return setError(CCDH_E_DEVICE, "restart upload");
-        int rc = slotLaunchInit(M_ANY, kMixed, dSlots_, dSt_, dOne_, 1, threads_, schedStream_);
+        int rc = slotLaunchInit(M_ANY, kMixed, dSlots_, dSt_, dOne_, 1, threads_, schedStream_, false);
         if (rc) return rc;
         if (hipStreamSynchronize(schedStream_) != hipSuccess) return setError(CCDH_E_DEVICE, "restart");
         continue;

@@ -142,18 +142,22 @@ int64_t maxColumnLength(const ccdh_data* d);
 
 // Kernel launches over `n` slots listed in `active` (device array of slot
 // indices into `slots`).  model is a CCDH model code or M_ANY.
+// full = false: a restart from the starting coefficients that keeps the
+// weight-dependent terms (nW, XjY) already in the slot.
 int slotLaunchInit(int model, bool mixed, const void* slots, FitStatus* st,
-                   const int32_t* active, int n, int threads, hipStream_t s);
+                   const int32_t* active, int n, int threads, hipStream_t s, bool full = true);
 // One sweep launch over `n` workgroups described by `map` (device array).
 // quantumTicks > 0: a slot stops at the first coordinate boundary after that
 // many device clock ticks (100 MHz) and resumes there in the next launch.
 // gangs: some slot runs on more than one workgroup (cooperative launch; the
 // exchange counters of `xch`, indexed by slot, are reset first).
+// itl: nonzeros per lane in one pass over a long column (8 or 4; 512-lane
+// workgroups use 8).
 int slotLaunchSweeps(bool mixed, const void* slots, FitStatus* st, const GangEntry* map, int n,
-                     Xch* xch, bool gangs, int threads, int sweeps, long long quantumTicks,
-                     hipStream_t s);
+                     Xch* xch, bool gangs, int threads, int itl, int sweeps,
+                     long long quantumTicks, hipStream_t s);
 // Workgroups a cooperative sweep launch can hold at once.
-int slotGangCapacity(bool mixed, int threads);
+int slotGangCapacity(bool mixed, int threads, int itl);
 // XCDs of the device when workgroups are dealt to them round-robin by index
 // (measured once with a probe kernel), else 0 (gangs are then not used).
 int xcdCount();

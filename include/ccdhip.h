@@ -179,6 +179,8 @@ typedef struct {
   int use_graph;         /* replay the sweep launch from a hipGraph (only with max_gang 1) */
   int max_gang;          /* workgroups one fit may use (1: one each; 0: 64) */
   int64_t gang_min_nnz;  /* nonzeros per workgroup below which a fit's gang stops growing (0: 32768) */
+  int pass_items;        /* long-column pass width, nonzeros per lane: 8 (0) or 4 (256 lanes) */
+  int autotune;          /* time pass width x gangs on this workload first; the choice is cached */
 } ccdh_batch_options;
 
 /* Batched engine: n_fits fits over one data handle, device-resident sweeps.
@@ -209,6 +211,10 @@ typedef struct {
   float launch_ms[128];
   int32_t launch_fits[128];
   int32_t launch_gang[128];
+  /* what the batch ran with; with autotune, the candidates' rates (coordinates per ms) */
+  int32_t tuned_pass_items, tuned_max_gang;
+  double tune_ms;
+  double tune_rates[4];
 } ccdh_batch_stats;
 int ccdh_batch_last_stats(ccdh_batch_stats* out);
 

@@ -168,3 +168,31 @@ def test_batch_gangs_long_columns_many_fits():
         b2, i2 = api.fit_batch(d, specs, max_gang=2, gang_min_nnz=1)
         np.testing.assert_allclose(b2, b1, rtol=1e-10, atol=1e-13)
         assert [r["iterations"] for r in i2] == [r["iterations"] for r in i1]
+
+
+def test_batch_pass_width_changes_only_rounding():
+    """Long-column passes of 4 or 8 nonzeros per lane give the same fits up to
+    the order of the sums."""
+    p, prior, var, maxit = SYNTH["clr"]()
+    specs, _ = cv_specs(p, prior, var, maxit, n_fold=2, grid=(0.5,))
+    d = api.Data(p)
+    b8, i8 = api.fit_batch(d, specs, pass_items=8)
+    b4, i4 = api.fit_batch(d, specs, pass_items=4)
+    np.testing.assert_allclose(b4, b8, rtol=1e-10, atol=1e-13)
+    assert [r["iterations"] for r in i4] == [r["iterations"] for r in i8]
+
+
+def test_batch_autotune_matches_oracle_and_caches():
+    p, prior, var, maxit = SYNTH["poisson"]()
+    specs, held = cv_specs(p, prior, var, maxit, n_fold=2, grid=(1.0, 0.1))
+    d = api.Data(p)
+    bg, ig = api.fit_batch(d, specs, heldouts=held, autotune=1, gang_min_nnz=1)
+    st = api.batch_last_stats()
+    assert st["tuned_pass_items"] in (4, 8) and st["tuned_max_gang"] >= 1
+    assert st["tune_ms"] > 0 and max(st["tune_rates"]) > 0
+    for i, sp in enumerate(specs):
+        bc, ic = orc.fit(p, sp, heldout=held[i])
+        check_fit(bg[i], ig[i], bc, ic)
+    b2, _ = api.fit_batch(d, specs, heldouts=held, autotune=1, gang_min_nnz=1)
+    assert api.batch_last_stats()["tune_ms"] == 0.0  # the cached choice is reused
+    np.testing.assert_allclose(b2, bg, rtol=1e-10, atol=1e-13)
