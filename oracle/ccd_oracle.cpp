@@ -711,6 +711,28 @@ int orc_eval(const orc_data* od, const double* beta, const double* weights, int 
   return 0;
 }
 
+// Log likelihood and the gradient/Hessian of each coordinate in cols[0..ncols)
+// at one beta, from one set of fresh statistics (orc_eval for many columns).
+int orc_eval_cols(const orc_data* od, const double* beta, const double* weights,
+                  const int32_t* cols, int ncols, double* loglik, double* grad, double* hess) {
+  Data d{od->K, od->J, od->col_ptr, od->rows, od->vals, od->fmt, od->y, od->offs, od->strata,
+         od->model};
+  Engine e(d);
+  const bool useCV = weights != nullptr;
+  e.setWeights(weights, useCV);
+  e.computeFixedTermsInLogLikelihood(useCV);
+  e.computeXjY(useCV);
+  e.zeroXBeta();
+  for (int c = 0; c < d.J; ++c) e.axpyXBeta(beta[c], c);
+  e.computeRemainingStatistics(true);
+  *loglik = e.logLikelihood(useCV);
+  for (int i = 0; i < ncols; ++i) {
+    e.computeNumeratorForGradient(cols[i], useCV);
+    e.gradHess(cols[i], useCV, grad + i, hess + i);
+  }
+  return 0;
+}
+
 double orc_prior_delta(int type, double var, double g, double h, double beta) {
   return priorDelta(type, var, g, h, beta);
 }

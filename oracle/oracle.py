@@ -54,6 +54,8 @@ def lib():
         _lib.orc_fit.argtypes = [vp] * 9
         _lib.orc_eval.restype = ctypes.c_int
         _lib.orc_eval.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, vp]
+        _lib.orc_eval_cols.restype = ctypes.c_int
+        _lib.orc_eval_cols.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp, vp, vp]
         _lib.orc_prior_delta.restype = ctypes.c_double
         _lib.orc_prior_delta.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double]
@@ -122,6 +124,22 @@ def evaluate(p, beta, j=-1, weights=None):
     lib().orc_eval(ctypes.byref(d), _ptr(b), _ptr(w), ctypes.c_int(j), ctypes.byref(ll),
                    ctypes.byref(g), ctypes.byref(h))
     return ll.value, g.value, h.value
+
+
+def evaluate_cols(p, beta, cols, weights=None):
+    """(log likelihood, gradients, Hessians) of the coordinates `cols` at `beta`,
+    from one set of fresh statistics."""
+    keep = []
+    d = _data_struct(p, keep)
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    c = np.ascontiguousarray(cols, dtype=np.int32)
+    g = np.zeros(len(c))
+    h = np.zeros(len(c))
+    ll = ctypes.c_double()
+    lib().orc_eval_cols(ctypes.byref(d), _ptr(b), _ptr(w), _ptr(c), len(c), ctypes.byref(ll),
+                        _ptr(g), _ptr(h))
+    return ll.value, g, h
 
 
 def prior_delta(ptype, var, g, h, beta):

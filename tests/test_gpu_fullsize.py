@@ -1,0 +1,166 @@
+"""GPU parity at the benchmark configurations' full sizes.
+
+  lr      10,000 x 1,000 indicator columns, Laplace var 1 — fitted to convergence
+  poisson 1,000,000 x 100,000 Zipf columns (~4e7 nonzeros) with offset and
+          intercept, Normal var 0.1 — the first two sweeps (the oracle takes ~12 s
+          per sweep here, the fit needs hundreds: the trajectory is compared)
+  cox     2,000,000 rows x 50,000 Zipf columns (~6e7 nonzeros), one stratum, tied
+          integer-day times, Laplace var 0.05 — the 64 free coordinates (32 heaviest,
+          32 spread over the rest; the others fixed at 0) fitted to convergence, since
+          each coordinate costs an O(N) risk-set scan on the CPU
+  clr     200,000 strata (~1.3e6 rows) x 20,000 columns, 10-fold BY_PID CV weights,
+          Laplace var 1e-4 and 1e-3 — two of the 160 CV fits, with held-out scores
+
+FP64: coefficients and log likelihood within 1e-8 relative of the oracle (atol
+1e-10), same iterations and return flag; per-coordinate gradient and Hessian at the
+oracle's estimate within 1e-9.  Mixed precision (FP32 xBeta/exp streams, FP64
+reductions): within 2e-4 relative on coefficients (atol 2e-5) and 1e-6 on the log
+likelihood of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from ccd_amd import api, synth
+from ccd_amd.data import PRIOR_LAPLACE, PRIOR_NORMAL, default_spec
+from oracle import oracle as orc
+from tests.test_gpu_plain import check_fit
+
+# data generation (~15 s) and the CPU oracle (~45 s for the Cox case) run inside the tests
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(300)]
+
+SEED = synth.DEFAULT_SEED
+
+
+@pytest.fixture(scope="module")
+def poisson_full():
+    p, _ = synth.poisson_zipf(seed=SEED + 2)
+    sp = default_spec(p, prior=PRIOR_NORMAL, variance=0.1, max_iterations=2)
+    bc, ic = orc.fit(p, sp)
+    return p, sp, bc, ic
+
+
+@pytest.fixture(scope="module")
+def cox_full():
+    p, _ = synth.cox_zipf(seed=SEED + 3)
+    sp = default_spec(p, prior=PRIOR_LAPLACE, variance=0.05)
+    nnz = np.diff(p.col_ptr)
+    free = np.unique(np.concatenate([np.argsort(nnz)[-32:],
+                                     np.linspace(0, p.J - 1, 32).astype(int)]))
+    sp.fixed[:] = 1
+    sp.fixed[free] = 0
+    bc, ic = orc.fit(p, sp)
+    return p, sp, bc, ic
+
+
+def _probe_columns(p, beta):
+    """Heaviest, lightest non-empty and median columns plus two with non-zero beta."""
+    nnz = np.diff(p.col_ptr)
+    order = np.argsort(nnz, kind="stable")
+    ne = order[nnz[order] > 0]
+    cols = [ne[-1], ne[-2], ne[0], ne[len(ne) // 2]] + list(np.flatnonzero(beta)[:2])
+    if p.has_offset:
+        cols = [c for c in cols if c != 0]
+    return np.array(sorted(set(int(c) for c in cols)), dtype=np.int32)
+
+
+def _check_per_coordinate(p, beta, d):
+    cols = _probe_columns(p, beta)
+    ll_c, g_c, h_c = orc.evaluate_cols(p, beta, cols)
+    fh = api.FitHandle(d)
+    try:
+        fh.set_weights(None, use_cv=False)
+        fh.fixed_terms(use_cv=False)
+        fh.compute_xbeta(beta)
+        fh.remaining_stats(use_w=False)
+        gh = np.array([fh.grad_hess(int(j)) for j in cols])
+        np.testing.assert_allclose(fh.loglik(), ll_c, rtol=1e-11)
+    finally:
+        fh.close()
+    np.testing.assert_allclose(gh[:, 0], g_c, rtol=1e-9, atol=1e-9 * np.abs(g_c).max())
+    np.testing.assert_allclose(gh[:, 1], h_c, rtol=1e-9)
+
+
+def test_full_lr_plain_and_batched():
+    p, _ = synth.lr_indicator(seed=SEED + 1)
+    sp = default_spec(p, prior=PRIOR_LAPLACE, variance=1.0)
+    bc, ic = orc.fit(p, sp)
+    d = api.Data(p)
+    bg, ig = api.fit_plain(d, sp)
+    check_fit(bg, ig, bc, ic)
+    bb, ib = api.fit_batch(d, [sp])
+    check_fit(bb[0], ib[0], bc, ic)
+    d.close()
+
+
+def test_full_poisson_plain_trajectory(poisson_full):
+    p, sp, bc, ic = poisson_full
+    assert p.col_ptr[-1] > 3.5e7
+    d = api.Data(p)
+    bg, ig = api.fit_plain(d, sp)
+    check_fit(bg, ig, bc, ic)
+    _check_per_coordinate(p, bc, d)
+    d.close()
+
+
+def test_full_poisson_batched_trajectory(poisson_full):
+    p, sp, bc, ic = poisson_full
+    d = api.Data(p)
+    bb, ib = api.fit_batch(d, [sp, sp])
+    st = api.batch_last_stats()
+    for i in range(2):
+        check_fit(bb[i], ib[i], bc, ic)
+    assert max(t[2] for t in st["timeline"]) > 1  # a fit this size runs on a gang
+    d.close()
+
+
+def test_full_poisson_mixed_precision(poisson_full):
+    p, sp, bc, ic = poisson_full
+    d = api.Data(p, precision=api.MIXED)
+    bg, ig = api.fit_plain(d, sp)
+    check_fit(bg, ig, bc, ic, rtol=2e-4, atol=2e-5, ll_rtol=1e-6, same_path=False)
+    d.close()
+
+
+def test_full_cox_plain(cox_full):
+    p, sp, bc, ic = cox_full
+    assert p.K == 2_000_000 and p.col_ptr[-1] > 5e7
+    d = api.Data(p)
+    bg, ig = api.fit_plain(d, sp)
+    check_fit(bg, ig, bc, ic)
+    _check_per_coordinate(p, bc, d)
+    d.close()
+
+
+def test_full_cox_batched(cox_full):
+    p, sp, bc, ic = cox_full
+    d = api.Data(p)
+    bb, ib = api.fit_batch(d, [sp])
+    check_fit(bb[0], ib[0], bc, ic)
+    d.close()
+
+
+def test_full_cox_mixed_precision(cox_full):
+    p, sp, bc, ic = cox_full
+    d = api.Data(p, precision=api.MIXED)
+    bg, ig = api.fit_plain(d, sp)
+    check_fit(bg, ig, bc, ic, rtol=2e-4, atol=2e-5, ll_rtol=1e-6, same_path=False)
+    d.close()
+
+
+def test_full_clr_cv_batched():
+    p, _ = synth.clr_strata(seed=SEED + 4)
+    assert p.K > 1_200_000
+    specs, held = [], []
+    for fold, step in ((0, 0), (1, 3)):
+        w = orc.cv_weights(p.strata, 10, 123, fold, by_pid=True)
+        var = orc.grid_point(1e-4, 1.0, 13, step)  # 1e-4, 1e-3 on the log grid
+        specs.append(default_spec(p, prior=PRIOR_LAPLACE, variance=var, weights=w))
+        held.append(1.0 - w)
+    d = api.Data(p)
+    bb, ib = api.fit_batch(d, specs, heldouts=held)
+    for i, sp in enumerate(specs):
+        bc, ic = orc.fit(p, sp, heldout=held[i])
+        check_fit(bb[i], ib[i], bc, ic)
+        np.testing.assert_allclose(ib[i]["pred_log_likelihood"], ic["pred_log_likelihood"],
+                                   rtol=1e-8)
+    d.close()

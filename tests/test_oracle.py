@@ -125,6 +125,12 @@ def test_oracle_gradient_hessian_closed_form(name, p, weighted):
         np.testing.assert_allclose(ll, ll_i, rtol=1e-11, atol=1e-11)
         np.testing.assert_allclose(g, -score[j], rtol=1e-10, atol=1e-10)
         np.testing.assert_allclose(h, -H[j, j], rtol=1e-10, atol=1e-10)
+    # every column from one set of statistics (the full-size checks use this form)
+    cols = np.arange(1 if p.has_offset else 0, p.J)[::-1]
+    ll, gs, hs = orc.evaluate_cols(p, beta, cols, weights=w)
+    np.testing.assert_allclose(ll, ll_i, rtol=1e-11, atol=1e-11)
+    np.testing.assert_allclose(gs, -score[cols], rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(hs, -np.diag(H)[cols], rtol=1e-10, atol=1e-10)
 
 
 def _brute_min(f, lo=-6, hi=6):
