@@ -48,41 +48,42 @@ using namespace ccdamd;
 
 namespace {
 
-constexpr int kItems = 4;  // Cox pid items per lane per scan chunk
+constexpr int kItems = 4;  // Cox pids loaded together per lane in a scan pass
 
 // ------------------------------------------------------------ Cox pieces
-
+// Is nonzero p (of a column starting at cb) the first of its pid's run within
+// [lo, p]?  Rows are sorted by pid; rows of pid N (zero weight) are skipped.
 template <typename R>
-__device__ __forceinline__ bool coxHead(const DevX& d, int fmt, int64_t b, int64_t p,
+__device__ __forceinline__ bool coxHead(const DevX& d, int fmt, int64_t cb, int64_t lo, int64_t p,
                                         const int32_t* pid, int N, int* myPid) {
-  const int pp = pid[colRow(d, fmt, p, b)];
+  const int pp = pid[colRow(d, fmt, p, cb)];
   *myPid = pp;
   if (pp >= N) return false;
-  for (int64_t q = p - 1; q >= b; --q) {
-    const int pq = pid[colRow(d, fmt, q, b)];
+  for (int64_t q = p - 1; q >= lo; --q) {
+    const int pq = pid[colRow(d, fmt, q, cb)];
     if (pq < N) return pq != pp;
   }
   return true;
 }
 
-// Cox inside one workgroup: numerators by run heads, then one pass over all
-// pids with a block-level segmented scan carrying across chunks.
+// Numerators of the pids a column touches, from its nonzeros [b, e) (column
+// start cb): the first nonzero of each pid's run sums the run
+// (computeNumeratorForGradient, ModelSpecifics.hpp:1353-1464).  A part [b, e)
+// starts and ends at tie-group boundaries, so no run crosses it.
 template <typename R>
-__device__ __forceinline__ void ghCox(const Slot<R>& s, int j, int fmt, int useW, double (&v)[2]) {
+__device__ __forceinline__ void coxNumerators(const Slot<R>& s, int fmt, int64_t cb, int64_t b,
+                                              int64_t e, int useW) {
   const DevX& d = s.d;
   const double* kw = s.kw;
   const int32_t* pid = s.pid;
   const int N = s.N;
-  double* numer = s.numer;
-  double* numer2 = s.numer2;
-  const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
   const bool ind = isIndicator(fmt);
   for (int64_t p = b + threadIdx.x; p < e; p += blockDim.x) {
     int pp;
-    if (!coxHead<R>(d, fmt, b, p, pid, N, &pp)) continue;
+    if (!coxHead<R>(d, fmt, cb, b, p, pid, N, &pp)) continue;
     double n1 = 0.0, n2 = 0.0;
     for (int64_t q = p; q < e; ++q) {
-      const int k = colRow(d, fmt, q, b);
+      const int k = colRow(d, fmt, q, cb);
       const int pq = pid[k];
       if (pq >= N) continue;
       if (pq != pp) break;
@@ -91,69 +92,27 @@ __device__ __forceinline__ void ghCox(const Slot<R>& s, int j, int fmt, int useW
       n1 += useW ? kw[k] * (o * x) : o * x;
       if (!ind) n2 += useW ? kw[k] * (o * x * x) : o * x * x;
     }
-    numer[pp] = n1;
-    if (!ind) numer2[pp] = n2;
-  }
-  __syncthreads();
-  Seg<3> carry;
-  carry.f = 0;
-  carry.v[0] = carry.v[1] = carry.v[2] = 0.0;
-  const int chunk = blockDim.x * kItems;
-  for (int c0 = 0; c0 < N; c0 += chunk) {
-    const int base = c0 + threadIdx.x * kItems;
-    Seg<3> items[kItems];
-    Seg<3> agg;
-    agg.f = 0;
-    agg.v[0] = agg.v[1] = agg.v[2] = 0.0;
-#pragma unroll
-    for (int it = 0; it < kItems; ++it) {
-      const int i = base + it;
-      Seg<3> x;
-      x.f = 0;
-      x.v[0] = x.v[1] = x.v[2] = 0.0;
-      if (i < N) {
-        x.f = s.reset[i];
-        x.v[0] = s.grp[i].den;
-        x.v[1] = numer[i];
-        x.v[2] = ind ? 0.0 : numer2[i];
-      }
-      items[it] = x;
-      agg = segCombine(agg, x);
-    }
-    Seg<3> total;
-    Seg<3> run = segCombine(carry, blockSegScanExcl<3>(agg, total));
-#pragma unroll
-    for (int it = 0; it < kItems; ++it) {
-      const int i = base + it;
-      if (i >= N) break;
-      run = segCombine(run, items[it]);
-      const double accD = run.v[0];
-      const double t = run.v[1] / accD;
-      const double wn = s.grp[i].nw;
-      const double g = wn * t;
-      v[0] += g;
-      v[1] += ind ? g * (1.0 - t) : wn * (run.v[2] / accD - t * t);
-      if (items[it].v[1] != 0.0) numer[i] = 0.0;
-      if (!ind && items[it].v[2] != 0.0) numer2[i] = 0.0;
-    }
-    carry = segCombine(carry, total);
+    s.numer[pp] = n1;
+    if (!ind) s.numer2[pp] = n2;
   }
 }
 
+// xBeta, exp(xBeta) of the rows of nonzeros [b, e) and the denominators of
+// their pids (updateXBeta, ModelSpecifics.hpp:1524-1586).
 template <typename R>
-__device__ __forceinline__ void updCox(const Slot<R>& s, int j, int fmt, int useW, double delta) {
+__device__ __forceinline__ void updCox(const Slot<R>& s, int fmt, int64_t cb, int64_t b, int64_t e,
+                                       int useW, double delta) {
   const DevX& d = s.d;
   RowSt<R>* rowSt = s.row;
   const double* kw = s.kw;
   const int32_t* pid = s.pid;
   const int N = s.N;
-  const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
   const R rd = (R)delta;
   for (int64_t p = b + threadIdx.x; p < e; p += blockDim.x) {
     int pp;
-    if (!coxHead<R>(d, fmt, b, p, pid, N, &pp)) {
+    if (!coxHead<R>(d, fmt, cb, b, p, pid, N, &pp)) {
       if (pp >= N) {
-        const int k = colRow(d, fmt, p, b);
+        const int k = colRow(d, fmt, p, cb);
         const R nx = rowSt[k].x + rd * colValue<R>(d, fmt, p);
         rowSt[k] = RowSt<R>{nx, rexp(nx)};
       }
@@ -161,7 +120,7 @@ __device__ __forceinline__ void updCox(const Slot<R>& s, int j, int fmt, int use
     }
     double den = s.grp[pp].den;
     for (int64_t q = p; q < e; ++q) {
-      const int k = colRow(d, fmt, q, b);
+      const int k = colRow(d, fmt, q, cb);
       const int pq = pid[k];
       if (pq >= N) continue;
       if (pq != pp) break;
@@ -186,48 +145,6 @@ __device__ __forceinline__ bool quantumOver(long long deadline) {
   if (threadIdx.x == 0) sOver = wall_clock64() >= deadline;
   __syncthreads();
   return sOver != 0;
-}
-
-// One coordinate cycle of a Cox fit inside one workgroup, from coordinate j0.
-// Returns the coordinate to resume at when the quantum ran out, else J.
-template <typename R>
-__device__ __forceinline__ int cycleCox(const Slot<R>& s, FitStatus& cs, int j0, long long deadline) {
-  const DevX& d = s.d;
-  const int useW = s.ctl.useW;
-  __shared__ double sDelta;
-  for (int j = j0; j < s.J; ++j) {
-    if (j > j0 && (j & 3) == 0 && quantumOver(deadline)) return j;
-    if (s.fixed[j]) continue;
-    const int64_t nj = d.colPtr[j + 1] - d.colPtr[j];
-    const int fmt = d.fmt[j];
-    double v[2] = {0.0, 0.0};
-    if (nj > 0) {
-      ghCox<R>(s, j, fmt, useW, v);
-      blockSum<2>(v);
-    }
-    if (threadIdx.x == 0) {
-      cs.gatherNnz += nj;
-      if (nj > 0) cs.scanPids += s.N;
-      double g = nj > 0 ? v[0] - s.xjy[j] : 0.0;
-      double h = nj > 0 ? v[1] : 0.0;
-      if (h < 0.0) {
-        g = 0.0;
-        h = 0.0;
-      }
-      double delta = priorDelta(s.ptype[j], s.pvar[j], g, h, s.beta[j]);
-      delta = applyBounds(delta, &s.bound[j]);
-      if (delta != 0.0) {
-        s.beta[j] += delta;
-        cs.updateNnz += nj;
-      }
-      sDelta = delta;
-    }
-    __syncthreads();
-    const double delta = sDelta;
-    if (delta != 0.0) updCox<R>(s, j, fmt, useW, delta);
-    __syncthreads();
-  }
-  return s.J;
 }
 
 // ------------------------------------------------------------ column passes
@@ -713,6 +630,279 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus& cs, LDS& S,
   return jEnd;
 }
 
+// ------------------------------------------------------------ Cox risk-set scans
+template <int NV>
+__device__ __forceinline__ Seg<NV> segZero() {
+  Seg<NV> z;
+  z.f = 0;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) z.v[i] = 0.0;
+  return z;
+}
+
+// Carry into this rank of a gang-wide segmented scan: every rank publishes the
+// segmented total of its pid range and receives the combination, in rank
+// order, of the totals of the ranks before it (the identity on rank 0).  Same
+// publication protocol as gangExchange; the two share the exchange rounds.
+template <int NV>
+__device__ __forceinline__ bool gangCarry(Gang& g, const Seg<NV>& total, Seg<NV>& carry) {
+  static_assert(NV + 1 <= kXchSum, "exchange too wide");
+  __shared__ Seg<NV> sCarry[2];
+  __shared__ int sOk[2];
+  const int par = (int)(g.seq & 1u);
+  if (threadIdx.x == 0) {
+    double* mine = &g.x->v[par][g.rank][0];
+    double chk = 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) chk += publishShared(mine + i, total.v[i]);
+    chk += publishShared(mine + NV, (double)total.f);
+    asm volatile("; swap results %0" : : "v"(chk));
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __hip_atomic_fetch_add(&g.x->cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned target = (unsigned)g.G * (g.seq + 1u);
+    long long spins = 0;
+    int ok = 1;
+    while (__hip_atomic_load(&g.x->cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > kSpinLimit) {
+        ok = 0;
+        break;
+      }
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    sOk[par] = ok;
+  }
+  __syncthreads();
+  const bool ok = sOk[par] != 0;
+  if (threadIdx.x < kWave) {
+    const int lane = threadIdx.x;
+    Seg<NV> x = segZero<NV>();
+    if (ok && lane < g.rank) {
+      const double* src = &g.x->v[par][lane][0];
+#pragma unroll
+      for (int i = 0; i < NV; ++i) x.v[i] = loadShared(src + i);
+      x.f = loadShared(src + NV) != 0.0 ? 1 : 0;
+    }
+    // inclusive scan over lanes in lane (= rank) order
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      Seg<NV> o = segShflUp(x, off);
+      if (lane >= off) x = segCombine(o, x);
+    }
+    if (lane == kWave - 1) sCarry[par] = x;  // ranks >= g.rank contributed the identity
+  }
+  __syncthreads();
+  carry = sCarry[par];
+  g.seq++;
+  g.ok = g.ok && ok;
+  return ok;
+}
+
+// Thread t's contiguous block of the pids [pLo, pHi).
+__device__ __forceinline__ void pidBlock(int pLo, int pHi, int* a, int* z) {
+  const int n = pHi - pLo;
+  const int per = (n + (int)blockDim.x - 1) / (int)blockDim.x;
+  *a = pLo + min(n, (int)threadIdx.x * per);
+  *z = pLo + min(n, ((int)threadIdx.x + 1) * per);
+}
+
+// Gradient and Hessian terms of one coordinate over this rank's pids [pLo, pHi)
+// (computeGradientAndHessianImpl's cumulative branch, ModelSpecifics.hpp:904-1020,
+// with BreslowTiedCoxProportionalHazards::incrementGradientAndHessian,
+// ModelSpecifics.h:1185-1202).  The accumulated denominator and numerators are a
+// segmented prefix sum over pids that restarts at each stratum
+// (computeAccumlatedDenominator, ModelSpecifics.hpp:1670-1694).  Each thread
+// owns a contiguous block of pids: pass 1 folds its block into one segmented
+// total, a block scan of the totals gives each thread its carry (and a gang
+// scan across ranks the rank's carry), and pass 2 walks the block again with the
+// running sums, adding nW*t and the Hessian term to v and clearing the
+// numerators for the next coordinate.  Two block barriers per coordinate.
+template <typename R>
+__device__ __forceinline__ bool coxScan(const Slot<R>& s, int pLo, int pHi, bool ind, Gang& gg,
+                                        double (&v)[2]) {
+  const GrpSt* grp = s.grp;
+  double* numer = s.numer;
+  double* numer2 = s.numer2;
+  const uint8_t* reset = s.reset;
+  int a, z;
+  pidBlock(pLo, pHi, &a, &z);
+  Seg<3> agg = segZero<3>();
+  for (int i0 = a; i0 < z; i0 += kItems) {
+    Seg<3> it[kItems];
+#pragma unroll
+    for (int u = 0; u < kItems; ++u) {
+      const int i = i0 + u;
+      it[u] = segZero<3>();
+      if (i < z) {
+        it[u].f = reset[i];
+        it[u].v[0] = grp[i].den;
+        it[u].v[1] = numer[i];
+        it[u].v[2] = ind ? 0.0 : numer2[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kItems; ++u) agg = segCombine(agg, it[u]);
+  }
+  Seg<3> total;
+  Seg<3> run = blockSegScanExcl<3>(agg, total);
+  if (gg.G > 1) {
+    Seg<3> carry;
+    if (!gangCarry<3>(gg, total, carry)) return false;
+    run = segCombine(carry, run);
+  }
+  for (int i0 = a; i0 < z; i0 += kItems) {
+    Seg<3> it[kItems];
+    double wn[kItems];
+#pragma unroll
+    for (int u = 0; u < kItems; ++u) {
+      const int i = i0 + u;
+      it[u] = segZero<3>();
+      wn[u] = 0.0;
+      if (i < z) {
+        const GrpSt gs = grp[i];
+        it[u].f = reset[i];
+        it[u].v[0] = gs.den;
+        it[u].v[1] = numer[i];
+        it[u].v[2] = ind ? 0.0 : numer2[i];
+        wn[u] = gs.nw;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kItems; ++u) {
+      const int i = i0 + u;
+      if (i >= z) break;
+      run = segCombine(run, it[u]);
+      const double accD = run.v[0];
+      const double t = run.v[1] / accD;
+      const double g = wn[u] * t;
+      v[0] += g;
+      v[1] += ind ? g * (1.0 - t) : wn[u] * (run.v[2] / accD - t * t);
+      if (it[u].v[1] != 0.0) numer[i] = 0.0;
+      if (!ind && it[u].v[2] != 0.0) numer2[i] = 0.0;
+    }
+  }
+  return true;
+}
+
+// Sum of nW_i * log(accumulated denominator_i) over this rank's pids (the Cox
+// term of getLogLikelihood, ModelSpecifics.hpp:624-628), by the same two-pass
+// scan; the per-thread part is returned (not block-summed).  gg == nullptr:
+// the whole fit in one workgroup.  On a failed gang exchange gg->ok turns false.
+template <typename R>
+__device__ __forceinline__ double coxLogDen(const Slot<R>& s, int pLo, int pHi, Gang* gg) {
+  const GrpSt* grp = s.grp;
+  const uint8_t* reset = s.reset;
+  int a, z;
+  pidBlock(pLo, pHi, &a, &z);
+  Seg<1> agg = segZero<1>();
+  for (int i = a; i < z; ++i) {
+    Seg<1> x;
+    x.f = reset[i];
+    x.v[0] = grp[i].den;
+    agg = segCombine(agg, x);
+  }
+  Seg<1> total;
+  Seg<1> run = blockSegScanExcl<1>(agg, total);
+  if (gg != nullptr && gg->G > 1) {
+    Seg<1> carry;
+    if (!gangCarry<1>(*gg, total, carry)) return 0.0;
+    run = segCombine(carry, run);
+  }
+  double acc = 0.0;
+  for (int i = a; i < z; ++i) {
+    const GrpSt gs = grp[i];
+    Seg<1> x;
+    x.f = reset[i];
+    x.v[0] = gs.den;
+    run = segCombine(run, x);
+    if (gs.nw != 0.0) acc += gs.nw * log(run.v[0]);
+  }
+  return acc;
+}
+
+// One coordinate cycle of a Cox fit over this rank's part, from coordinate j0:
+// numerators of its rows, the risk-set scan of its pids (carry across the
+// gang), one exchange of (gradient, Hessian), the prior step, the update of its
+// rows and pids.  Rank r owns the rows of its shards and the pids of those rows
+// (shards start at tie groups, so no pid spans two ranks).  Returns as cycleFit.
+template <typename R>
+__device__ __forceinline__ int cycleCox(const Slot<R>& s, FitStatus& cs, int j0, long long deadline,
+                                        Gang& gg) {
+  const DevX& d = s.d;
+  const int useW = s.ctl.useW;
+  const bool multi = gg.G > 1;
+  const bool lead = gg.rank == 0;
+  const int pLo = multi ? s.pidShard[gg.s0] : 0;
+  const int pHi = multi ? s.pidShard[gg.s1] : s.N;
+  long long cG = 0, cU = 0, cP = 0;
+  int jEnd = s.J;
+  for (int j = j0; j < s.J; ++j) {
+    if (!multi && j > j0 && (j & 3) == 0 && quantumOver(deadline)) {
+      jEnd = j;
+      break;
+    }
+    if (s.fixed[j]) continue;
+    const int64_t cb = d.colPtr[j], ce = d.colPtr[j + 1];
+    int64_t b = cb, e = ce;
+    if (multi) {
+      const int32_t* off = d.colShard + (size_t)j * (d.nShards + 1);
+      b = cb + off[gg.s0];
+      e = cb + off[gg.s1];
+    }
+    const int64_t njAll = ce - cb;
+    const int fmt = d.fmt[j];
+    // in a gang, only the lead rank's reads count (it publishes them)
+    double bj = s.beta[j];
+    double bnd = s.bound[j];
+    const double pv = s.pvar[j], xy = s.xjy[j];
+    const int pt = s.ptype[j];
+    double v[2] = {0.0, 0.0};
+    bool stop = false;
+    if (njAll > 0) {
+      coxNumerators<R>(s, fmt, cb, b, e, useW);
+      __syncthreads();
+      if (!coxScan<R>(s, pLo, pHi, isIndicator(fmt), gg, v)) return -1;
+      cP += pHi - pLo;
+    }
+    blockSum<2>(v);
+    if (multi) {
+      double ld[3] = {bj, bnd, (deadline > 0 && wall_clock64() >= deadline) ? 1.0 : 0.0};
+      if (!gangExchange<2, 3>(gg, v, ld)) return -1;
+      bj = ld[0];
+      bnd = ld[1];
+      stop = ld[2] != 0.0;
+    }
+    double g = njAll > 0 ? v[0] - xy : 0.0, h = njAll > 0 ? v[1] : 0.0;
+    if (h < 0.0) {
+      g = 0.0;
+      h = 0.0;
+    }
+    const double delta = applyBounds(priorDelta(pt, pv, g, h, bj), &bnd);
+    if (threadIdx.x == 0 && lead) {
+      s.bound[j] = bnd;
+      if (delta != 0.0) s.beta[j] = bj + delta;
+    }
+    cG += e - b;
+    if (delta != 0.0) {
+      updCox<R>(s, fmt, cb, b, e, useW, delta);
+      cU += e - b;
+    }
+    __syncthreads();
+    if (stop) {
+      jEnd = j + 1;
+      break;
+    }
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd((unsigned long long*)&cs.gatherNnz, (unsigned long long)cG);
+    atomicAdd((unsigned long long*)&cs.updateNnz, (unsigned long long)cU);
+    atomicAdd((unsigned long long*)&cs.scanPids, (unsigned long long)cP);
+  }
+  return jEnd;
+}
+
 // ------------------------------------------------------------ whole-fit pieces
 // Rows [rowLo, rowHi) and groups [grpLo, grpHi) a workgroup sums over.
 struct Part {
@@ -725,12 +915,12 @@ __device__ __forceinline__ Part wholeFit(const Slot<R>& s) {
 }
 
 // log likelihood terms of one fit over a part (ModelSpecifics.hpp:605-650);
-// accumulated denominators for Cox come from an in-block scan (whole fit only)
+// Cox: the accumulated denominators of the part's pids [grpLo, grpHi) come from
+// a segmented scan carried across the gang `gg` (nullptr: one workgroup)
 template <int MODEL, typename R>
-__device__ __forceinline__ double fitLogLik(const Slot<R>& s, const Part& pt) {
+__device__ __forceinline__ double fitLogLik(const Slot<R>& s, const Part& pt, Gang* gg = nullptr) {
   const DevX& d = s.d;
   const int useW = s.ctl.useW;
-  const int N = s.N;
   double v[2] = {0.0, 0.0};
   for (int k = pt.rowLo + threadIdx.x; k < pt.rowHi; k += blockDim.x) {
     const RowSt<R> rs = s.row[k];
@@ -748,42 +938,7 @@ __device__ __forceinline__ double fitLogLik(const Slot<R>& s, const Part& pt) {
       if (gs.nw != 0.0) v[1] += gs.nw * log(gs.den);
     }
   }
-  if (MODEL == M_COX) {
-    Seg<3> carry;
-    carry.f = 0;
-    carry.v[0] = carry.v[1] = carry.v[2] = 0.0;
-    const int chunk = blockDim.x * kItems;
-    for (int c0 = 0; c0 < N; c0 += chunk) {
-      const int base = c0 + threadIdx.x * kItems;
-      Seg<3> items[kItems];
-      Seg<3> agg;
-      agg.f = 0;
-      agg.v[0] = agg.v[1] = agg.v[2] = 0.0;
-#pragma unroll
-      for (int it = 0; it < kItems; ++it) {
-        const int i = base + it;
-        Seg<3> x;
-        x.f = 0;
-        x.v[0] = x.v[1] = x.v[2] = 0.0;
-        if (i < N) {
-          x.f = s.reset[i];
-          x.v[0] = s.grp[i].den;
-        }
-        items[it] = x;
-        agg = segCombine(agg, x);
-      }
-      Seg<3> total;
-      Seg<3> run = segCombine(carry, blockSegScanExcl<3>(agg, total));
-#pragma unroll
-      for (int it = 0; it < kItems; ++it) {
-        const int i = base + it;
-        if (i >= N) break;
-        run = segCombine(run, items[it]);
-        v[1] += s.grp[i].nw * log(run.v[0]);
-      }
-      carry = segCombine(carry, total);
-    }
-  }
+  if (MODEL == M_COX) v[1] += coxLogDen<R>(s, pt.grpLo, pt.grpHi, gg);
   blockSum<2>(v);
   return v[0] - v[1];
 }
@@ -833,9 +988,16 @@ __device__ __forceinline__ bool gangObjective(const Slot<R>& s, Gang& gg, long l
     pt.grpLo = s.d.strata[gg.rowLo];
     pt.grpHi = s.d.strata[gg.rowHi - 1] + 1;
   }
+  if (MODEL == M_COX) {
+    pt.grpLo = s.pidShard[gg.s0];
+    pt.grpHi = s.pidShard[gg.s1];
+  }
   double v[1] = {0.0};
   if (ct == 0) v[0] = fitGradObjective<R>(s, pt);
-  if (ct == 1 || ct == 2) v[0] = fitLogLik<MODEL, R>(s, pt);
+  if (ct == 1 || ct == 2) {
+    v[0] = fitLogLik<MODEL, R>(s, pt, &gg);
+    if (!gg.ok) return false;
+  }
   // the coefficients belong to the lead rank: it adds the log prior
   double ld[2] = {0.0, (deadline > 0 && wall_clock64() >= deadline) ? 1.0 : 0.0};
   if (ct == 1 && gg.rank == 0) ld[0] = priorPart(s.beta, s.pvar, s.ptype, s.J);
@@ -848,33 +1010,46 @@ __device__ __forceinline__ bool gangObjective(const Slot<R>& s, Gang& gg, long l
   return true;
 }
 
-// Per-fit set-up: event weights, XjY, xBeta from the starting coefficients,
-// exp / denominators, bounds and the starting objective (findMode's lazy part).
+// Per-fit set-up (findMode's lazy part), one rank of G over the row shards
+// [r*S/G, (r+1)*S/G): event weights of the rank's groups, XjY of every G-th
+// coordinate, xBeta of its rows from the starting coefficients (column order,
+// as computeXBeta), exp and the denominators of its groups, bounds of every G-th
+// coordinate.  Groups and rows of a rank are its own (shards start at strata /
+// tie groups), so ranks need no exchange and the result does not depend on G.
 template <int MODEL, typename R>
-__device__ __forceinline__ void initSlot(const Slot<R>& s, FitStatus& st, bool full) {
+__device__ __forceinline__ void initPart(const Slot<R>& s, bool full, int rank, int G) {
   const DevX& d = s.d;
   const int useW = s.ctl.useW;
-  const int N = s.N;
   RowSt<R>* rowSt = s.row;
   const double* kw = s.kw;
+  const int s0 = rank * d.nShards / G, s1 = (rank + 1) * d.nShards / G;
+  const int rowLo = d.shardRow[s0], rowHi = d.shardRow[s1];
+  // the rank's groups tile [0, groups): from its first row's group to the next rank's
+  int gLo = 0, gHi = 0;
+  if (MODEL == M_CLR) {
+    auto first = [&](int sh) { return sh == 0 ? 0 : (d.shardRow[sh] < s.K ? d.strata[d.shardRow[sh]] : d.nStrata); };
+    gLo = first(s0);
+    gHi = s1 == d.nShards ? d.nStrata : first(s1);
+  } else if (MODEL == M_COX) {
+    gLo = s.pidShard[s0];
+    gHi = s.pidShard[s1];
+  }
   // nW and XjY depend on the weights only: a restart keeps them
-  if (!full) {
-  } else if (MODEL == M_CLR) {
-    for (int g = threadIdx.x; g < d.nStrata; g += blockDim.x) {
+  if (full && MODEL == M_CLR) {
+    for (int g = gLo + threadIdx.x; g < gHi; g += blockDim.x) {
       double acc = 0.0;
       for (int k = d.stratStart[g]; k < d.stratStart[g + 1]; ++k) acc += d.y[k] * kw[k];
       s.grp[g].nw = acc;
     }
-  } else if (MODEL == M_COX) {
-    for (int p = threadIdx.x; p < N; p += blockDim.x) {
+  } else if (full && MODEL == M_COX) {
+    for (int p = gLo + threadIdx.x; p < gHi; p += blockDim.x) {
       double acc = 0.0;
       for (int k = s.pidFirst[p]; k < s.pidFirst[p + 1]; ++k)
         if (s.pid[k] == p) acc += d.y[k] * kw[k];
       s.grp[p].nw = acc;
     }
   }
-  // XjY
-  for (int j = 0; full && j < s.J; ++j) {
+  for (int j = rank; full && j < s.J; j += G) {
     const int fmt = d.fmt[j];
     const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
     double t[1] = {0.0};
@@ -886,24 +1061,26 @@ __device__ __forceinline__ void initSlot(const Slot<R>& s, FitStatus& st, bool f
     blockSum<1>(t);
     if (threadIdx.x == 0) s.xjy[j] = t[0];
   }
-  // xBeta = sum_j beta_j X_j in column order
-  for (int k = threadIdx.x; k < s.K; k += blockDim.x) rowSt[k].x = (R)0;
+  // xBeta = sum_j beta_j X_j in column order, over the rank's part of each column
+  for (int k = rowLo + threadIdx.x; k < rowHi; k += blockDim.x) rowSt[k].x = (R)0;
   __syncthreads();
   for (int j = 0; j < s.J; ++j) {
     const double bj = s.beta[j];
     if (bj == 0.0) continue;
     const int fmt = d.fmt[j];
-    const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
+    const int64_t cb = d.colPtr[j];
+    const int32_t* off = d.colShard + (size_t)j * (d.nShards + 1);
+    const int64_t b = cb + off[s0], e = cb + off[s1];
     const R rb = (R)bj;
     for (int64_t p = b + threadIdx.x; p < e; p += blockDim.x) {
-      const int k = colRow(d, fmt, p, b);
+      const int k = colRow(d, fmt, p, cb);
       rowSt[k].x += rb * colValue<R>(d, fmt, p);
     }
     __syncthreads();
   }
   // exp and denominators
   if (MODEL == M_CLR) {
-    for (int g = threadIdx.x; g < d.nStrata; g += blockDim.x) {
+    for (int g = gLo + threadIdx.x; g < gHi; g += blockDim.x) {
       double acc = 0.0;
       for (int k = d.stratStart[g]; k < d.stratStart[g + 1]; ++k) {
         const R o = rexp(rowSt[k].x);
@@ -913,10 +1090,10 @@ __device__ __forceinline__ void initSlot(const Slot<R>& s, FitStatus& st, bool f
       s.grp[g].den = acc;
     }
   } else {
-    for (int k = threadIdx.x; k < s.K; k += blockDim.x) rowSt[k].e = rexp(rowSt[k].x);
+    for (int k = rowLo + threadIdx.x; k < rowHi; k += blockDim.x) rowSt[k].e = rexp(rowSt[k].x);
     if (MODEL == M_COX) {
       __syncthreads();
-      for (int p = threadIdx.x; p < N; p += blockDim.x) {
+      for (int p = gLo + threadIdx.x; p < gHi; p += blockDim.x) {
         double acc = 0.0;
         for (int k = s.pidFirst[p]; k < s.pidFirst[p + 1]; ++k) {
           if (s.pid[k] != p) continue;
@@ -926,8 +1103,13 @@ __device__ __forceinline__ void initSlot(const Slot<R>& s, FitStatus& st, bool f
       }
     }
   }
-  for (int j = threadIdx.x; j < s.J; j += blockDim.x) s.bound[j] = s.ctl.initialBound;
-  __syncthreads();
+  for (int j = rank * (int)blockDim.x + threadIdx.x; j < s.J; j += G * (int)blockDim.x)
+    s.bound[j] = s.ctl.initialBound;
+}
+
+// The starting objective and status of a set-up fit (one workgroup).
+template <int MODEL, typename R>
+__device__ __forceinline__ void initObjective(const Slot<R>& s, FitStatus& st) {
   const double obj = fitObjective<MODEL, R>(s);
   if (threadIdx.x == 0) {
     st.lastObj = obj;
@@ -952,7 +1134,7 @@ __device__ __forceinline__ void sweepSlot(const Slot<R>& s, FitStatus& st, LDS& 
   int iteration = st.iteration;
   double lastObj = st.lastObj;
   for (int sw = 0; sw < sweeps; ++sw) {
-    const int jEnd = MODEL == M_COX ? cycleCox<R>(s, st, j0, deadline)
+    const int jEnd = MODEL == M_COX ? cycleCox<R>(s, st, j0, deadline, gg)
                                     : cycleFit<MODEL, R, ITL>(s, st, S, j0, deadline, gg);
     __syncthreads();
     if (jEnd < 0) {
@@ -1038,15 +1220,29 @@ __device__ __forceinline__ void finishSlot(const Slot<R>& s, FitStatus& st) {
 // Each kernel runs one workgroup per listed slot and dispatches on the slot's
 // model (uniform per workgroup).
 
+// `ranks` workgroups per listed slot, each setting up its shards.
 template <typename R>
-__global__ void k_slot_init(const Slot<R>* slots, FitStatus* st, const int32_t* active, int full) {
+__global__ void k_slot_init(const Slot<R>* slots, const int32_t* active, int full, int ranks) {
+  const int f = active[blockIdx.x / ranks];
+  const int rank = blockIdx.x % ranks;
+  const Slot<R>& s = slots[f];
+  switch (s.model) {
+    case M_LOGISTIC: initPart<M_LOGISTIC, R>(s, full != 0, rank, ranks); break;
+    case M_POISSON: initPart<M_POISSON, R>(s, full != 0, rank, ranks); break;
+    case M_CLR: initPart<M_CLR, R>(s, full != 0, rank, ranks); break;
+    case M_COX: initPart<M_COX, R>(s, full != 0, rank, ranks); break;
+  }
+}
+
+template <typename R>
+__global__ void k_slot_objective(const Slot<R>* slots, FitStatus* st, const int32_t* active) {
   const int f = active[blockIdx.x];
   const Slot<R>& s = slots[f];
   switch (s.model) {
-    case M_LOGISTIC: initSlot<M_LOGISTIC, R>(s, st[f], full != 0); break;
-    case M_POISSON: initSlot<M_POISSON, R>(s, st[f], full != 0); break;
-    case M_CLR: initSlot<M_CLR, R>(s, st[f], full != 0); break;
-    case M_COX: initSlot<M_COX, R>(s, st[f], full != 0); break;
+    case M_LOGISTIC: initObjective<M_LOGISTIC, R>(s, st[f]); break;
+    case M_POISSON: initObjective<M_POISSON, R>(s, st[f]); break;
+    case M_CLR: initObjective<M_CLR, R>(s, st[f]); break;
+    case M_COX: initObjective<M_COX, R>(s, st[f]); break;
   }
 }
 
@@ -1135,15 +1331,27 @@ int clampThreads(int threads) { return threads <= 256 ? 256 : 512; }
 namespace ccdamd {
 
 int slotLaunchInit(int /*model*/, bool mixed, const void* slots, FitStatus* st,
-                   const int32_t* active, int n, int threads, hipStream_t s, bool full) {
+                   const int32_t* active, int n, int threads, hipStream_t s, bool full, int ranks) {
   if (n <= 0) return 0;
   threads = clampThreads(threads);
-  if (mixed)
-    k_slot_init<float><<<n, threads, 0, s>>>((const Slot<float>*)slots, st, active, full ? 1 : 0);
-  else
-    k_slot_init<double><<<n, threads, 0, s>>>((const Slot<double>*)slots, st, active, full ? 1 : 0);
+  ranks = std::max(1, std::min(ranks, kShards));
+  if (mixed) {
+    k_slot_init<float><<<n * ranks, threads, 0, s>>>((const Slot<float>*)slots, active,
+                                                     full ? 1 : 0, ranks);
+    k_slot_objective<float><<<n, threads, 0, s>>>((const Slot<float>*)slots, st, active);
+  } else {
+    k_slot_init<double><<<n * ranks, threads, 0, s>>>((const Slot<double>*)slots, active,
+                                                      full ? 1 : 0, ranks);
+    k_slot_objective<double><<<n, threads, 0, s>>>((const Slot<double>*)slots, st, active);
+  }
   HIP_TRY(hipGetLastError());
   return 0;
+}
+
+int initRanks(int64_t nnz, int K) {
+  int r = 1;
+  while (r * 2 <= kShards && nnz / (r * 2) >= (1 << 17) && K / (r * 2) >= (1 << 12)) r *= 2;
+  return r;
 }
 
 // Sweep kernel variants: (256 lanes, pass width 8 or 4), (512 lanes, width 8).
@@ -1243,7 +1451,7 @@ static int deviceCUs() {
 }
 
 int gangSize(int n, int capacity, int64_t nnz, int model, int maxGang, int64_t minNnzPerRank) {
-  if (model == CCDH_COX || n <= 0) return 1;
+  if (n <= 0) return 1;
   const int nX = xcdCount();
   if (nX <= 0) return 1;
   // a rank per CU: two ranks sharing a CU gain less than the exchange costs
@@ -1277,13 +1485,17 @@ void assignGangs(std::vector<GangEntry>& want, const std::vector<double>& priori
     grew = false;
     for (int i : order) {
       const int G = want[i].G;
-      if (model[i] == CCDH_COX || G * 2 > cap || nnz[i] / (G * 2) < minNnzPerRank) continue;
+      if (G * 2 > cap || nnz[i] / (G * 2) < minNnzPerRank) continue;
       if (used + G > budget) continue;
       want[i].G = G * 2;
       used += G;
       grew = true;
     }
   }
+}
+
+int64_t slotWork(const ccdh_data* d, int N) {
+  return d->model == CCDH_COX ? d->nnz + (int64_t)d->J * N : d->nnz;
 }
 
 int slotGangCapacity(bool mixed, int threads, int itl) {
@@ -1334,6 +1546,7 @@ SlotLayout slotLayout(int model, int K, int J, int nCap, int64_t maxLen, bool he
   L.pid = cox ? take((size_t)K * 4) : kNone;
   L.pidFirst = cox ? take(nG * 4) : kNone;
   L.reset = cox ? take(nG) : kNone;
+  L.pidShard = cox ? take((kShards + 1) * 4) : kNone;
   L.xjy = take((size_t)J * 8);
   L.beta = take((size_t)J * 8);
   L.bound = take((size_t)J * 8);
@@ -1399,6 +1612,7 @@ int slotPrepare(const ccdh_data* d, const ccdh_fit_spec& sp, char* block, const 
   x.pid = at<int32_t>(block, L.pid);
   x.pidFirst = at<int32_t>(block, L.pidFirst);
   x.reset = at<uint8_t>(block, L.reset);
+  x.pidShard = at<int32_t>(block, L.pidShard);
   x.rec.row = at<int32_t>(block, L.recRow);
   x.rec.key = at<int32_t>(block, L.recKey);
   x.rec.flags = at<uint8_t>(block, L.recFlags);
@@ -1418,6 +1632,22 @@ int slotPrepare(const ccdh_data* d, const ccdh_fit_spec& sp, char* block, const 
     HIP_TRY(hipMemcpyAsync(x.pidFirst, pidFirst.data(), (N + 1) * sizeof(int32_t),
                            hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(x.reset, reset.data(), N + 1, hipMemcpyHostToDevice, s));
+    // first pid at or after each shard's first row (pids grow with the row
+    // among rows of non-zero weight; shards start at tie groups)
+    std::vector<int32_t> pidShard(kShards + 1, N);
+    {
+      int k = K;
+      int next = N;
+      for (int sh = kShards; sh >= 0; --sh) {
+        const int lo = sh == kShards ? K : d->hShardRow[sh];
+        for (--k; k >= lo; --k)
+          if (pid[k] < N) next = pid[k];
+        k = lo;
+        pidShard[sh] = sh == kShards ? N : (sh == 0 ? 0 : next);
+      }
+    }
+    HIP_TRY(hipMemcpyAsync(x.pidShard, pidShard.data(), (kShards + 1) * sizeof(int32_t),
+                           hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(x.numer, 0, (N + 1) * sizeof(double), s));
     HIP_TRY(hipMemsetAsync(x.numer2, 0, (N + 1) * sizeof(double), s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -1572,7 +1802,7 @@ int tuneBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, int threads, int 
     for (int f = 0; f < F; ++f) all[f] = f;
     HIP_TRY(hipMemcpyAsync(dactive, all.data(), F * sizeof(int32_t), hipMemcpyHostToDevice, s));
     rc = slotLaunchInit(d->model, std::is_same<R, float>::value, dslots, dst, dactive, F, threads, s,
-                        false);
+                        false, initRanks(d->nnz, d->K));
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(s));
   }
@@ -1647,8 +1877,12 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
   HIP_TRY(hipMemcpyAsync(dslots, slots.data(), F * sizeof(Slot<R>), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(dst, st.data(), F * sizeof(FitStatus), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(dactive, active.data(), F * sizeof(int32_t), hipMemcpyHostToDevice, s));
-  rc = slotLaunchInit(model, mixed, dslots, dst, dactive, F, threads, s);
+  const int setupRanks = initRanks(d->nnz, d->K);
+  rc = slotLaunchInit(model, mixed, dslots, dst, dactive, F, threads, s, true, setupRanks);
   if (rc) return rc;
+  // work per sweep of each fit (Cox: its own tie-group count) for the gang sizes
+  std::vector<int64_t> work(F);
+  for (int f = 0; f < F; ++f) work[f] = slotWork(d, st[f].N);
 
   // Launch map: every unfinished fit, each on a gang sized by how many fits are
   // left (the tail of slow fits spreads over the idle CUs).
@@ -1674,14 +1908,16 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
   auto buildMap = [&]() -> int {
     std::vector<GangEntry> gangs;
     std::vector<double> prio;
+    std::vector<int64_t> fw;
     for (int f = 0; f < F; ++f)
       if (running[f] || useGraph) {
         gangs.push_back(GangEntry{f, 0, 1, 0});
         prio.push_back(priorStrength[f]);
+        fw.push_back(work[f]);
       }
     if (!useGraph && gangCap > 1)
-      assignGangs(gangs, prio, std::vector<int64_t>(gangs.size(), d->nnz),
-                  std::vector<int>(gangs.size(), model), capacity, gangCap, minNnz);
+      assignGangs(gangs, prio, fw, std::vector<int>(gangs.size(), model), capacity, gangCap,
+                  minNnz);
     int G = 1;
     for (const GangEntry& g : gangs) G = std::max(G, g.G);
     if (G > 1 && !layoutGangs(gangs, capacity, xcdCount(), map)) G = 1;
@@ -1807,7 +2043,7 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
         retry = true;
         active[0] = f;
         HIP_TRY(hipMemcpyAsync(dactive, active.data(), sizeof(int32_t), hipMemcpyHostToDevice, s));
-        rc = slotLaunchInit(model, mixed, dslots, dst, dactive, 1, threads, s, false);
+        rc = slotLaunchInit(model, mixed, dslots, dst, dactive, 1, threads, s, false, setupRanks);
         if (rc) return rc;
         HIP_TRY(hipStreamSynchronize(s));
         running[f] = 1;

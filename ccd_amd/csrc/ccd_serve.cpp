@@ -343,7 +343,8 @@ class Server final : public ccdh_server {
         hipMemcpyAsync(dF, &r->st0, sizeof(FitStatus), hipMemcpyHostToDevice, admitStream_) != hipSuccess ||
         hipMemcpyAsync(dA, &zero, sizeof(zero), hipMemcpyHostToDevice, admitStream_) != hipSuccess)
       return setError(CCDH_E_DEVICE, "request set-up copies");
-    rc = slotLaunchInit(M_ANY, kMixed, dS, dF, dA, 1, threads_, admitStream_);
+    rc = slotLaunchInit(M_ANY, kMixed, dS, dF, dA, 1, threads_, admitStream_, true,
+                        initRanks(r->data->nnz, r->K));
     if (rc) return rc;
     if (hipMemcpyAsync(&r->st0, dF, sizeof(FitStatus), hipMemcpyDeviceToHost, admitStream_) != hipSuccess)
       return setError(CCDH_E_DEVICE, "request set-up status");
@@ -392,7 +393,7 @@ class Server final : public ccdh_server {
   }
 
   // One launch over every fit in flight; each fit gets a gang sized by how many
-  // fits share the device and by its own nonzeros.
+  // fits share the device and by its own work (nonzeros; Cox adds its pid scans).
   int runLaunch(const std::vector<int32_t>& active) {
     std::vector<GangEntry> want;
     std::vector<double> prio;
@@ -401,8 +402,9 @@ class Server final : public ccdh_server {
     for (int32_t slot : active) {
       const Request* r = slotReq_[slot];
       want.push_back(GangEntry{slot, 0, 1, 0});
-      nnz.push_back(r->data->nnz);
-      prio.push_back((double)r->data->nnz);  // spare CUs to the largest fits
+      const int64_t w = slotWork(r->data, r->st0.N);  // Cox: nonzeros + a pid scan per coordinate
+      nnz.push_back(w);
+      prio.push_back((double)w);  // spare CUs to the largest fits
       models.push_back(r->model);
     }
     if (maxGang_ > 1) assignGangs(want, prio, nnz, models, capacity_, maxGang_, kGangMinNnz);
@@ -465,7 +467,8 @@ class Server final : public ccdh_server {
             hipMemcpyAsync(dSlots_ + slot, &s, sizeof(s), hipMemcpyHostToDevice, schedStream_) != hipSuccess ||
             hipMemcpyAsync(dOne_, &slot, sizeof(int32_t), hipMemcpyHostToDevice, schedStream_) != hipSuccess)
           return setError(CCDH_E_DEVICE, "restart upload");
-        int rc = slotLaunchInit(M_ANY, kMixed, dSlots_, dSt_, dOne_, 1, threads_, schedStream_, false);
+        int rc = slotLaunchInit(M_ANY, kMixed, dSlots_, dSt_, dOne_, 1, threads_, schedStream_, false,
+                                initRanks(r->data->nnz, r->K));
         if (rc) return rc;
         if (hipStreamSynchronize(schedStream_) != hipSuccess) return setError(CCDH_E_DEVICE, "restart");
         continue;

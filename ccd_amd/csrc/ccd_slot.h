@@ -94,6 +94,7 @@ struct Slot {
   int32_t* pid;
   int32_t* pidFirst;
   uint8_t* reset;
+  int32_t* pidShard;  // Cox: first pid of each row shard (kShards + 1 entries, last = N)
   Records<R> rec;
   int64_t recStride;  // records per buffer; the second buffer follows the first
   FitCtl ctl;
@@ -118,7 +119,7 @@ struct GangEntry {
 // Byte offsets of every array of one slot inside its block (256-byte aligned).
 struct SlotLayout {
   size_t row, kw, heldout, grp, numer, numer2, xjy, beta, bound, pvar, ptype, fixed,
-      pid, pidFirst, reset, recRow, recKey, recFlags, recO, recXo, recDen;
+      pid, pidFirst, reset, pidShard, recRow, recKey, recFlags, recO, recXo, recDen;
   int64_t recStride;
   size_t bytes;
 };
@@ -143,9 +144,14 @@ int64_t maxColumnLength(const ccdh_data* d);
 // Kernel launches over `n` slots listed in `active` (device array of slot
 // indices into `slots`).  model is a CCDH model code or M_ANY.
 // full = false: a restart from the starting coefficients that keeps the
-// weight-dependent terms (nW, XjY) already in the slot.
+// weight-dependent terms (nW, XjY) already in the slot.  ranks: workgroups per
+// slot for the set-up (each takes a range of row shards); the starting
+// objective follows in one workgroup per slot.
 int slotLaunchInit(int model, bool mixed, const void* slots, FitStatus* st,
-                   const int32_t* active, int n, int threads, hipStream_t s, bool full = true);
+                   const int32_t* active, int n, int threads, hipStream_t s, bool full = true,
+                   int ranks = 1);
+// Set-up workgroups for a fit of nnz nonzeros over K rows (a power of two <= kShards).
+int initRanks(int64_t nnz, int K);
 // One sweep launch over `n` workgroups described by `map` (device array).
 // quantumTicks > 0: a slot stops at the first coordinate boundary after that
 // many device clock ticks (100 MHz) and resumes there in the next launch.
@@ -167,13 +173,16 @@ bool layoutGangs(const std::vector<GangEntry>& gangs, int capacity, int nX,
                  std::vector<GangEntry>& map);
 // Gang size for one of n slots sharing `capacity` workgroups: the largest
 // power of two <= min(kShards, maxGang) with n * G <= capacity and at least
-// minNnzPerRank of the slot's nonzeros per rank; Cox slots run on one workgroup.
+// minNnzPerRank of the slot's work per rank (slotWork).
 int gangSize(int n, int capacity, int64_t nnz, int model, int maxGang, int64_t minNnzPerRank);
 // Gang sizes for slots sharing the device: the uniform size above, then spare
 // CUs (one rank per CU) to the slots of highest priority, doubling one at a time.
 void assignGangs(std::vector<GangEntry>& want, const std::vector<double>& priority,
                  const std::vector<int64_t>& nnz, const std::vector<int>& model, int capacity,
                  int maxGang, int64_t minNnzPerRank);
+// Work of one sweep of a fit in nonzero-equivalents: its nonzeros, plus for Cox
+// an N-pid risk-set scan per coordinate.
+int64_t slotWork(const ccdh_data* d, int N);
 int slotLaunchFinish(int model, bool mixed, const void* slots, FitStatus* st,
                      const int32_t* active, int n, int threads, hipStream_t s);
 

@@ -58,13 +58,16 @@ def test_batch_fixtures_match_oracle(name):
 
 
 @pytest.mark.parametrize("name", sorted(SYNTH))
-def test_batch_same_alone_and_in_the_mix(name):
+@pytest.mark.parametrize("gang", [1, 4])
+def test_batch_same_alone_and_in_the_mix(name, gang):
+    """Same bits alone and among other fits at the same gang size (a gang-size
+    change reorders the per-coordinate sums; the cap keeps it fixed here)."""
     p, prior, var, maxit = SYNTH[name]()
     specs, _ = cv_specs(p, prior, var, maxit, n_fold=3, grid=(0.1, 1.0, 10.0))
     d = api.Data(p)
-    b_mix, i_mix = api.fit_batch(d, specs)
+    b_mix, i_mix = api.fit_batch(d, specs, max_gang=gang, gang_min_nnz=1)
     for i in (0, 4, len(specs) - 1):
-        b_one, i_one = api.fit_batch(d, [specs[i]])
+        b_one, i_one = api.fit_batch(d, [specs[i]], max_gang=gang, gang_min_nnz=1)
         assert np.array_equal(b_one[0], b_mix[i])
         assert i_one[0]["iterations"] == i_mix[i]["iterations"]
         assert i_one[0]["log_likelihood"] == i_mix[i]["log_likelihood"]
@@ -97,7 +100,7 @@ def test_batch_mixed_precision(name):
     check_fit(bg[0], ig[0], bc, ic, rtol=2e-4, atol=2e-5, ll_rtol=1e-6, same_path=False)
 
 
-@pytest.mark.parametrize("name", ["lr", "poisson", "clr"])
+@pytest.mark.parametrize("name", ["lr", "poisson", "clr", "cox"])
 @pytest.mark.parametrize("gang", [2, 8, 64])
 def test_batch_gangs_match_oracle(name, gang):
     """Fits run by gangs of workgroups (row shards, one exchange per coordinate)
@@ -111,6 +114,51 @@ def test_batch_gangs_match_oracle(name, gang):
         check_fit(bg[i], ig[i], bc, ic)
         np.testing.assert_allclose(ig[i]["pred_log_likelihood"], ic["pred_log_likelihood"],
                                    rtol=1e-8)
+
+
+def _stratified_cox(n_strata=5):
+    """The synthetic Cox design split into contiguous row blocks as strata (rows
+    stay sorted by stratum, -time, y), so risk sets restart inside rank ranges."""
+    import dataclasses
+    p, _, _, _ = SYNTH["cox"]()
+    strata = np.minimum(np.arange(p.K) * n_strata // p.K, n_strata - 1).astype(np.int32)
+    return dataclasses.replace(p, strata=strata)
+
+
+@pytest.mark.parametrize("gang", [4, 32])
+def test_batch_cox_gangs_strata_and_weights(gang):
+    """Cox gangs: the segmented risk-set scan carried across ranks restarts at
+    strata inside and at the edges of rank ranges; zero-weight rows (pid N)
+    under CV weights; the log-likelihood convergence criterion."""
+    p = _stratified_cox()
+    _, prior, var, maxit = SYNTH["cox"]()
+    specs = []
+    for fold in range(2):
+        w = orc.cv_weights(np.zeros(p.K, np.int32), 4, 23, fold, by_pid=False)
+        specs.append(default_spec(p, prior=prior, variance=var, weights=w, max_iterations=maxit))
+    specs.append(default_spec(p, prior=prior, variance=var, max_iterations=maxit,
+                              convergence_type=1))  # log-likelihood criterion: gang LL scan
+    d = api.Data(p)
+    bg, ig = api.fit_batch(d, specs, max_gang=gang, gang_min_nnz=1)
+    st = api.batch_last_stats()
+    assert max(t[2] for t in st["timeline"]) == gang
+    for i, sp in enumerate(specs):
+        bc, ic = orc.fit(p, sp)
+        check_fit(bg[i], ig[i], bc, ic)
+
+
+@pytest.mark.parametrize("name", [n for n in fixtures.names() if n.startswith("cox")])
+def test_batch_cox_gangs_tiny_tables(name):
+    """Gangs of 8 over 7-row tables: most ranks own no rows and no pids."""
+    p, extra = fixtures.load(name)
+    specs = [default_spec(p), default_spec(p, prior=PRIOR_LAPLACE, variance=1.0)]
+    if "weights" in extra:
+        for s in specs:
+            s.weights = extra["weights"]
+    bg, ig = api.fit_batch(api.Data(p), specs, max_gang=8, gang_min_nnz=1)
+    for i, sp in enumerate(specs):
+        bc, ic = orc.fit(p, sp)
+        check_fit(bg[i], ig[i], bc, ic)
 
 
 def test_batch_gang_size_changes_only_rounding():

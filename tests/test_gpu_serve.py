@@ -42,14 +42,17 @@ def test_serve_mixed_requests_match_oracle():
 
 
 def test_serve_same_alone_and_in_the_mix():
+    """Same bits alone and in the mix; gangs are capped at 4 workgroups, which
+    every fit large enough for a gang gets both ways (a gang-size change only
+    reorders the per-coordinate sums)."""
     reqs = _requests()
-    with api.Server(quantum_ms=0.05) as srv:  # fits pause and resume many times
+    with api.Server(quantum_ms=0.05, max_gang=4) as srv:  # fits pause and resume many times
         ids = [srv.submit(p, sp) for p, sp in reqs]
         mix = [srv.wait(i) for i in ids]
     assert max(m[2]["launches"] for m in mix) > 3
     for k in (0, 3, 4, 7):
         p, sp = reqs[k]
-        with api.Server(quantum_ms=50.0) as srv:
+        with api.Server(quantum_ms=50.0, max_gang=4) as srv:
             b, r, _ = srv.wait(srv.submit(p, sp))
         assert np.array_equal(b, mix[k][0])
         assert r["iterations"] == mix[k][1]["iterations"]
@@ -119,12 +122,20 @@ def test_serve_mixed_precision():
         check_fit(b, r, bc, ic, rtol=2e-4, atol=2e-5, ll_rtol=1e-6, same_path=False)
 
 
-def test_serve_gangs_match_oracle():
-    """A request large enough for a gang (about 240k nonzeros: 4 workgroups)
-    matches the oracle when served alone with a short quantum."""
-    p, prior, var, maxit = SYNTH["poisson"]()
+@pytest.mark.parametrize("name", ["poisson", "cox"])
+def test_serve_gangs_match_oracle(name):
+    """A request large enough for a gang (Poisson: about 240k nonzeros, 4
+    workgroups; Cox: a 6,000-pid risk-set scan per coordinate, up to 32) matches
+    the oracle when served with a short quantum, alone and beside others."""
+    p, prior, var, maxit = SYNTH[name]()
     sp = default_spec(p, prior=prior, variance=var, max_iterations=maxit)
+    q, qprior, qvar, qmaxit = SYNTH["lr"]()
+    qs = default_spec(q, prior=qprior, variance=qvar, max_iterations=qmaxit)
     with api.Server(quantum_ms=1.0) as srv:
         b, r, info = srv.wait(srv.submit(p, sp))
+        ids = [srv.submit(p, sp), srv.submit(q, qs), srv.submit(p, sp)]
+        more = [srv.wait(i) for i in ids]
     bc, ic = orc.fit(p, sp)
     check_fit(b, r, bc, ic)
+    for bm, rm, _ in (more[0], more[2]):
+        check_fit(bm, rm, bc, ic)
