@@ -93,7 +93,7 @@ class BatchOptions(ctypes.Structure):
     _fields_ = [("threads", ctypes.c_int), ("sweeps_per_sync", ctypes.c_int),
                 ("use_graph", ctypes.c_int), ("max_gang", ctypes.c_int),
                 ("gang_min_nnz", ctypes.c_int64), ("pass_items", ctypes.c_int),
-                ("autotune", ctypes.c_int)]
+                ("autotune", ctypes.c_int), ("quantum_ms", ctypes.c_double)]
 
 
 class ServerOptions(ctypes.Structure):
@@ -270,7 +270,7 @@ def fit_plain(data: Data, spec: FitSpec, heldout=None):
 
 
 def fit_batch(data: Data, specs, heldouts=None, threads=0, sweeps_per_sync=0, use_graph=1,
-              max_gang=0, gang_min_nnz=0, pass_items=0, autotune=0):
+              max_gang=0, gang_min_nnz=0, pass_items=0, autotune=0, quantum_ms=0.0):
     """Many fits over one resident data set with device-resident sweeps; a fit
     runs on up to `max_gang` workgroups when fewer fits than CUs remain."""
     lib = load_library()
@@ -282,7 +282,7 @@ def fit_batch(data: Data, specs, heldouts=None, threads=0, sweeps_per_sync=0, us
     betas = np.zeros((n, data.problem.J))
     res = (FitResult * n)()
     opt = BatchOptions(threads, sweeps_per_sync, use_graph, max_gang, gang_min_nnz, pass_items,
-                       autotune)
+                       autotune, quantum_ms)
     _check(lib.ccdh_batch_fit(data.handle, n, arr, _p(betas), res, ctypes.byref(opt)))
     return betas, [r.as_dict() for r in res]
 

@@ -1464,7 +1464,7 @@ int gangSize(int n, int capacity, int64_t nnz, int model, int maxGang, int64_t m
 
 void assignGangs(std::vector<GangEntry>& want, const std::vector<double>& priority,
                  const std::vector<int64_t>& nnz, const std::vector<int>& model, int capacity,
-                 int maxGang, int64_t minNnzPerRank) {
+                 int maxGang, int64_t minNnzPerRank, bool tiers) {
   const int n = (int)want.size();
   if (n == 0) return;
   const int nX = xcdCount();
@@ -1480,6 +1480,30 @@ void assignGangs(std::vector<GangEntry>& want, const std::vector<double>& priori
   for (int i = 0; i < n; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return priority[a] > priority[b]; });
+  if (tiers) {
+    // fits of equal priority form a tier; the first tier doubles together as
+    // far as it can, then the next (fits expected to need the most sweeps get
+    // their gangs from the start, and a tier's fits keep equal gangs)
+    for (size_t t0 = 0; t0 < order.size();) {
+      size_t t1 = t0 + 1;
+      while (t1 < order.size() && priority[order[t1]] == priority[order[t0]]) ++t1;
+      for (;;) {
+        int need = 0;
+        bool ok = nX > 0;
+        for (size_t k = t0; k < t1 && ok; ++k) {
+          const int i = order[k];
+          const int G = want[i].G;
+          ok = G * 2 <= cap && nnz[i] / (G * 2) >= minNnzPerRank;
+          need += G;
+        }
+        if (!ok || used + need > budget) break;
+        for (size_t k = t0; k < t1; ++k) want[order[k]].G *= 2;
+        used += need;
+      }
+      t0 = t1;
+    }
+    return;
+  }
   bool grew = true;
   while (grew && nX > 0) {
     grew = false;
@@ -1825,13 +1849,21 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
   const int model = d->model;
   int threads = (opt && opt->threads) ? opt->threads : 256;
   threads = std::max(64, std::min(1024, threads / 64 * 64));
-  const int sweepsPerSync = (opt && opt->sweeps_per_sync > 0) ? opt->sweeps_per_sync : 4;
   const int maxGang = (opt && opt->max_gang > 0) ? opt->max_gang : kShards;
   const int64_t minNnz = (opt && opt->gang_min_nnz > 0) ? opt->gang_min_nnz : 32768;
   int itl = (opt && opt->pass_items == 4 && clampThreads(threads) == 256) ? 4 : kITLDefault;
   const bool autotune = opt && opt->autotune && model != CCDH_COX;
   // graph replay of a fixed launch when gangs are off
   const bool useGraph = (opt ? opt->use_graph != 0 : true) && maxGang == 1 && !autotune;
+  // Launch length: a fixed number of sweeps without gangs (every fit moves in
+  // step, so the launch sequence replays); with gangs a time quantum, so a fit
+  // on a large gang keeps sweeping instead of waiting for the one-workgroup fits
+  // of the same launch, and gang sizes are re-dealt between launches.
+  const bool timed = maxGang > 1;
+  const int sweepsPerSync = (opt && opt->sweeps_per_sync > 0) ? opt->sweeps_per_sync
+                                                              : (timed ? (1 << 20) : 4);
+  const double quantumMs = (opt && opt->quantum_ms > 0.0) ? opt->quantum_ms : 200.0;
+  const long long quantumTicks = timed ? (long long)(quantumMs * 1e5) : 0;  // 100 MHz clock
 
   Owned o;
   HIP_TRY(hipStreamCreateWithFlags(&o.stream, hipStreamNonBlocking));
@@ -1917,7 +1949,7 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
       }
     if (!useGraph && gangCap > 1)
       assignGangs(gangs, prio, fw, std::vector<int>(gangs.size(), model), capacity, gangCap,
-                  minNnz);
+                  minNnz, true);
     int G = 1;
     for (const GangEntry& g : gangs) G = std::max(G, g.G);
     if (G > 1 && !layoutGangs(gangs, capacity, xcdCount(), map)) G = 1;
@@ -1935,7 +1967,7 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
   // sweep launches, optionally replayed from a captured graph
   auto enqueueSweeps = [&]() -> int {
     int r = slotLaunchSweeps(mixed, dslots, dst, dmap, mapN, dxch, mapGangs, threads, itl,
-                             sweepsPerSync, 0, s);
+                             sweepsPerSync, quantumTicks, s);
     if (r) return r;
     HIP_TRY(hipMemcpyAsync(o.hst, dst, F * sizeof(FitStatus), hipMemcpyDeviceToHost, s));
     return 0;

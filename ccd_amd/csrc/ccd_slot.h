@@ -176,10 +176,11 @@ bool layoutGangs(const std::vector<GangEntry>& gangs, int capacity, int nX,
 // minNnzPerRank of the slot's work per rank (slotWork).
 int gangSize(int n, int capacity, int64_t nnz, int model, int maxGang, int64_t minNnzPerRank);
 // Gang sizes for slots sharing the device: the uniform size above, then spare
-// CUs (one rank per CU) to the slots of highest priority, doubling one at a time.
+// CUs (one rank per CU) to the slots of highest priority, doubling one at a
+// time; tiers = true: slots of equal priority double together, tier by tier.
 void assignGangs(std::vector<GangEntry>& want, const std::vector<double>& priority,
                  const std::vector<int64_t>& nnz, const std::vector<int>& model, int capacity,
-                 int maxGang, int64_t minNnzPerRank);
+                 int maxGang, int64_t minNnzPerRank, bool tiers = false);
 // Work of one sweep of a fit in nonzero-equivalents: its nonzeros, plus for Cox
 // an N-pid risk-set scan per coordinate.
 int64_t slotWork(const ccdh_data* d, int N);

@@ -172,15 +172,18 @@ int ccdh_get_xbeta(ccdh_fit_t fit, double* out);
 int ccdh_fit_plain(ccdh_data_t data, const ccdh_fit_spec* spec, double* beta_out,
                    ccdh_fit_result* result);
 
-/* Batched engine options. tile_fits / threads = 0 picks tuned defaults. */
+/* Batched engine options; zero fields pick the defaults. */
 typedef struct {
   int threads;           /* workgroup size: 256 (<= 256) or 512 */
-  int sweeps_per_sync;   /* sweeps launched between host convergence checks */
+  int sweeps_per_sync;   /* sweeps per launch: with max_gang 1 (0: 4); with gangs a cap on top of
+                            the time quantum (0: none) */
   int use_graph;         /* replay the sweep launch from a hipGraph (only with max_gang 1) */
   int max_gang;          /* workgroups one fit may use (1: one each; 0: 64) */
   int64_t gang_min_nnz;  /* nonzeros per workgroup below which a fit's gang stops growing (0: 32768) */
   int pass_items;        /* long-column pass width, nonzeros per lane: 8 (0) or 4 (256 lanes) */
   int autotune;          /* time pass width x gangs on this workload first; the choice is cached */
+  double quantum_ms;     /* with gangs: a launch ends at the first coordinate boundary after
+                            this long, and gang sizes are re-dealt between launches (0: 200) */
 } ccdh_batch_options;
 
 /* Batched engine: n_fits fits over one data handle, device-resident sweeps.
