@@ -173,6 +173,7 @@ def main():
                     help="tune pass width and gangs on the workload first (inside the timed step)")
     ap.add_argument("--pass-items", type=int, default=0, help="long-column pass width without tuning")
     ap.add_argument("--max-gang", type=int, default=0, help="workgroups per fit at most (0: 64)")
+    ap.add_argument("--bundle", type=int, default=0, help="fits per slot in lock step (0: default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -226,7 +227,8 @@ def main():
         hip.hipFree(dst)
         log(f"copy calibration: {nbytes} bytes read + {nbytes} bytes written")
     opt = dict(threads=args.threads, sweeps_per_sync=args.sweeps_per_sync, use_graph=1,
-               autotune=args.autotune, pass_items=args.pass_items, max_gang=args.max_gang)
+               autotune=args.autotune, pass_items=args.pass_items, max_gang=args.max_gang,
+               bundle=args.bundle)
 
     # load the code objects and touch the allocator (untimed)
     mini = [default_spec(prob, prior=PRIOR_LAPLACE, variance=1.0, max_iterations=1)]

@@ -77,12 +77,12 @@ class BatchStats(ctypes.Structure):
         ("launch_ms", ctypes.c_float * 128), ("launch_fits", ctypes.c_int32 * 128),
         ("launch_gang", ctypes.c_int32 * 128), ("tuned_pass_items", ctypes.c_int32),
         ("tuned_max_gang", ctypes.c_int32), ("tune_ms", ctypes.c_double),
-        ("tune_rates", ctypes.c_double * 4)]
+        ("tune_rates", ctypes.c_double * 4), ("bundles", ctypes.c_int32)]
 
     def as_dict(self):
         out = {n: getattr(self, n) for n, _ in self._fields_[:15]}
         out.update(tuned_pass_items=self.tuned_pass_items, tuned_max_gang=self.tuned_max_gang,
-                   tune_ms=self.tune_ms, tune_rates=list(self.tune_rates))
+                   tune_ms=self.tune_ms, tune_rates=list(self.tune_rates), bundles=self.bundles)
         k = min(128, self.launches)
         out["timeline"] = [(round(self.launch_ms[i], 2), self.launch_fits[i], self.launch_gang[i])
                            for i in range(k)]
@@ -93,7 +93,8 @@ class BatchOptions(ctypes.Structure):
     _fields_ = [("threads", ctypes.c_int), ("sweeps_per_sync", ctypes.c_int),
                 ("use_graph", ctypes.c_int), ("max_gang", ctypes.c_int),
                 ("gang_min_nnz", ctypes.c_int64), ("pass_items", ctypes.c_int),
-                ("autotune", ctypes.c_int), ("quantum_ms", ctypes.c_double)]
+                ("autotune", ctypes.c_int), ("quantum_ms", ctypes.c_double),
+                ("bundle", ctypes.c_int)]
 
 
 class ServerOptions(ctypes.Structure):
@@ -270,7 +271,7 @@ def fit_plain(data: Data, spec: FitSpec, heldout=None):
 
 
 def fit_batch(data: Data, specs, heldouts=None, threads=0, sweeps_per_sync=0, use_graph=1,
-              max_gang=0, gang_min_nnz=0, pass_items=0, autotune=0, quantum_ms=0.0):
+              max_gang=0, gang_min_nnz=0, pass_items=0, autotune=0, quantum_ms=0.0, bundle=0):
     """Many fits over one resident data set with device-resident sweeps; a fit
     runs on up to `max_gang` workgroups when fewer fits than CUs remain."""
     lib = load_library()
@@ -282,7 +283,7 @@ def fit_batch(data: Data, specs, heldouts=None, threads=0, sweeps_per_sync=0, us
     betas = np.zeros((n, data.problem.J))
     res = (FitResult * n)()
     opt = BatchOptions(threads, sweeps_per_sync, use_graph, max_gang, gang_min_nnz, pass_items,
-                       autotune, quantum_ms)
+                       autotune, quantum_ms, bundle)
     _check(lib.ccdh_batch_fit(data.handle, n, arr, _p(betas), res, ctypes.byref(opt)))
     return betas, [r.as_dict() for r in res]
 

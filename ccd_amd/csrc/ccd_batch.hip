@@ -168,10 +168,14 @@ struct PassLds {
   double v1[NMAX];
   double v2[NMAX];
   unsigned char start[NMAX];
-  double c1, c2, n1, n2;
-  int cValid, nValid;
+  double c1[kMaxBundle], c2[kMaxBundle], n1[kMaxBundle], n2[kMaxBundle];
+  int cValid[kMaxBundle], nValid[kMaxBundle];
 };
 
+// One lane's items of a pass.  In a slot of F fits (a bundle) lane u of a pass
+// carries nonzero c0 + (u >> log2 F) for fit u & (F-1): the F fits of one
+// nonzero sit in adjacent lanes and read the F adjacent entries of the
+// interleaved row and group state (entry r*F + f), one line for all of them.
 template <int IT, typename R>
 struct Lanes {
   int row[IT];
@@ -182,15 +186,19 @@ struct Lanes {
   double den[IT], wn[IT], w[IT];
 };
 
+// Loads one pass of nonzeros [c0, ...) of a column part ending at e (column
+// start b) for the lanes whose fit runs (on).
 template <int MODEL, int IT, typename R>
 __device__ __forceinline__ void passLoad(const Slot<R>& s, int fmt, int64_t b, int64_t e,
-                                         int64_t c0, int useW, Lanes<IT, R>& L) {
+                                         int64_t c0, int useW, bool on, Lanes<IT, R>& L) {
   const DevX& d = s.d;
+  const int F = s.F, lF = s.lF;
+  const int f = threadIdx.x & (F - 1);
   int kp[IT], kn[IT];
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
-    const int64_t p = c0 + (int64_t)it * blockDim.x + threadIdx.x;
-    const bool valid = p < e;
+    const int64_t p = c0 + (((int64_t)it * blockDim.x + threadIdx.x) >> lF);
+    const bool valid = on && p < e;
     L.row[it] = valid ? colRow(d, fmt, p, b) : -1;
     L.x[it] = valid ? colValue<R>(d, fmt, p) : (R)0;
     if (MODEL == M_CLR) {
@@ -205,11 +213,13 @@ __device__ __forceinline__ void passLoad(const Slot<R>& s, int fmt, int64_t b, i
     const bool valid = r >= 0;
     L.own[it] = MODEL == M_CLR ? (valid && L.key[it] != kn[it]) : valid;
     L.start[it] = MODEL == M_CLR ? (valid && L.key[it] != kp[it]) : valid;
-    const RowSt<R> rs = valid ? s.row[r] : RowSt<R>{(R)0, (R)0};
+    const size_t rf = (size_t)r * F + f;
+    const RowSt<R> rs = valid ? s.row[rf] : RowSt<R>{(R)0, (R)0};
     L.o[it] = rs.e;
     L.xo[it] = rs.x;
-    L.w[it] = (MODEL != M_CLR && useW && valid) ? s.kw[r] : 1.0;
-    const GrpSt gs = (MODEL == M_CLR && L.own[it]) ? s.grp[L.key[it]] : GrpSt{1.0, 0.0};
+    L.w[it] = (MODEL != M_CLR && useW && valid) ? s.kw[rf] : 1.0;
+    const GrpSt gs = (MODEL == M_CLR && L.own[it]) ? s.grp[(size_t)L.key[it] * F + f]
+                                                   : GrpSt{1.0, 0.0};
     L.den[it] = gs.den;
     L.wn[it] = gs.nw;
   }
@@ -217,10 +227,12 @@ __device__ __forceinline__ void passLoad(const Slot<R>& s, int fmt, int64_t b, i
 
 // Run totals of NV per-nonzero values for the owners of one pass (see above).
 // `staged` is uniform: false when every run of the column has one nonzero.
+// nLocal counts the pass's (nonzero, fit) pairs; a fit's pairs are F apart, and
+// each fit keeps its own carry for a run that continues into the next pass.
 template <int IT, int NV, class LDS>
 __device__ __forceinline__ void runSums(LDS& S, int nLocal, const bool (&own)[IT],
                                         const bool (&start)[IT], double (&val)[IT][NV],
-                                        bool staged) {
+                                        bool staged, int F, bool on) {
   if (!staged) return;  // every owner's run is its own nonzero
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
@@ -235,42 +247,44 @@ __device__ __forceinline__ void runSums(LDS& S, int nLocal, const bool (&own)[IT
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const int li = it * blockDim.x + threadIdx.x;
-    const bool last = li == nLocal - 1;
-    if (li >= nLocal || !(own[it] || last)) continue;
+    const int f = li & (F - 1);
+    const bool last = li >= nLocal - F;  // the pass's last nonzero
+    if (!on || li >= nLocal || !(own[it] || last)) continue;
     if (own[it] && start[it]) {
-      if (last) S.nValid = 0;
+      if (last) S.nValid[f] = 0;
       continue;
     }
     int k = li;
-    while (k > 0 && !S.start[k]) --k;
+    while (k >= F && !S.start[k]) k -= F;
     const bool before = !S.start[k];  // run began in an earlier pass
-    double s1 = (before && S.cValid) ? S.c1 : 0.0;
-    double s2 = (before && S.cValid) ? S.c2 : 0.0;
-    for (int q = k; q <= li; ++q) {
+    double s1 = (before && S.cValid[f]) ? S.c1[f] : 0.0;
+    double s2 = (before && S.cValid[f]) ? S.c2[f] : 0.0;
+    for (int q = k; q <= li; q += F) {
       s1 += S.v1[q];
       if constexpr (NV > 1) s2 += S.v2[q];
     }
     if (own[it]) {
       val[it][0] = s1;
       if constexpr (NV > 1) val[it][1] = s2;
-      if (last) S.nValid = 0;
+      if (last) S.nValid[f] = 0;
     } else {  // last nonzero of the pass, run continues
-      S.n1 = s1;
-      S.n2 = s2;
-      S.nValid = 1;
+      S.n1[f] = s1;
+      S.n2[f] = s2;
+      S.nValid[f] = 1;
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    S.c1 = S.n1;
-    S.c2 = S.n2;
-    S.cValid = S.nValid;
+  if ((int)threadIdx.x < F) {
+    S.c1[threadIdx.x] = S.n1[threadIdx.x];
+    S.c2[threadIdx.x] = S.n2[threadIdx.x];
+    S.cValid[threadIdx.x] = S.nValid[threadIdx.x];
   }
 }
 
 template <int MODEL, int IT, typename R, class LDS>
 __device__ __forceinline__ void passGather(int fmt, LDS& S, int nLocal, bool staged,
-                                           const Lanes<IT, R>& L, int useW, double (&v)[2]) {
+                                           const Lanes<IT, R>& L, int useW, int F, bool on,
+                                           double (&v)[2]) {
   const bool ind = isIndicator(fmt);
   if (MODEL == M_CLR) {
     double val[IT][2];
@@ -280,7 +294,7 @@ __device__ __forceinline__ void passGather(int fmt, LDS& S, int nLocal, bool sta
       val[it][0] = L.row[it] >= 0 ? (double)m : 0.0;
       val[it][1] = (L.row[it] >= 0 && !ind) ? (double)(m * L.x[it]) : 0.0;
     }
-    runSums<IT, 2>(S, nLocal, L.own, L.start, val, staged);
+    runSums<IT, 2>(S, nLocal, L.own, L.start, val, staged, F, on);
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       if (!L.own[it]) continue;
@@ -309,29 +323,33 @@ __device__ __forceinline__ void passGather(int fmt, LDS& S, int nLocal, bool sta
   }
 }
 
+// Update of one pass by this lane's fit's step `delta` (0: the fit stays).
 template <int MODEL, int IT, typename R, class LDS>
 __device__ __forceinline__ void passUpdate(const Slot<R>& s, LDS& S, int nLocal, bool staged,
-                                           const Lanes<IT, R>& L, double delta) {
+                                           const Lanes<IT, R>& L, double delta, bool on) {
+  const int F = s.F;
+  const int f = threadIdx.x & (F - 1);
   const R rd = (R)delta;
   double val[IT][1];
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     val[it][0] = 0.0;
     const int r = L.row[it];
-    if (r < 0) continue;
+    if (r < 0 || delta == 0.0) continue;
     const R nx = L.xo[it] + rd * L.x[it];
     const R no = rexp(nx);
-    s.row[r] = RowSt<R>{nx, no};
+    s.row[(size_t)r * F + f] = RowSt<R>{nx, no};
     val[it][0] = (double)no - (double)L.o[it];
   }
   if (MODEL == M_CLR) {
-    runSums<IT, 1>(S, nLocal, L.own, L.start, val, staged);
+    runSums<IT, 1>(S, nLocal, L.own, L.start, val, staged, F, on && delta != 0.0);
 #pragma unroll
     for (int it = 0; it < IT; ++it)
-      if (L.own[it]) s.grp[L.key[it]].den = L.den[it] + val[it][0];
+      if (L.own[it] && delta != 0.0) s.grp[(size_t)L.key[it] * F + f].den = L.den[it] + val[it][0];
   }
 }
 
+// Records of one pass at pair offset `off` of the column part.
 template <int MODEL, int IT, typename R>
 __device__ __forceinline__ void storeRecords(const Records<R>& rec, int64_t off,
                                              const Lanes<IT, R>& L) {
@@ -352,12 +370,13 @@ __device__ __forceinline__ void storeRecords(const Records<R>& rec, int64_t off,
 
 template <int MODEL, int IT, typename R>
 __device__ __forceinline__ void loadRecords(const DevX& d, int fmt, int64_t e, int64_t c0,
-                                            int64_t off, const Records<R>& rec, Lanes<IT, R>& L) {
+                                            int64_t off, const Records<R>& rec, bool on, int lF,
+                                            Lanes<IT, R>& L) {
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
-    const int64_t p = c0 + (int64_t)it * blockDim.x + threadIdx.x;
+    const int64_t p = c0 + (((int64_t)it * blockDim.x + threadIdx.x) >> lF);
     const int64_t q = off + (int64_t)it * blockDim.x + threadIdx.x;
-    const bool valid = p < e;
+    const bool valid = on && p < e;
     L.row[it] = valid ? rec.row[q] : -1;
     L.x[it] = valid ? colValue<R>(d, fmt, p) : (R)0;
     L.o[it] = valid ? rec.o[q] : (R)0;
@@ -373,6 +392,38 @@ __device__ __forceinline__ void loadRecords(const DevX& d, int fmt, int64_t e, i
       L.start[it] = valid;
     }
   }
+}
+
+// Per-fit totals of NV values when lane u carries fit u & (F-1): every lane gets
+// the totals of its own fit.  F = 1 sums in blockSum's order.
+template <int NV>
+__device__ __forceinline__ void blockSumFits(double (&v)[NV], int F) {
+  __shared__ double sh[NV][kMaxBlockThreads / kWave][kMaxBundle];
+  __shared__ double tot[NV][kMaxBundle];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  const int nw = (blockDim.x + kWave - 1) / kWave;
+#pragma unroll
+  for (int i = 0; i < NV; ++i)
+    for (int off = kWave / 2; off >= F; off >>= 1) v[i] += __shfl_down(v[i], off, kWave);
+  if (lane < F) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) sh[i][wid][lane] = v[i];
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < F) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double t = 0.0;
+      for (int w = 0; w < nw; ++w) t += sh[i][w][threadIdx.x];
+      tot[i][threadIdx.x] = t;
+    }
+  }
+  __syncthreads();
+  const int f = threadIdx.x & (F - 1);
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = tot[i][f];
+  __syncthreads();
 }
 
 // ------------------------------------------------------------ gangs
@@ -483,22 +534,84 @@ __device__ __forceinline__ bool gangExchange(Gang& g, double (&sum)[NS], double 
   return ok;
 }
 
-// One coordinate cycle of a fit (independent rows or grouped) over this rank's
-// rows, from coordinate j0.  Every lane takes the (deterministic) prior step
-// from the reduced gradient and Hessian, so no broadcast barrier is needed; a
-// barrier follows an update only.  Returns the coordinate to resume at when the
-// quantum ran out (J when the cycle completed), or -1 when a gang exchange
-// failed.
+// gangExchange with run-time widths through LDS arrays: on entry xs[0..ns) and
+// xl[0..nl) hold this rank's values (written by any threads of the block); on
+// return xs holds the sums over the gang in rank order and xl the lead rank's
+// values, for every thread.
+__device__ __forceinline__ bool gangExchangeN(Gang& g, double* xs, int ns, double* xl, int nl) {
+  __shared__ int sOk[2];
+  const int par = (int)(g.seq & 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double* mine = &g.x->v[par][g.rank][0];
+    double chk = 0.0;
+    for (int i = 0; i < ns; ++i) chk += publishShared(mine + i, xs[i]);
+    if (g.rank == 0)
+      for (int i = 0; i < nl; ++i) chk += publishShared(mine + kXchSum + i, xl[i]);
+    asm volatile("; swap results %0" : : "v"(chk));
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __hip_atomic_fetch_add(&g.x->cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned target = (unsigned)g.G * (g.seq + 1u);
+    long long spins = 0;
+    int ok = 1;
+    while (__hip_atomic_load(&g.x->cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > kSpinLimit) {
+        ok = 0;
+        break;
+      }
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    sOk[par] = ok;
+  }
+  __syncthreads();
+  const bool ok = sOk[par] != 0;
+  if (threadIdx.x < kWave) {
+    // all loads in flight before the first sum
+    const bool mine = ok && (int)threadIdx.x < g.G;
+    double t[kXchSum];
+#pragma unroll
+    for (int i = 0; i < kXchSum; ++i)
+      t[i] = (mine && i < ns) ? loadShared(&g.x->v[par][threadIdx.x][i]) : 0.0;
+    const double l = (ok && (int)threadIdx.x < nl) ? loadShared(&g.x->v[par][0][kXchSum + threadIdx.x]) : 0.0;
+#pragma unroll
+    for (int i = 0; i < kXchSum; ++i) {
+      if (i >= ns) break;
+      const double r = waveSum(t[i]);
+      if (threadIdx.x == 0) xs[i] = r;
+    }
+    if ((int)threadIdx.x < nl) xl[threadIdx.x] = l;
+  }
+  __syncthreads();
+  g.seq++;
+  g.ok = g.ok && ok;
+  return ok;
+}
+
+// One coordinate cycle of a slot's fits (independent rows or grouped) over this
+// rank's rows, from coordinate j0; `act` has bit f set for each fit still
+// running (the others' lanes load and store nothing).  Every lane takes its
+// fit's (deterministic) prior step from the reduced gradient and Hessian, so no
+// broadcast barrier is needed; a barrier follows an update only.  Returns the
+// coordinate to resume at when the quantum ran out (J when the cycle
+// completed), or -1 when a gang exchange failed.
 template <int MODEL, typename R, int ITL, class LDS>
-__device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus& cs, LDS& S, int j0,
-                                        long long deadline, Gang& gg) {
+__device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S, int j0,
+                                        long long deadline, Gang& gg, unsigned act) {
   const DevX& d = s.d;
-  const int64_t CH = (int64_t)blockDim.x * kIT;
-  const int64_t CHL = (int64_t)blockDim.x * ITL;
+  const int F = s.F, lF = s.lF;
+  const int f = threadIdx.x & (F - 1);
+  const bool on = (act >> f) & 1u;
+  const int nAct = __popc(act);
+  const int64_t CH = ((int64_t)blockDim.x * kIT) >> lF;    // nonzeros of a single pass
+  const int64_t CHL = ((int64_t)blockDim.x * ITL) >> lF;  // nonzeros of a long-column pass
   const int J = s.J;
   const int useW = s.ctl.useW;
   const bool multi = gg.G > 1;
   const bool lead = gg.rank == 0;
+  __shared__ double xs[kXchSum], xl[kXchLead];
   long long cG = 0, cGR = 0, cU = 0, cUR = 0, tS = 0, tL = 0, nL = 0;
   int jEnd = J;
   // Records are indexed by position in the column, and a rank's positions move
@@ -533,69 +646,86 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus& cs, LDS& S,
     const int64_t nj = e - b;      // this rank's nonzeros
     const int64_t njAll = ce - cb;  // the column's
     const int fmt = d.fmt[j];
-    // in a gang, only the lead rank's reads count (it publishes them)
-    double bj = s.beta[j];
-    double bnd = s.bound[j];
-    const double pv = s.pvar[j], xy = s.xjy[j];
-    const int pt = s.ptype[j];
+    // this lane's fit; in a gang only the lead rank's reads count (it publishes them)
+    const size_t jf = (size_t)j * F + f;
+    double bj = s.beta[jf];
+    double bnd = s.bound[jf];
+    const double pv = s.pvar[jf], xy = s.xjy[jf];
+    const int pt = s.ptype[jf];
     const long long runs = MODEL == M_CLR ? (long long)(d.segPtr[j + 1] - d.segPtr[j]) : njAll;
     const bool staged = MODEL == M_CLR && runs < njAll;
-    if (threadIdx.x == 0) S.cValid = 0;
+    if ((int)threadIdx.x < F) S.cValid[threadIdx.x] = 0;
     const long long t0 = wall_clock64();
     double v[2] = {0.0, 0.0};
     bool stop = false;
     // the exchange also carries the lead rank's end-of-quantum decision
     auto reduce = [&]() -> bool {
-      blockSum<2>(v);
+      blockSumFits<2>(v, F);
       if (!multi) return true;
-      double ld[3] = {bj, bnd,
-                      (deadline > 0 && wall_clock64() >= deadline) ? 1.0 : 0.0};
-      if (!gangExchange<2, 3>(gg, v, ld)) return false;
-      bj = ld[0];
-      bnd = ld[1];
-      stop = ld[2] != 0.0;
+      if ((int)threadIdx.x < F) {  // lane f holds fit f's totals and coefficient state
+        xs[2 * threadIdx.x] = v[0];
+        xs[2 * threadIdx.x + 1] = v[1];
+        xl[2 * threadIdx.x] = bj;
+        xl[2 * threadIdx.x + 1] = bnd;
+      }
+      if (threadIdx.x == 0) xl[2 * F] = (deadline > 0 && wall_clock64() >= deadline) ? 1.0 : 0.0;
+      if (!gangExchangeN(gg, xs, 2 * F, xl, 2 * F + 1)) return false;
+      v[0] = xs[2 * f];
+      v[1] = xs[2 * f + 1];
+      bj = xl[2 * f];
+      bnd = xl[2 * f + 1];
+      stop = xl[2 * F] != 0.0;
       return true;
     };
     auto step = [&]() -> double {
+      if (!on) return 0.0;
       double g = njAll > 0 ? v[0] - xy : 0.0, h = njAll > 0 ? v[1] : 0.0;
       if (h < 0.0) {
         g = 0.0;
         h = 0.0;
       }
       const double dl = applyBounds(priorDelta(pt, pv, g, h, bj), &bnd);
-      if (threadIdx.x == 0 && lead) {
-        s.bound[j] = bnd;
-        if (dl != 0.0) s.beta[j] = bj + dl;
+      if ((int)threadIdx.x < F && lead) {  // thread f writes fit f's
+        s.bound[jf] = bnd;
+        if (dl != 0.0) s.beta[jf] = bj + dl;
       }
       return dl;
     };
+    // fits that take a step (all lanes of a fit agree)
+    auto moving = [&](double dl) -> int {
+      if (F == 1) return dl != 0.0 ? 1 : 0;
+      return __syncthreads_count(dl != 0.0) / ((int)blockDim.x / F);
+    };
     double delta;
+    int mv;
     if (nj <= CH) {
       Lanes<kIT, R> L;
-      passLoad<MODEL, kIT, R>(s, fmt, cb, e, b, useW, L);
-      passGather<MODEL, kIT, R>(fmt, S, (int)nj, staged, L, useW, v);
+      passLoad<MODEL, kIT, R>(s, fmt, cb, e, b, useW, on, L);
+      passGather<MODEL, kIT, R>(fmt, S, (int)(nj << lF), staged, L, useW, F, on, v);
       if (!reduce()) return -1;
       delta = step();
-      if (delta != 0.0 && nj > 0) {
-        passUpdate<MODEL, kIT, R>(s, S, (int)nj, staged, L, delta);
+      mv = moving(delta);
+      if (mv > 0 && nj > 0) {
+        passUpdate<MODEL, kIT, R>(s, S, (int)(nj << lF), staged, L, delta, on);
         __syncthreads();
       }
     } else {
       for (int64_t c0 = b; c0 < e; c0 += CHL) {
         Lanes<ITL, R> L;
-        passLoad<MODEL, ITL, R>(s, fmt, cb, e, c0, useW, L);
-        storeRecords<MODEL, ITL, R>(rec, c0 - cb, L);
-        passGather<MODEL, ITL, R>(fmt, S, (int)min(CHL, e - c0), staged, L, useW, v);
+        passLoad<MODEL, ITL, R>(s, fmt, cb, e, c0, useW, on, L);
+        storeRecords<MODEL, ITL, R>(rec, (c0 - cb) << lF, L);
+        passGather<MODEL, ITL, R>(fmt, S, (int)(min(CHL, e - c0) << lF), staged, L, useW, F, on, v);
       }
       if (!reduce()) return -1;
       delta = step();
-      if (threadIdx.x == 0) S.cValid = 0;
-      if (delta != 0.0) {
+      if ((int)threadIdx.x < F) S.cValid[threadIdx.x] = 0;
+      mv = moving(delta);
+      if (mv > 0) {
         __syncthreads();  // carry reset visible; records were written by these lanes
         for (int64_t c0 = b; c0 < e; c0 += CHL) {
           Lanes<ITL, R> L;
-          loadRecords<MODEL, ITL, R>(d, fmt, e, c0, c0 - cb, rec, L);
-          passUpdate<MODEL, ITL, R>(s, S, (int)min(CHL, e - c0), staged, L, delta);
+          loadRecords<MODEL, ITL, R>(d, fmt, e, c0, (c0 - cb) << lF, rec, on, lF, L);
+          passUpdate<MODEL, ITL, R>(s, S, (int)(min(CHL, e - c0) << lF), staged, L, delta, on);
         }
         __syncthreads();
       }
@@ -607,18 +737,17 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus& cs, LDS& S,
     } else {
       tS += t1 - t0;
     }
-    cG += nj;
-    cGR += MODEL == M_CLR && lead ? runs : 0;
-    if (delta != 0.0) {
-      cU += nj;
-      cUR += MODEL == M_CLR && lead ? runs : 0;
-    }
+    cG += nj * nAct;
+    cGR += MODEL == M_CLR && lead ? runs * nAct : 0;
+    cU += nj * mv;
+    cUR += MODEL == M_CLR && lead ? runs * mv : 0;
     if (stop) {
       jEnd = j + 1;
       break;
     }
   }
   if (threadIdx.x == 0) {
+    FitStatus& cs = st[0];
     atomicAdd((unsigned long long*)&cs.tSmall, (unsigned long long)tS);
     atomicAdd((unsigned long long*)&cs.tLarge, (unsigned long long)tL);
     atomicAdd((unsigned long long*)&cs.nLarge, (unsigned long long)nL);
@@ -904,7 +1033,9 @@ __device__ __forceinline__ int cycleCox(const Slot<R>& s, FitStatus& cs, int j0,
 }
 
 // ------------------------------------------------------------ whole-fit pieces
-// Rows [rowLo, rowHi) and groups [grpLo, grpHi) a workgroup sums over.
+// Rows [rowLo, rowHi) and groups [grpLo, grpHi) a workgroup sums over.  In a
+// bundle every loop walks (row, fit) or (group, fit) pairs, so lane u serves fit
+// u & (F-1) and the sums come back per fit (blockSumFits).
 struct Part {
   int rowLo, rowHi, grpLo, grpHi;
 };
@@ -914,67 +1045,75 @@ __device__ __forceinline__ Part wholeFit(const Slot<R>& s) {
   return Part{0, s.K, 0, s.N};
 }
 
-// log likelihood terms of one fit over a part (ModelSpecifics.hpp:605-650);
-// Cox: the accumulated denominators of the part's pids [grpLo, grpHi) come from
-// a segmented scan carried across the gang `gg` (nullptr: one workgroup)
+// log likelihood terms of this lane's fit over a part (ModelSpecifics.hpp:605-650);
+// Cox (one fit per slot): the accumulated denominators of the part's pids
+// [grpLo, grpHi) come from a segmented scan carried across the gang `gg`
+// (nullptr: one workgroup)
 template <int MODEL, typename R>
 __device__ __forceinline__ double fitLogLik(const Slot<R>& s, const Part& pt, Gang* gg = nullptr) {
   const DevX& d = s.d;
   const int useW = s.ctl.useW;
+  const int F = s.F, lF = s.lF;
   double v[2] = {0.0, 0.0};
-  for (int k = pt.rowLo + threadIdx.x; k < pt.rowHi; k += blockDim.x) {
-    const RowSt<R> rs = s.row[k];
+  for (int64_t u = ((int64_t)pt.rowLo << lF) + threadIdx.x; u < ((int64_t)pt.rowHi << lF);
+       u += blockDim.x) {
+    const int k = (int)(u >> lF);
+    const RowSt<R> rs = s.row[u];
     const double x = (double)rs.x;
     const int yi = (int)d.y[k];
-    const double w = useW ? s.kw[k] : 1.0;
+    const double w = useW ? s.kw[u] : 1.0;
     double c = (MODEL == M_COX) ? (yi != 1 ? 0.0 : x) : yi * x;
     v[0] += useW ? c * w : c;
     if (MODEL == M_LOGISTIC) v[1] += w * log(1.0 + (double)rs.e);
     if (MODEL == M_POISSON) v[1] += w * (double)rs.e;
   }
   if (MODEL == M_CLR) {
-    for (int i = pt.grpLo + threadIdx.x; i < pt.grpHi; i += blockDim.x) {
-      const GrpSt gs = s.grp[i];
+    for (int64_t u = ((int64_t)pt.grpLo << lF) + threadIdx.x; u < ((int64_t)pt.grpHi << lF);
+         u += blockDim.x) {
+      const GrpSt gs = s.grp[u];
       if (gs.nw != 0.0) v[1] += gs.nw * log(gs.den);
     }
   }
   if (MODEL == M_COX) v[1] += coxLogDen<R>(s, pt.grpLo, pt.grpHi, gg);
-  blockSum<2>(v);
+  blockSumFits<2>(v, F);
   return v[0] - v[1];
 }
 
 template <typename R>
 __device__ __forceinline__ double fitGradObjective(const Slot<R>& s, const Part& pt) {
   const int useW = s.ctl.useW;
+  const int lF = s.lF;
   double v[1] = {0.0};
-  for (int k = pt.rowLo + threadIdx.x; k < pt.rowHi; k += blockDim.x) {
-    const double x = (double)s.row[k].x;
-    v[0] += useW ? x * s.d.y[k] * s.kw[k] : x * s.d.y[k];
+  for (int64_t u = ((int64_t)pt.rowLo << lF) + threadIdx.x; u < ((int64_t)pt.rowHi << lF);
+       u += blockDim.x) {
+    const int k = (int)(u >> lF);
+    const double x = (double)s.row[u].x;
+    v[0] += useW ? x * s.d.y[k] * s.kw[u] : x * s.d.y[k];
   }
-  blockSum<1>(v);
+  blockSumFits<1>(v, s.F);
   return v[0];
 }
 
-__device__ __forceinline__ double priorPart(const double* beta, const double* pvar,
-                                            const int8_t* ptype, int J) {
+template <typename R>
+__device__ __forceinline__ double priorPart(const Slot<R>& s) {
   double v[1] = {0.0};
-  for (int j = threadIdx.x; j < J; j += blockDim.x)
-    v[0] += priorLogDensity(ptype[j], pvar[j], beta[j]);
-  blockSum<1>(v);
+  for (int64_t u = threadIdx.x; u < ((int64_t)s.J << s.lF); u += blockDim.x)
+    v[0] += priorLogDensity(s.ptype[u], s.pvar[u], s.beta[u]);
+  blockSumFits<1>(v, s.F);
   return v[0];
 }
 
 // Convergence objective (getObjectiveFunction, CyclicCoordinateDescent.cpp:541-558)
-// of the whole fit (single workgroup).
+// of this lane's fit (single workgroup).
 template <int MODEL, typename R>
 __device__ __forceinline__ double fitObjective(const Slot<R>& s) {
   const int ct = s.ctl.convergenceType;
   const Part all = wholeFit(s);
   if (ct == 0) return fitGradObjective<R>(s, all);
   if (ct == 3) return 0.0;
-  const double ll = fitLogLik<MODEL, R>(s, all) + s.fixedLL;
+  const double ll = fitLogLik<MODEL, R>(s, all) + s.fixedLL[threadIdx.x & (s.F - 1)];
   if (ct == 2) return ll;
-  return ll + priorPart(s.beta, s.pvar, s.ptype, s.J);
+  return ll + priorPart<R>(s);
 }
 
 // The same objective summed over a gang; `stop` carries the lead rank's
@@ -983,6 +1122,9 @@ template <int MODEL, typename R>
 __device__ __forceinline__ bool gangObjective(const Slot<R>& s, Gang& gg, long long deadline,
                                               double* obj, bool* stop) {
   const int ct = s.ctl.convergenceType;
+  const int F = s.F;
+  const int f = threadIdx.x & (F - 1);
+  __shared__ double xs[kXchSum], xl[kXchLead];
   Part pt{gg.rowLo, gg.rowHi, 0, 0};
   if (MODEL == M_CLR && gg.rowHi > gg.rowLo) {
     pt.grpLo = s.d.strata[gg.rowLo];
@@ -992,34 +1134,44 @@ __device__ __forceinline__ bool gangObjective(const Slot<R>& s, Gang& gg, long l
     pt.grpLo = s.pidShard[gg.s0];
     pt.grpHi = s.pidShard[gg.s1];
   }
-  double v[1] = {0.0};
-  if (ct == 0) v[0] = fitGradObjective<R>(s, pt);
+  double v = 0.0;
+  if (ct == 0) v = fitGradObjective<R>(s, pt);
   if (ct == 1 || ct == 2) {
-    v[0] = fitLogLik<MODEL, R>(s, pt, &gg);
+    v = fitLogLik<MODEL, R>(s, pt, &gg);
     if (!gg.ok) return false;
   }
   // the coefficients belong to the lead rank: it adds the log prior
-  double ld[2] = {0.0, (deadline > 0 && wall_clock64() >= deadline) ? 1.0 : 0.0};
-  if (ct == 1 && gg.rank == 0) ld[0] = priorPart(s.beta, s.pvar, s.ptype, s.J);
-  if (!gangExchange<1, 2>(gg, v, ld)) return false;
-  *stop = ld[1] != 0.0;
-  if (ct == 0) *obj = v[0];
+  const double lp = (ct == 1 && gg.rank == 0) ? priorPart<R>(s) : 0.0;
+  if ((int)threadIdx.x < F) {
+    xs[threadIdx.x] = v;
+    xl[threadIdx.x] = lp;
+  }
+  if (threadIdx.x == 0) xl[F] = (deadline > 0 && wall_clock64() >= deadline) ? 1.0 : 0.0;
+  if (!gangExchangeN(gg, xs, F, xl, F + 1)) return false;
+  *stop = xl[F] != 0.0;
+  const double fl = s.fixedLL[f];
+  if (ct == 0) *obj = xs[f];
   else if (ct == 3) *obj = 0.0;
-  else if (ct == 2) *obj = v[0] + s.fixedLL;
-  else *obj = v[0] + s.fixedLL + ld[0];
+  else if (ct == 2) *obj = xs[f] + fl;
+  else *obj = xs[f] + fl + xl[f];
   return true;
 }
 
 // Per-fit set-up (findMode's lazy part), one rank of G over the row shards
-// [r*S/G, (r+1)*S/G): event weights of the rank's groups, XjY of every G-th
-// coordinate, xBeta of its rows from the starting coefficients (column order,
-// as computeXBeta), exp and the denominators of its groups, bounds of every G-th
-// coordinate.  Groups and rows of a rank are its own (shards start at strata /
-// tie groups), so ranks need no exchange and the result does not depend on G.
+// [r*S/G, (r+1)*S/G), for the fits of the slot in `mask`: event weights of the
+// rank's groups, XjY of every G-th coordinate, xBeta of its rows from the
+// starting coefficients (column order, as computeXBeta), exp and the
+// denominators of its groups, bounds of every G-th coordinate.  Groups and rows
+// of a rank are its own (shards start at strata / tie groups), so ranks need no
+// exchange and the result does not depend on G.
 template <int MODEL, typename R>
-__device__ __forceinline__ void initPart(const Slot<R>& s, bool full, int rank, int G) {
+__device__ __forceinline__ void initPart(const Slot<R>& s, bool full, int rank, int G,
+                                         unsigned mask) {
   const DevX& d = s.d;
   const int useW = s.ctl.useW;
+  const int F = s.F, lF = s.lF;
+  const int f = threadIdx.x & (F - 1);
+  const bool mine = (mask >> f) & 1u;
   RowSt<R>* rowSt = s.row;
   const double* kw = s.kw;
   const int s0 = rank * d.nShards / G, s1 = (rank + 1) * d.nShards / G;
@@ -1036,10 +1188,12 @@ __device__ __forceinline__ void initPart(const Slot<R>& s, bool full, int rank, 
   }
   // nW and XjY depend on the weights only: a restart keeps them
   if (full && MODEL == M_CLR) {
-    for (int g = gLo + threadIdx.x; g < gHi; g += blockDim.x) {
+    for (int64_t u = ((int64_t)gLo << lF) + threadIdx.x; u < ((int64_t)gHi << lF); u += blockDim.x) {
+      if (!mine) continue;
+      const int g = (int)(u >> lF);
       double acc = 0.0;
-      for (int k = d.stratStart[g]; k < d.stratStart[g + 1]; ++k) acc += d.y[k] * kw[k];
-      s.grp[g].nw = acc;
+      for (int k = d.stratStart[g]; k < d.stratStart[g + 1]; ++k) acc += d.y[k] * kw[(size_t)k * F + f];
+      s.grp[u].nw = acc;
     }
   } else if (full && MODEL == M_COX) {
     for (int p = gLo + threadIdx.x; p < gHi; p += blockDim.x) {
@@ -1053,44 +1207,55 @@ __device__ __forceinline__ void initPart(const Slot<R>& s, bool full, int rank, 
     const int fmt = d.fmt[j];
     const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
     double t[1] = {0.0};
-    for (int64_t p = b + threadIdx.x; p < e; p += blockDim.x) {
+    for (int64_t u = threadIdx.x; u < ((e - b) << lF); u += blockDim.x) {
+      const int64_t p = b + (u >> lF);
       const int k = colRow(d, fmt, p, b);
       const double x = colValue<double>(d, fmt, p);
-      t[0] += useW ? x * d.y[k] * kw[k] : x * d.y[k];
+      t[0] += useW ? x * d.y[k] * kw[(size_t)k * F + f] : x * d.y[k];
     }
-    blockSum<1>(t);
-    if (threadIdx.x == 0) s.xjy[j] = t[0];
+    blockSumFits<1>(t, F);
+    if ((int)threadIdx.x < F && mine) s.xjy[(size_t)j * F + f] = t[0];
   }
   // xBeta = sum_j beta_j X_j in column order, over the rank's part of each column
-  for (int k = rowLo + threadIdx.x; k < rowHi; k += blockDim.x) rowSt[k].x = (R)0;
+  for (int64_t u = ((int64_t)rowLo << lF) + threadIdx.x; u < ((int64_t)rowHi << lF); u += blockDim.x)
+    if (mine) rowSt[u].x = (R)0;
   __syncthreads();
   for (int j = 0; j < s.J; ++j) {
-    const double bj = s.beta[j];
-    if (bj == 0.0) continue;
+    bool any = false;
+    for (int i = 0; i < F; ++i) any |= ((mask >> i) & 1u) && s.beta[(size_t)j * F + i] != 0.0;
+    if (!any) continue;
+    const double bj = s.beta[(size_t)j * F + f];
     const int fmt = d.fmt[j];
     const int64_t cb = d.colPtr[j];
     const int32_t* off = d.colShard + (size_t)j * (d.nShards + 1);
     const int64_t b = cb + off[s0], e = cb + off[s1];
     const R rb = (R)bj;
-    for (int64_t p = b + threadIdx.x; p < e; p += blockDim.x) {
-      const int k = colRow(d, fmt, p, cb);
-      rowSt[k].x += rb * colValue<R>(d, fmt, p);
+    if (mine && bj != 0.0) {
+      for (int64_t u = threadIdx.x; u < ((e - b) << lF); u += blockDim.x) {
+        const int64_t p = b + (u >> lF);
+        const int k = colRow(d, fmt, p, cb);
+        rowSt[(size_t)k * F + f].x += rb * colValue<R>(d, fmt, p);
+      }
     }
     __syncthreads();
   }
   // exp and denominators
   if (MODEL == M_CLR) {
-    for (int g = gLo + threadIdx.x; g < gHi; g += blockDim.x) {
+    for (int64_t u = ((int64_t)gLo << lF) + threadIdx.x; u < ((int64_t)gHi << lF); u += blockDim.x) {
+      if (!mine) continue;
+      const int g = (int)(u >> lF);
       double acc = 0.0;
       for (int k = d.stratStart[g]; k < d.stratStart[g + 1]; ++k) {
-        const R o = rexp(rowSt[k].x);
-        rowSt[k].e = o;
+        const size_t kf = (size_t)k * F + f;
+        const R o = rexp(rowSt[kf].x);
+        rowSt[kf].e = o;
         acc += (double)o;
       }
-      s.grp[g].den = acc;
+      s.grp[u].den = acc;
     }
   } else {
-    for (int k = rowLo + threadIdx.x; k < rowHi; k += blockDim.x) rowSt[k].e = rexp(rowSt[k].x);
+    for (int64_t u = ((int64_t)rowLo << lF) + threadIdx.x; u < ((int64_t)rowHi << lF); u += blockDim.x)
+      if (mine) rowSt[u].e = rexp(rowSt[u].x);
     if (MODEL == M_COX) {
       __syncthreads();
       for (int p = gLo + threadIdx.x; p < gHi; p += blockDim.x) {
@@ -1103,46 +1268,55 @@ __device__ __forceinline__ void initPart(const Slot<R>& s, bool full, int rank, 
       }
     }
   }
-  for (int j = rank * (int)blockDim.x + threadIdx.x; j < s.J; j += G * (int)blockDim.x)
-    s.bound[j] = s.ctl.initialBound;
+  for (int64_t u = (int64_t)rank * blockDim.x + threadIdx.x; u < ((int64_t)s.J << lF);
+       u += (int64_t)G * blockDim.x)
+    if (mine) s.bound[u] = s.initBound[f];
 }
 
-// The starting objective and status of a set-up fit (one workgroup).
+// The starting objective and status of the set-up fits in `mask` (one workgroup).
 template <int MODEL, typename R>
-__device__ __forceinline__ void initObjective(const Slot<R>& s, FitStatus& st) {
+__device__ __forceinline__ void initObjective(const Slot<R>& s, FitStatus* st, unsigned mask) {
   const double obj = fitObjective<MODEL, R>(s);
-  if (threadIdx.x == 0) {
-    st.lastObj = obj;
-    st.iteration = 0;
-    st.jNext = 0;
-    st.done = 0;
-    st.flag = CCDH_SUCCESS;
+  const int f = threadIdx.x;
+  if (f < s.F && ((mask >> f) & 1u)) {
+    st[f].lastObj = obj;
+    st[f].iteration = 0;
+    st[f].done = 0;
+    st[f].flag = CCDH_SUCCESS;
   }
+  if (threadIdx.x == 0) st[0].jNext = 0;
 }
 
-// Up to `sweeps` complete CCD cycles plus the convergence check after each;
-// with a deadline, the slot stops at a coordinate boundary once it has passed
-// and the next launch resumes the sweep there.  Every rank of a gang computes
-// the same objective and convergence decision; the lead rank records them.
+// Up to `sweeps` complete CCD cycles of the slot's running fits plus the
+// convergence check of each after every cycle; with a deadline, the slot stops
+// at a coordinate boundary once it has passed and the next launch resumes the
+// sweep there.  The fits of a slot move in lock step; one that converges stops
+// loading and updating while the others go on.  Every rank of a gang computes
+// the same objectives and decisions; the lead rank records them (thread f, fit f).
 template <int MODEL, typename R, int ITL, class LDS>
-__device__ __forceinline__ void sweepSlot(const Slot<R>& s, FitStatus& st, LDS& S, int sweeps,
+__device__ __forceinline__ void sweepSlot(const Slot<R>& s, FitStatus* st, LDS& S, int sweeps,
                                           long long deadline, Gang& gg) {
   const FitCtl c = s.ctl;
+  const int F = s.F;
+  const int f = threadIdx.x & (F - 1);
   const bool multi = gg.G > 1;
-  const bool rec = threadIdx.x == 0 && gg.rank == 0;
-  int j0 = st.jNext;
-  int iteration = st.iteration;
-  double lastObj = st.lastObj;
+  const bool rec = (int)threadIdx.x < F && gg.rank == 0;
+  __shared__ unsigned sAct;
+  unsigned act = 0;
+  for (int i = 0; i < F; ++i) act |= (st[i].done ? 0u : 1u) << i;
+  int j0 = st[0].jNext;
+  int iteration = st[f].iteration;
+  double lastObj = st[f].lastObj;
   for (int sw = 0; sw < sweeps; ++sw) {
-    const int jEnd = MODEL == M_COX ? cycleCox<R>(s, st, j0, deadline, gg)
-                                    : cycleFit<MODEL, R, ITL>(s, st, S, j0, deadline, gg);
+    const int jEnd = MODEL == M_COX ? cycleCox<R>(s, st[0], j0, deadline, gg)
+                                    : cycleFit<MODEL, R, ITL>(s, st, S, j0, deadline, gg, act);
     __syncthreads();
     if (jEnd < 0) {
-      if (threadIdx.x == 0) st.fault = 1;
+      if (threadIdx.x == 0) st[0].fault = 1;
       return;
     }
     if (jEnd < s.J) {
-      if (rec) st.jNext = jEnd;
+      if (threadIdx.x == 0 && gg.rank == 0) st[0].jNext = jEnd;
       return;
     }
     j0 = 0;
@@ -1152,97 +1326,114 @@ __device__ __forceinline__ void sweepSlot(const Slot<R>& s, FitStatus& st, LDS& 
     bool stop = false;
     if (multi) {
       if (!gangObjective<MODEL, R>(s, gg, deadline, &obj, &stop)) {
-        if (threadIdx.x == 0) st.fault = 1;
+        if (threadIdx.x == 0) st[0].fault = 1;
         return;
       }
     } else {
       obj = fitObjective<MODEL, R>(s);
     }
-    iteration += 1;
-    double conv;
-    bool ill = false;
-    if (obj != obj) {
-      conv = 0.0;
-      ill = true;
-    } else {
-      conv = fabs(obj - lastObj) / (fabs(obj) + 1.0);
-    }
-    lastObj = obj;
+    const bool on = (act >> f) & 1u;
     int done = 0, flag = CCDH_SUCCESS;
-    if (c.tolerance > 0 && conv < c.tolerance) {
-      flag = ill ? CCDH_ILLCONDITIONED : CCDH_SUCCESS;
-      done = 1;
-    } else if (iteration == c.maxIterations) {
-      flag = CCDH_MAX_ITERATIONS;
-      done = 1;
-    }
-    if (rec) {
-      st.tObj += wall_clock64() - tq0;
-      st.objRows += (c.convergenceType == 0 ? s.K : (c.convergenceType == 3 ? 0 : s.K + s.N));
-      st.iteration = iteration;
-      st.lastObj = obj;
-      st.jNext = 0;
-      if (done) {
-        st.flag = flag;
-        st.done = 1;
+    if (on) {
+      iteration += 1;
+      double conv;
+      bool ill = false;
+      if (obj != obj) {
+        conv = 0.0;
+        ill = true;
+      } else {
+        conv = fabs(obj - lastObj) / (fabs(obj) + 1.0);
+      }
+      lastObj = obj;
+      if (c.tolerance > 0 && conv < c.tolerance) {
+        flag = ill ? CCDH_ILLCONDITIONED : CCDH_SUCCESS;
+        done = 1;
+      } else if (iteration == c.maxIterations) {
+        flag = CCDH_MAX_ITERATIONS;
+        done = 1;
       }
     }
-    if (done) return;
+    if (rec && on) {
+      if (f == 0) st[0].tObj += wall_clock64() - tq0;
+      st[f].objRows += (c.convergenceType == 0 ? s.K : (c.convergenceType == 3 ? 0 : s.K + s.N));
+      st[f].iteration = iteration;
+      st[f].lastObj = obj;
+      if (done) {
+        st[f].flag = flag;
+        st[f].done = 1;
+      }
+    }
+    if (threadIdx.x == 0 && gg.rank == 0) st[0].jNext = 0;
+    // fits still running (lane f < F speaks for fit f)
+    if (threadIdx.x == 0) sAct = act;
+    __syncthreads();
+    if ((int)threadIdx.x < F && done) atomicAnd(&sAct, ~(1u << threadIdx.x));
+    __syncthreads();
+    act = sAct;
+    __syncthreads();
+    if (act == 0) return;
     if (multi ? stop : quantumOver(deadline)) return;
   }
 }
 
 // Final log likelihood and, when held-out weights are given, the predictive
-// log likelihood (ModelSpecifics.hpp:652-690; LR / Poisson / CLR forms).
+// log likelihood (ModelSpecifics.hpp:652-690; LR / Poisson / CLR forms) of
+// every fit of the slot.
 template <int MODEL, typename R>
-__device__ __forceinline__ void finishSlot(const Slot<R>& s, FitStatus& st) {
+__device__ __forceinline__ void finishSlot(const Slot<R>& s, FitStatus* st) {
+  const int F = s.F, lF = s.lF;
+  const int f = threadIdx.x & (F - 1);
   const double ll = fitLogLik<MODEL, R>(s, wholeFit(s));
   double v[1] = {0.0};
   if (s.heldout && MODEL != M_COX) {
     const double* w2 = s.heldout;
-    for (int k = threadIdx.x; k < s.K; k += blockDim.x) {
-      const RowSt<R> rs = s.row[k];
+    for (int64_t u = threadIdx.x; u < ((int64_t)s.K << lF); u += blockDim.x) {
+      const int k = (int)(u >> lF);
+      const RowSt<R> rs = s.row[u];
       const double x = (double)rs.x;
       const double y = s.d.y[k];
-      if (MODEL == M_LOGISTIC) v[0] += y * w2[k] * (x - log(1.0 + (double)rs.e));
-      if (MODEL == M_POISSON) v[0] += (y * x - exp(x)) * w2[k];
-      if (MODEL == M_CLR) v[0] += y * w2[k] * (x - log(s.grp[s.d.strata[k]].den));
+      if (MODEL == M_LOGISTIC) v[0] += y * w2[u] * (x - log(1.0 + (double)rs.e));
+      if (MODEL == M_POISSON) v[0] += (y * x - exp(x)) * w2[u];
+      if (MODEL == M_CLR) v[0] += y * w2[u] * (x - log(s.grp[(size_t)s.d.strata[k] * F + f].den));
     }
   }
-  blockSum<1>(v);
-  if (threadIdx.x == 0) {
-    st.logLik = ll + (MODEL == M_POISSON ? s.fixedLL : 0.0);
-    st.predLogLik = v[0];
+  blockSumFits<1>(v, F);
+  if ((int)threadIdx.x < F) {
+    st[f].logLik = ll + (MODEL == M_POISSON ? s.fixedLL[f] : 0.0);
+    st[f].predLogLik = v[0];
   }
 }
 
 // ------------------------------------------------------------------ kernels
-// Each kernel runs one workgroup per listed slot and dispatches on the slot's
-// model (uniform per workgroup).
+// Each kernel runs one workgroup (or a gang, or set-up ranks) per listed slot
+// and dispatches on the slot's model (uniform per workgroup).  A slot's fits
+// have their statuses at st[s.fit0 .. s.fit0 + s.F).
 
-// `ranks` workgroups per listed slot, each setting up its shards.
+// `ranks` workgroups per listed slot, each setting up its shards, for the fits in `mask`.
 template <typename R>
-__global__ void k_slot_init(const Slot<R>* slots, const int32_t* active, int full, int ranks) {
+__global__ void k_slot_init(const Slot<R>* slots, const int32_t* active, int full, int ranks,
+                            unsigned mask) {
   const int f = active[blockIdx.x / ranks];
   const int rank = blockIdx.x % ranks;
   const Slot<R>& s = slots[f];
   switch (s.model) {
-    case M_LOGISTIC: initPart<M_LOGISTIC, R>(s, full != 0, rank, ranks); break;
-    case M_POISSON: initPart<M_POISSON, R>(s, full != 0, rank, ranks); break;
-    case M_CLR: initPart<M_CLR, R>(s, full != 0, rank, ranks); break;
-    case M_COX: initPart<M_COX, R>(s, full != 0, rank, ranks); break;
+    case M_LOGISTIC: initPart<M_LOGISTIC, R>(s, full != 0, rank, ranks, mask); break;
+    case M_POISSON: initPart<M_POISSON, R>(s, full != 0, rank, ranks, mask); break;
+    case M_CLR: initPart<M_CLR, R>(s, full != 0, rank, ranks, mask); break;
+    case M_COX: initPart<M_COX, R>(s, full != 0, rank, ranks, mask); break;
   }
 }
 
 template <typename R>
-__global__ void k_slot_objective(const Slot<R>* slots, FitStatus* st, const int32_t* active) {
-  const int f = active[blockIdx.x];
-  const Slot<R>& s = slots[f];
+__global__ void k_slot_objective(const Slot<R>* slots, FitStatus* st, const int32_t* active,
+                                 unsigned mask) {
+  const Slot<R>& s = slots[active[blockIdx.x]];
+  FitStatus* fs = st + s.fit0;
   switch (s.model) {
-    case M_LOGISTIC: initObjective<M_LOGISTIC, R>(s, st[f]); break;
-    case M_POISSON: initObjective<M_POISSON, R>(s, st[f]); break;
-    case M_CLR: initObjective<M_CLR, R>(s, st[f]); break;
-    case M_COX: initObjective<M_COX, R>(s, st[f]); break;
+    case M_LOGISTIC: initObjective<M_LOGISTIC, R>(s, fs, mask); break;
+    case M_POISSON: initObjective<M_POISSON, R>(s, fs, mask); break;
+    case M_CLR: initObjective<M_CLR, R>(s, fs, mask); break;
+    case M_COX: initObjective<M_COX, R>(s, fs, mask); break;
   }
 }
 
@@ -1254,29 +1445,31 @@ __global__ void __launch_bounds__(BLOCK)
   __shared__ long long sStart;
   const GangEntry ge = map[blockIdx.x];
   if (ge.slot < 0) return;
-  const int f = ge.slot;
-  if (st[f].done) return;
+  const Slot<R>& s = slots[ge.slot];
+  FitStatus* fs = st + s.fit0;
+  bool running = false;
+  for (int i = 0; i < s.F; ++i) running |= fs[i].done == 0;
+  if (!running) return;
   if (threadIdx.x == 0) sStart = wall_clock64();
   __syncthreads();
   const long long deadline = quantum > 0 ? sStart + quantum : 0;
-  const Slot<R>& s = slots[f];
   Gang gg;
   gg.rank = ge.rank;
   gg.G = ge.G;
   gg.seq = 0;
   gg.ok = true;
-  gg.x = ge.G > 1 ? xch + f : nullptr;
+  gg.x = ge.G > 1 ? xch + ge.slot : nullptr;
   if (ge.G > 1) {
     // a gang shares one XCD (one L2): check before touching any state
     const double x = (double)xccId();
     double v[2] = {x, x * x};
     double ld[1] = {0.0};
     if (!gangExchange<2, 1>(gg, v, ld)) {
-      if (threadIdx.x == 0) st[f].fault = 1;
+      if (threadIdx.x == 0) fs[0].fault = 1;
       return;
     }
     if (v[0] * v[0] != v[1] * ge.G) {
-      if (threadIdx.x == 0) st[f].fault = 2;
+      if (threadIdx.x == 0) fs[0].fault = 2;
       return;
     }
   }
@@ -1292,10 +1485,10 @@ __global__ void __launch_bounds__(BLOCK)
     gg.rowHi = s.K;
   }
   switch (s.model) {
-    case M_LOGISTIC: sweepSlot<M_LOGISTIC, R, ITL>(s, st[f], S, sweeps, deadline, gg); break;
-    case M_POISSON: sweepSlot<M_POISSON, R, ITL>(s, st[f], S, sweeps, deadline, gg); break;
-    case M_CLR: sweepSlot<M_CLR, R, ITL>(s, st[f], S, sweeps, deadline, gg); break;
-    case M_COX: sweepSlot<M_COX, R, ITL>(s, st[f], S, sweeps, deadline, gg); break;
+    case M_LOGISTIC: sweepSlot<M_LOGISTIC, R, ITL>(s, fs, S, sweeps, deadline, gg); break;
+    case M_POISSON: sweepSlot<M_POISSON, R, ITL>(s, fs, S, sweeps, deadline, gg); break;
+    case M_CLR: sweepSlot<M_CLR, R, ITL>(s, fs, S, sweeps, deadline, gg); break;
+    case M_COX: sweepSlot<M_COX, R, ITL>(s, fs, S, sweeps, deadline, gg); break;
   }
 }
 
@@ -1307,13 +1500,13 @@ __global__ void k_gang_reset(const GangEntry* map, int n, Xch* xch) {
 
 template <typename R>
 __global__ void k_slot_finish(const Slot<R>* slots, FitStatus* st, const int32_t* active) {
-  const int f = active[blockIdx.x];
-  const Slot<R>& s = slots[f];
+  const Slot<R>& s = slots[active[blockIdx.x]];
+  FitStatus* fs = st + s.fit0;
   switch (s.model) {
-    case M_LOGISTIC: finishSlot<M_LOGISTIC, R>(s, st[f]); break;
-    case M_POISSON: finishSlot<M_POISSON, R>(s, st[f]); break;
-    case M_CLR: finishSlot<M_CLR, R>(s, st[f]); break;
-    case M_COX: finishSlot<M_COX, R>(s, st[f]); break;
+    case M_LOGISTIC: finishSlot<M_LOGISTIC, R>(s, fs); break;
+    case M_POISSON: finishSlot<M_POISSON, R>(s, fs); break;
+    case M_CLR: finishSlot<M_CLR, R>(s, fs); break;
+    case M_COX: finishSlot<M_COX, R>(s, fs); break;
   }
 }
 
@@ -1331,18 +1524,19 @@ int clampThreads(int threads) { return threads <= 256 ? 256 : 512; }
 namespace ccdamd {
 
 int slotLaunchInit(int /*model*/, bool mixed, const void* slots, FitStatus* st,
-                   const int32_t* active, int n, int threads, hipStream_t s, bool full, int ranks) {
+                   const int32_t* active, int n, int threads, hipStream_t s, bool full, int ranks,
+                   unsigned mask) {
   if (n <= 0) return 0;
   threads = clampThreads(threads);
   ranks = std::max(1, std::min(ranks, kShards));
   if (mixed) {
     k_slot_init<float><<<n * ranks, threads, 0, s>>>((const Slot<float>*)slots, active,
-                                                     full ? 1 : 0, ranks);
-    k_slot_objective<float><<<n, threads, 0, s>>>((const Slot<float>*)slots, st, active);
+                                                     full ? 1 : 0, ranks, mask);
+    k_slot_objective<float><<<n, threads, 0, s>>>((const Slot<float>*)slots, st, active, mask);
   } else {
     k_slot_init<double><<<n * ranks, threads, 0, s>>>((const Slot<double>*)slots, active,
-                                                      full ? 1 : 0, ranks);
-    k_slot_objective<double><<<n, threads, 0, s>>>((const Slot<double>*)slots, st, active);
+                                                      full ? 1 : 0, ranks, mask);
+    k_slot_objective<double><<<n, threads, 0, s>>>((const Slot<double>*)slots, st, active, mask);
   }
   HIP_TRY(hipGetLastError());
   return 0;
@@ -1519,7 +1713,10 @@ void assignGangs(std::vector<GangEntry>& want, const std::vector<double>& priori
 }
 
 int64_t slotWork(const ccdh_data* d, int N) {
-  return d->model == CCDH_COX ? d->nnz + (int64_t)d->J * N : d->nnz;
+  // a Cox rank also scans its pids twice per coordinate: count them as
+  // nonzeros, so a gang grows with the risk-set length as with the column
+  // length (a short scan split over many ranks costs more in exchanges than it saves)
+  return d->model == CCDH_COX ? d->nnz + 2 * (int64_t)N : d->nnz;
 }
 
 int slotGangCapacity(bool mixed, int threads, int itl) {
@@ -1549,7 +1746,8 @@ constexpr size_t kNone = ~(size_t)0;
 size_t roundUp(size_t n, size_t m) { return (n + m - 1) / m * m; }
 }  // namespace
 
-SlotLayout slotLayout(int model, int K, int J, int nCap, int64_t maxLen, bool heldout, size_t rs) {
+SlotLayout slotLayout(int model, int K, int J, int nCap, int64_t maxLen, bool heldout, size_t rs,
+                      int F) {
   SlotLayout L;
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -1561,31 +1759,34 @@ SlotLayout slotLayout(int model, int K, int J, int nCap, int64_t maxLen, bool he
   const bool cox = model == CCDH_COX;
   const size_t nG = grouped ? (size_t)nCap + 1 : 1;
   const size_t nR = cox ? 1 : (size_t)std::max<int64_t>(maxLen, 1);
-  L.row = take((size_t)K * 2 * rs);
-  L.kw = take((size_t)K * 8);
-  L.heldout = heldout && !cox ? take((size_t)K * 8) : kNone;
-  L.grp = grouped ? take(nG * sizeof(GrpSt)) : kNone;
+  const size_t Fz = (size_t)std::max(1, F);
+  L.row = take((size_t)K * 2 * rs * Fz);
+  L.kw = take((size_t)K * 8 * Fz);
+  L.heldout = heldout && !cox ? take((size_t)K * 8 * Fz) : kNone;
+  L.grp = grouped ? take(nG * sizeof(GrpSt) * Fz) : kNone;
   L.numer = cox ? take(nG * 8) : kNone;
   L.numer2 = cox ? take(nG * 8) : kNone;
   L.pid = cox ? take((size_t)K * 4) : kNone;
   L.pidFirst = cox ? take(nG * 4) : kNone;
   L.reset = cox ? take(nG) : kNone;
   L.pidShard = cox ? take((kShards + 1) * 4) : kNone;
-  L.xjy = take((size_t)J * 8);
-  L.beta = take((size_t)J * 8);
-  L.bound = take((size_t)J * 8);
-  L.pvar = take((size_t)J * 8);
-  L.ptype = take((size_t)J);
+  L.xjy = take((size_t)J * 8 * Fz);
+  L.beta = take((size_t)J * 8 * Fz);
+  L.bound = take((size_t)J * 8 * Fz);
+  L.pvar = take((size_t)J * 8 * Fz);
+  L.ptype = take((size_t)J * Fz);
   L.fixed = take((size_t)J);
-  // two record buffers: a gang alternates them by coordinate (see cycleFit)
-  L.recStride = (int64_t)nR;
-  L.recRow = cox ? kNone : take(2 * nR * 4);
-  L.recO = cox ? kNone : take(2 * nR * rs);
-  L.recXo = cox ? kNone : take(2 * nR * rs);
+  // two record buffers of one record per (nonzero, fit): a gang alternates
+  // them by coordinate (see cycleFit)
+  L.recStride = (int64_t)(nR * Fz);
+  const size_t nRF = nR * Fz;
+  L.recRow = cox ? kNone : take(2 * nRF * 4);
+  L.recO = cox ? kNone : take(2 * nRF * rs);
+  L.recXo = cox ? kNone : take(2 * nRF * rs);
   const bool clr = model == CCDH_CONDITIONAL_LOGISTIC;
-  L.recKey = clr ? take(2 * nR * 4) : kNone;
-  L.recFlags = clr ? take(2 * nR) : kNone;
-  L.recDen = clr ? take(2 * nR * 8) : kNone;
+  L.recKey = clr ? take(2 * nRF * 4) : kNone;
+  L.recFlags = clr ? take(2 * nRF) : kNone;
+  L.recDen = clr ? take(2 * nRF * 8) : kNone;
   L.bytes = off;
   return L;
 }
@@ -1612,15 +1813,21 @@ static T* at(char* block, size_t off) {
 }
 
 template <typename R>
-int slotPrepare(const ccdh_data* d, const ccdh_fit_spec& sp, char* block, const SlotLayout& L,
-                Slot<R>* out, FitStatus* st, hipStream_t s) {
+int slotPrepare(const ccdh_data* d, const ccdh_fit_spec* sps, int F, char* block,
+                const SlotLayout& L, Slot<R>* out, FitStatus* st, hipStream_t s) {
   const int K = d->K, J = d->J;
+  if (F < 1 || F > kMaxBundle || (F & (F - 1)) != 0 || (F > 1 && d->model == CCDH_COX))
+    return setError(CCDH_E_INVALID, "a slot holds 1, 2, 4 or 8 fits (Cox: 1)");
+  const ccdh_fit_spec& sp = sps[0];
   Slot<R> x;
   std::memset(&x, 0, sizeof(x));
   x.d = d->view();
   x.model = d->model;
   x.K = K;
   x.J = J;
+  x.F = F;
+  x.lF = F == 8 ? 3 : (F == 4 ? 2 : (F == 2 ? 1 : 0));
+  x.fit0 = 0;
   x.row = at<RowSt<R>>(block, L.row);
   x.kw = at<double>(block, L.kw);
   x.heldout = sp.heldout ? at<double>(block, L.heldout) : nullptr;
@@ -1677,38 +1884,66 @@ int slotPrepare(const ccdh_data* d, const ccdh_fit_spec& sp, char* block, const 
     HIP_TRY(hipStreamSynchronize(s));
   }
   x.N = N;
-  // fixed term of the Poisson log likelihood: sum of w * -log(y!)
-  x.fixedLL = 0.0;
-  if (d->model == CCDH_POISSON) {
+  // per fit: fixed term of the Poisson log likelihood (sum of w * -log(y!)) and starting bound
+  for (int f = 0; f < F; ++f) {
+    x.initBound[f] = sps[f].initial_bound;
     double t = 0.0;
-    for (int k = 0; k < K; ++k) t += sp.weights ? d->hLogFact[k] * sp.weights[k] : d->hLogFact[k];
-    x.fixedLL = t;
+    if (d->model == CCDH_POISSON)
+      for (int k = 0; k < K; ++k)
+        t += sps[f].weights ? d->hLogFact[k] * sps[f].weights[k] : d->hLogFact[k];
+    x.fixedLL[f] = t;
   }
-  std::vector<int8_t> ht(J);
-  for (int j = 0; j < J; ++j) ht[j] = (int8_t)sp.prior_type[j];
-  HIP_TRY(hipMemcpyAsync(x.beta, sp.beta0, J * sizeof(double), hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(x.pvar, sp.prior_var, J * sizeof(double), hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(x.ptype, ht.data(), J, hipMemcpyHostToDevice, s));
+  // interleaved uploads (entry i of fit f at i*F + f)
+  std::vector<double> hb((size_t)J * F), hv((size_t)J * F);
+  std::vector<int8_t> ht((size_t)J * F);
+  for (int f = 0; f < F; ++f)
+    for (int j = 0; j < J; ++j) {
+      hb[(size_t)j * F + f] = sps[f].beta0[j];
+      hv[(size_t)j * F + f] = sps[f].prior_var[j];
+      ht[(size_t)j * F + f] = (int8_t)sps[f].prior_type[j];
+    }
+  HIP_TRY(hipMemcpyAsync(x.beta, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(x.pvar, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(x.ptype, ht.data(), ht.size(), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(x.fixed, sp.fixed, J, hipMemcpyHostToDevice, s));
+  std::vector<double> hw;
   if (sp.weights) {
-    HIP_TRY(hipMemcpyAsync(x.kw, sp.weights, K * sizeof(double), hipMemcpyHostToDevice, s));
+    if (F == 1) {
+      HIP_TRY(hipMemcpyAsync(x.kw, sp.weights, K * sizeof(double), hipMemcpyHostToDevice, s));
+    } else {
+      hw.resize((size_t)K * F);
+      for (int f = 0; f < F; ++f)
+        for (int k = 0; k < K; ++k) hw[(size_t)k * F + f] = sps[f].weights[k];
+      HIP_TRY(hipMemcpyAsync(x.kw, hw.data(), hw.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    }
   } else {
-    k_fill_d<<<std::min(1024, (K + 255) / 256), 256, 0, s>>>(x.kw, K, 1.0);
+    const int64_t n = (int64_t)K * F;
+    k_fill_d<<<(int)std::min<int64_t>(1024, (n + 255) / 256), 256, 0, s>>>(x.kw, n, 1.0);
     HIP_TRY(hipGetLastError());
   }
-  if (x.heldout)
-    HIP_TRY(hipMemcpyAsync(x.heldout, sp.heldout, K * sizeof(double), hipMemcpyHostToDevice, s));
-  HIP_TRY(hipStreamSynchronize(s));  // host staging (ht) goes out of scope
+  std::vector<double> hh;
+  if (x.heldout) {
+    if (F == 1) {
+      HIP_TRY(hipMemcpyAsync(x.heldout, sp.heldout, K * sizeof(double), hipMemcpyHostToDevice, s));
+    } else {
+      hh.resize((size_t)K * F);
+      for (int f = 0; f < F; ++f)
+        for (int k = 0; k < K; ++k) hh[(size_t)k * F + f] = sps[f].heldout[k];
+      HIP_TRY(hipMemcpyAsync(x.heldout, hh.data(), hh.size() * sizeof(double),
+                             hipMemcpyHostToDevice, s));
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(s));  // host staging goes out of scope
   *out = x;
-  std::memset(st, 0, sizeof(FitStatus));
-  st->N = N;
+  std::memset(st, 0, (size_t)F * sizeof(FitStatus));
+  for (int f = 0; f < F; ++f) st[f].N = N;
   return 0;
 }
 
-template int slotPrepare<double>(const ccdh_data*, const ccdh_fit_spec&, char*, const SlotLayout&,
-                                 Slot<double>*, FitStatus*, hipStream_t);
-template int slotPrepare<float>(const ccdh_data*, const ccdh_fit_spec&, char*, const SlotLayout&,
-                                Slot<float>*, FitStatus*, hipStream_t);
+template int slotPrepare<double>(const ccdh_data*, const ccdh_fit_spec*, int, char*,
+                                 const SlotLayout&, Slot<double>*, FitStatus*, hipStream_t);
+template int slotPrepare<float>(const ccdh_data*, const ccdh_fit_spec*, int, char*,
+                                const SlotLayout&, Slot<float>*, FitStatus*, hipStream_t);
 
 }  // namespace ccdamd
 
@@ -1752,11 +1987,11 @@ int devAllocOwned(T** p, size_t n, std::vector<void*>& owned) {
 // coordinates completed per millisecond of kernel time.  The fits are then put
 // back to their starting state.  The choice is cached per workload signature.
 struct TuneKey {
-  int model, K, J, F, precision, threads, maxGang;
+  int model, K, J, F, bundle, precision, threads, maxGang;
   int64_t nnz;
   bool operator<(const TuneKey& o) const {
-    return std::tie(model, K, J, F, precision, threads, maxGang, nnz) <
-           std::tie(o.model, o.K, o.J, o.F, o.precision, o.threads, o.maxGang, o.nnz);
+    return std::tie(model, K, J, F, bundle, precision, threads, maxGang, nnz) <
+           std::tie(o.model, o.K, o.J, o.F, o.bundle, o.precision, o.threads, o.maxGang, o.nnz);
   }
 };
 std::mutex g_tuneMu;
@@ -1764,34 +1999,36 @@ std::map<TuneKey, std::pair<int, int>> g_tuned;
 
 constexpr long long kTrialTicks = 5000000;  // 50 ms of device time per candidate
 
+// One quantum-long launch from the starting point; rate = fit-coordinates per ms.
 int timeTrial(bool mixed, const void* dslots, FitStatus* dst, const GangEntry* dmap, int mapN,
-              Xch* dxch, bool gangs, int threads, int itl, int F, int J, Owned& o, hipStream_t s,
-              double* rate) {
+              Xch* dxch, bool gangs, int threads, int itl, int nFit, int J,
+              const std::vector<int>& fit0, const std::vector<int>& fitsOf, Owned& o,
+              hipStream_t s, double* rate) {
   HIP_TRY(hipEventRecord(o.ev0, s));
   int rc = slotLaunchSweeps(mixed, dslots, dst, dmap, mapN, dxch, gangs, threads, itl, 1 << 30,
                             kTrialTicks, s);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(o.ev1, s));
-  HIP_TRY(hipMemcpyAsync(o.hst, dst, F * sizeof(FitStatus), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(o.hst, dst, nFit * sizeof(FitStatus), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, o.ev0, o.ev1));
   double coords = 0.0;
-  for (int f = 0; f < F; ++f) {
-    if (o.hst[f].fault) return setError(CCDH_E_DEVICE, "gang exchange failed during tuning");
-    coords += (double)o.hst[f].iteration * J + o.hst[f].jNext;
+  for (size_t b = 0; b < fit0.size(); ++b) {
+    const FitStatus& h = o.hst[fit0[b]];
+    if (h.fault) return setError(CCDH_E_DEVICE, "gang exchange failed during tuning");
+    coords += ((double)h.iteration * J + h.jNext) * fitsOf[b];
   }
   *rate = ms > 0.f ? coords / ms : 0.0;
   return 0;
 }
 
-template <typename R, class Trial>
-int tuneBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, int threads, int maxGang,
-              std::vector<Slot<R>>& slots, std::vector<FitStatus>& st, Slot<R>* dslots,
-              FitStatus* dst, int32_t* dactive, hipStream_t s, Owned& o, Trial trial, int* itlOut,
+// Pass width x gang policy timed on this workload (one quantum per candidate,
+// back to the starting point after each); the choice is cached per workload.
+template <class Trial, class Reset>
+int tuneBatch(const TuneKey& key, int threads, int maxGang, Trial trial, Reset reset, int* itlOut,
               int* gangOut) {
   const auto t0 = std::chrono::steady_clock::now();
-  const TuneKey key{d->model, d->K, d->J, F, d->precision, clampThreads(threads), maxGang, d->nnz};
   {
     std::lock_guard<std::mutex> g(g_tuneMu);
     auto it = g_tuned.find(key);
@@ -1801,8 +2038,6 @@ int tuneBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, int threads, int 
       return 0;
     }
   }
-  if (!o.ev0) HIP_TRY(hipEventCreate(&o.ev0));
-  if (!o.ev1) HIP_TRY(hipEventCreate(&o.ev1));
   std::vector<std::pair<int, int>> cands = {{8, maxGang}, {8, 1}};
   if (clampThreads(threads) == 256) cands.push_back({4, maxGang});
   if (maxGang == 1) cands.erase(cands.begin() + 1);
@@ -1817,18 +2052,8 @@ int tuneBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, int threads, int 
       best = rate;
       pick = cands[c];
     }
-    // back to the starting point: coefficients, statuses, then the set-up kernel
-    for (int f = 0; f < F; ++f)
-      HIP_TRY(hipMemcpyAsync(slots[f].beta, specs[f].beta0, d->J * sizeof(double),
-                             hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(dst, st.data(), F * sizeof(FitStatus), hipMemcpyHostToDevice, s));
-    std::vector<int32_t> all(F);
-    for (int f = 0; f < F; ++f) all[f] = f;
-    HIP_TRY(hipMemcpyAsync(dactive, all.data(), F * sizeof(int32_t), hipMemcpyHostToDevice, s));
-    rc = slotLaunchInit(d->model, std::is_same<R, float>::value, dslots, dst, dactive, F, threads, s,
-                        false, initRanks(d->nnz, d->K));
+    rc = reset();
     if (rc) return rc;
-    HIP_TRY(hipStreamSynchronize(s));
   }
   *itlOut = pick.first;
   *gangOut = pick.second;
@@ -1840,8 +2065,16 @@ int tuneBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, int threads, int 
   return 0;
 }
 
+// Fits that may share a slot: same fixed coordinates and the same controls.
+bool bundleable(const ccdh_fit_spec& a, const ccdh_fit_spec& b, int J) {
+  return a.tolerance == b.tolerance && a.max_iterations == b.max_iterations &&
+         a.convergence_type == b.convergence_type && a.initial_bound == b.initial_bound &&
+         a.max_bound_count == b.max_bound_count && (a.weights == nullptr) == (b.weights == nullptr) &&
+         (a.heldout == nullptr) == (b.heldout == nullptr) && std::memcmp(a.fixed, b.fixed, J) == 0;
+}
+
 template <typename R>
-int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
+int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOut,
              ccdh_fit_result* results, const ccdh_batch_options* opt) {
   const auto t0 = std::chrono::steady_clock::now();
   const int J = d->J;
@@ -1864,19 +2097,44 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
                                                               : (timed ? (1 << 20) : 4);
   const double quantumMs = (opt && opt->quantum_ms > 0.0) ? opt->quantum_ms : 200.0;
   const long long quantumTicks = timed ? (long long)(quantumMs * 1e5) : 0;  // 100 MHz clock
+  int bundleMax = (opt && opt->bundle > 0) ? opt->bundle : 1;
+  if (model == CCDH_COX) bundleMax = 1;
+  if (bundleMax != 1 && bundleMax != 2 && bundleMax != 4 && bundleMax != 8)
+    return setError(CCDH_E_INVALID, "bundle must be 1, 2, 4 or 8");
+
+  // slots: runs of consecutive fits that may share one, split into powers of two
+  std::vector<int> fit0, fitsOf;
+  for (int a = 0; a < nFit;) {
+    int e = a + 1;
+    while (e < nFit && e - a < bundleMax && bundleable(specs[a], specs[e], J)) ++e;
+    for (int left = e - a; left > 0;) {
+      int b = 1;
+      while (b * 2 <= left) b *= 2;
+      fit0.push_back(a);
+      fitsOf.push_back(b);
+      a += b;
+      left -= b;
+    }
+  }
+  const int nSlot = (int)fit0.size();
+  int Fmax = 1;
+  for (int b : fitsOf) Fmax = std::max(Fmax, b);
+  std::vector<int> slotOf(nFit);
+  for (int b = 0; b < nSlot; ++b)
+    for (int i = 0; i < fitsOf[b]; ++i) slotOf[fit0[b] + i] = b;
 
   Owned o;
   HIP_TRY(hipStreamCreateWithFlags(&o.stream, hipStreamNonBlocking));
   hipStream_t s = o.stream;
 
-  // one block per fit, all with the layout of the largest group count
+  // one block per slot, all with the layout of the largest group count and bundle
   int maxN = 1;
   bool anyHeld = false;
-  for (int f = 0; f < F; ++f) {
+  for (int f = 0; f < nFit; ++f) {
     maxN = std::max(maxN, slotGroups(d, specs[f].weights));
     anyHeld |= specs[f].heldout != nullptr;
   }
-  const SlotLayout L = slotLayout(model, d->K, J, maxN, maxColumnLength(d), anyHeld, sizeof(R));
+  const SlotLayout L = slotLayout(model, d->K, J, maxN, maxColumnLength(d), anyHeld, sizeof(R), Fmax);
   const int capacity = std::min(slotGangCapacity(mixed, threads, 8), slotGangCapacity(mixed, threads, 4));
   if (capacity <= 0) return setError(CCDH_E_DEVICE, "sweep kernel occupancy query failed");
   (void)xcdCount();
@@ -1886,42 +2144,44 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
   int32_t* dactive = nullptr;
   GangEntry* dmap = nullptr;
   Xch* dxch = nullptr;
-  const int mapCap = std::max(F, capacity);
+  const int mapCap = std::max(nSlot, capacity);
   int rc = 0;
-  rc = rc ? rc : devAllocOwned(&arena, (size_t)F * L.bytes, o.ptrs);
-  rc = rc ? rc : devAllocOwned(&dslots, F, o.ptrs);
-  rc = rc ? rc : devAllocOwned(&dst, F, o.ptrs);
-  rc = rc ? rc : devAllocOwned(&dactive, F, o.ptrs);
+  rc = rc ? rc : devAllocOwned(&arena, (size_t)nSlot * L.bytes, o.ptrs);
+  rc = rc ? rc : devAllocOwned(&dslots, nSlot, o.ptrs);
+  rc = rc ? rc : devAllocOwned(&dst, nFit, o.ptrs);
+  rc = rc ? rc : devAllocOwned(&dactive, nSlot, o.ptrs);
   rc = rc ? rc : devAllocOwned(&dmap, mapCap, o.ptrs);
-  rc = rc ? rc : devAllocOwned(&dxch, F, o.ptrs);
+  rc = rc ? rc : devAllocOwned(&dxch, nSlot, o.ptrs);
   if (rc) return rc;
-  HIP_TRY(hipHostMalloc((void**)&o.hst, F * sizeof(FitStatus), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void**)&o.hst, nFit * sizeof(FitStatus), hipHostMallocDefault));
 
-  std::vector<Slot<R>> slots(F);
-  std::vector<FitStatus> st(F);
-  std::vector<int32_t> active(F);
-  std::vector<int> boundCount(F, 1);
-  for (int f = 0; f < F; ++f) {
-    rc = slotPrepare<R>(d, specs[f], arena + (size_t)f * L.bytes, L, &slots[f], &st[f], s);
+  std::vector<Slot<R>> slots(nSlot);
+  std::vector<FitStatus> st(nFit);
+  std::vector<int32_t> active(nSlot);
+  std::vector<int> boundCount(nFit, 1);
+  for (int b = 0; b < nSlot; ++b) {
+    rc = slotPrepare<R>(d, specs + fit0[b], fitsOf[b], arena + (size_t)b * L.bytes, L, &slots[b],
+                        &st[fit0[b]], s);
     if (rc) return rc;
-    active[f] = f;
+    slots[b].fit0 = fit0[b];
+    active[b] = b;
   }
-  HIP_TRY(hipMemcpyAsync(dslots, slots.data(), F * sizeof(Slot<R>), hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(dst, st.data(), F * sizeof(FitStatus), hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(dactive, active.data(), F * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dslots, slots.data(), nSlot * sizeof(Slot<R>), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dst, st.data(), nFit * sizeof(FitStatus), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dactive, active.data(), nSlot * sizeof(int32_t), hipMemcpyHostToDevice, s));
   const int setupRanks = initRanks(d->nnz, d->K);
-  rc = slotLaunchInit(model, mixed, dslots, dst, dactive, F, threads, s, true, setupRanks);
+  rc = slotLaunchInit(model, mixed, dslots, dst, dactive, nSlot, threads, s, true, setupRanks);
   if (rc) return rc;
-  // work per sweep of each fit (Cox: its own tie-group count) for the gang sizes
-  std::vector<int64_t> work(F);
-  for (int f = 0; f < F; ++f) work[f] = slotWork(d, st[f].N);
+  // work per sweep of each slot (its fits; Cox: its own tie-group count) for the gang sizes
+  std::vector<int64_t> work(nSlot);
+  for (int b = 0; b < nSlot; ++b) work[b] = slotWork(d, st[fit0[b]].N) * fitsOf[b];
 
-  // Launch map: every unfinished fit, each on a gang sized by how many fits are
-  // left (the tail of slow fits spreads over the idle CUs).
-  // fits expected to need the most sweeps first: the weakest prior (largest
-  // mean variance over the free coordinates)
-  std::vector<double> priorStrength(F, 0.0);
-  for (int f = 0; f < F; ++f) {
+  // Launch map: every slot with a running fit, each on a gang sized by how many
+  // slots are left (the tail of slow fits spreads over the idle CUs).  Slots
+  // expected to need the most sweeps first: the weakest prior (largest mean
+  // variance over the free coordinates) of their fits.
+  std::vector<double> priority(nSlot, 0.0);
+  for (int f = 0; f < nFit; ++f) {
     double sum = 0.0;
     int cnt = 0;
     for (int j = 0; j < J; ++j)
@@ -1929,23 +2189,24 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
         sum += specs[f].prior_var[j];
         ++cnt;
       }
-    priorStrength[f] = cnt ? sum / cnt : 1e300;
+    const double strength = cnt ? sum / cnt : 1e300;
+    priority[slotOf[f]] = std::max(priority[slotOf[f]], strength);
   }
   std::vector<GangEntry> map;
   int mapN = 0;
   bool mapGangs = false;
-  std::vector<char> running(F, 1);
+  std::vector<char> running(nSlot, 1);
   int gangCap = maxGang;
-  int curFits = 0, curGang = 1;
+  int curFits = nFit, curGang = 1;
   auto buildMap = [&]() -> int {
     std::vector<GangEntry> gangs;
     std::vector<double> prio;
     std::vector<int64_t> fw;
-    for (int f = 0; f < F; ++f)
-      if (running[f] || useGraph) {
-        gangs.push_back(GangEntry{f, 0, 1, 0});
-        prio.push_back(priorStrength[f]);
-        fw.push_back(work[f]);
+    for (int b = 0; b < nSlot; ++b)
+      if (running[b] || useGraph) {
+        gangs.push_back(GangEntry{b, 0, 1, 0});
+        prio.push_back(priority[b]);
+        fw.push_back(work[b]);
       }
     if (!useGraph && gangCap > 1)
       assignGangs(gangs, prio, fw, std::vector<int>(gangs.size(), model), capacity, gangCap,
@@ -1959,7 +2220,6 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
     }
     mapN = (int)map.size();
     mapGangs = G > 1;
-    curFits = (int)gangs.size();
     curGang = G;
     HIP_TRY(hipMemcpyAsync(dmap, map.data(), mapN * sizeof(GangEntry), hipMemcpyHostToDevice, s));
     return 0;
@@ -1969,23 +2229,64 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
     int r = slotLaunchSweeps(mixed, dslots, dst, dmap, mapN, dxch, mapGangs, threads, itl,
                              sweepsPerSync, quantumTicks, s);
     if (r) return r;
-    HIP_TRY(hipMemcpyAsync(o.hst, dst, F * sizeof(FitStatus), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(o.hst, dst, nFit * sizeof(FitStatus), hipMemcpyDeviceToHost, s));
+    return 0;
+  };
+  // coefficients of the fits of slot b in `mask` back to b0(f) (interleaved in
+  // the slot: the others are read back and kept)
+  auto uploadStart = [&](int b, unsigned mask, auto b0) -> int {
+    const int Fb = fitsOf[b];
+    std::vector<double> hb((size_t)J * Fb);
+    if (mask != (1u << Fb) - 1u) {
+      HIP_TRY(hipMemcpyAsync(hb.data(), slots[b].beta, hb.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+    }
+    for (int i = 0; i < Fb; ++i) {
+      if (!((mask >> i) & 1u)) continue;
+      const double* src = b0(fit0[b] + i);
+      for (int j = 0; j < J; ++j) hb[(size_t)j * Fb + i] = src[j];
+    }
+    HIP_TRY(hipMemcpyAsync(slots[b].beta, hb.data(), hb.size() * sizeof(double),
+                           hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));  // hb goes out of scope
     return 0;
   };
   std::memset(&g_stats, 0, sizeof(g_stats));
   g_stats.tuned_pass_items = itl;
   g_stats.tuned_max_gang = gangCap;
+  if (!o.ev0) HIP_TRY(hipEventCreate(&o.ev0));
+  if (!o.ev1) HIP_TRY(hipEventCreate(&o.ev1));
   if (autotune) {
-    rc = tuneBatch<R>(d, F, specs, threads, maxGang, slots, st, dslots, dst, dactive, s, o,
-                      [&](int candItl, int candGang, double* rate) -> int {
-                        itl = candItl;
-                        gangCap = candGang;
-                        int r = buildMap();
-                        if (r) return r;
-                        return timeTrial(mixed, dslots, dst, dmap, mapN, dxch, mapGangs, threads,
-                                         itl, F, J, o, s, rate);
-                      },
-                      &itl, &gangCap);
+    const TuneKey key{d->model, d->K, d->J, nFit, bundleMax, d->precision, clampThreads(threads),
+                      maxGang, d->nnz};
+    rc = tuneBatch(
+        key, threads, maxGang,
+        [&](int candItl, int candGang, double* rate) -> int {
+          itl = candItl;
+          gangCap = candGang;
+          int r = buildMap();
+          if (r) return r;
+          return timeTrial(mixed, dslots, dst, dmap, mapN, dxch, mapGangs, threads, itl, nFit, J,
+                           fit0, fitsOf, o, s, rate);
+        },
+        [&]() -> int {
+          // back to the starting point: coefficients, statuses, then the set-up kernel
+          for (int b = 0; b < nSlot; ++b) {
+            const int r = uploadStart(b, (1u << fitsOf[b]) - 1u,
+                                      [&](int f) { return specs[f].beta0; });
+            if (r) return r;
+          }
+          HIP_TRY(hipMemcpyAsync(dst, st.data(), nFit * sizeof(FitStatus), hipMemcpyHostToDevice, s));
+          HIP_TRY(hipMemcpyAsync(dactive, active.data(), nSlot * sizeof(int32_t),
+                                 hipMemcpyHostToDevice, s));
+          const int r = slotLaunchInit(model, mixed, dslots, dst, dactive, nSlot, threads, s, false,
+                                       setupRanks);
+          if (r) return r;
+          HIP_TRY(hipStreamSynchronize(s));
+          return 0;
+        },
+        &itl, &gangCap);
     if (rc) return rc;
     g_stats.tuned_pass_items = itl;
     g_stats.tuned_max_gang = gangCap;
@@ -2003,10 +2304,9 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
     o.graph = g;
     HIP_TRY(hipGraphInstantiate(&o.exec, o.graph, nullptr, nullptr, 0));
   }
-  std::vector<double> doneAt(F, 0.0);
-  std::vector<int> lastIter(F, 0);
-  if (!o.ev0) HIP_TRY(hipEventCreate(&o.ev0));
-  if (!o.ev1) HIP_TRY(hipEventCreate(&o.ev1));
+  std::vector<double> doneAt(nFit, 0.0);
+  std::vector<int> lastIter(nFit, 0);
+  std::vector<char> pendingRestart(nFit, 0);
   int guard = 0;
   for (;;) {
     if (!useGraph) {
@@ -2033,72 +2333,106 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
       }
       g_stats.launches += 1;
       int rounds = 0;
-      for (int f = 0; f < F; ++f) {
+      for (int f = 0; f < nFit; ++f) {
         rounds = std::max(rounds, o.hst[f].iteration - lastIter[f]);
         lastIter[f] = o.hst[f].iteration;
       }
       g_stats.sweep_rounds += rounds;
     }
     const double now = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    bool allDone = true, retry = false;
     bool misplaced = false;
-    for (int f = 0; f < F; ++f) {
-      if (o.hst[f].fault == 1) return setError(CCDH_E_DEVICE, "gang exchange timed out");
-      if (o.hst[f].fault == 2) {  // a gang spanned XCDs: it backed off untouched
+    for (int b = 0; b < nSlot; ++b) {
+      FitStatus& h = o.hst[fit0[b]];
+      if (h.fault == 1) return setError(CCDH_E_DEVICE, "gang exchange timed out");
+      if (h.fault == 2) {  // a gang spanned XCDs: it backed off untouched
         misplaced = true;
-        o.hst[f].fault = 0;
-        HIP_TRY(hipMemcpyAsync(&dst[f].fault, &o.hst[f].fault, sizeof(int), hipMemcpyHostToDevice, s));
+        h.fault = 0;
+        HIP_TRY(hipMemcpyAsync(&dst[fit0[b]].fault, &h.fault, sizeof(int), hipMemcpyHostToDevice, s));
       }
     }
     if (misplaced) {
       gangCap = 1;
       HIP_TRY(hipStreamSynchronize(s));
     }
-    for (int f = 0; f < F; ++f) {
-      FitStatus& h = o.hst[f];
-      running[f] = !h.done;
+    // CyclicCoordinateDescent::update: ill-conditioned -> bound / 10, cold
+    // restart.  A fit of a bundle restarts from coordinate 0 once the other
+    // fits of its slot have finished (they move in lock step).
+    curFits = 0;
+    bool allDone = true;
+    for (int f = 0; f < nFit; ++f) {
+      const FitStatus& h = o.hst[f];
       if (!h.done) {
-        allDone = false;
+        ++curFits;
         continue;
       }
       if (doneAt[f] == 0.0) doneAt[f] = now;
-      // CyclicCoordinateDescent::update: ill-conditioned -> bound / 10, cold restart
-      if (h.flag == CCDH_ILLCONDITIONED && boundCount[f] < specs[f].max_bound_count) {
-        boundCount[f]++;
-        slots[f].ctl.initialBound /= 10.0;
-        std::vector<double> hb(J, 0.0);
-        if (d->hasOffset) hb[0] = specs[f].beta0[0];
-        HIP_TRY(hipMemcpyAsync(slots[f].beta, hb.data(), J * sizeof(double), hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(dslots + f, &slots[f], sizeof(Slot<R>), hipMemcpyHostToDevice, s));
-        doneAt[f] = 0.0;
-        lastIter[f] = 0;
-        retry = true;
-        active[0] = f;
+      if (h.flag == CCDH_ILLCONDITIONED && boundCount[f] < specs[f].max_bound_count)
+        pendingRestart[f] = 1;
+    }
+    for (int b = 0; b < nSlot; ++b) {
+      bool busy = false, pend = false;
+      unsigned mask = 0;
+      for (int i = 0; i < fitsOf[b]; ++i) {
+        const int f = fit0[b] + i;
+        busy |= !o.hst[f].done;
+        if (pendingRestart[f]) {
+          pend = true;
+          mask |= 1u << i;
+        }
+      }
+      if (pend && !busy) {
+        std::vector<double> zero(J, 0.0);  // cold restart (the offset keeps its 1)
+        std::vector<std::vector<double>> cold(fitsOf[b]);
+        for (int i = 0; i < fitsOf[b]; ++i) {
+          const int f = fit0[b] + i;
+          if (!((mask >> i) & 1u)) continue;
+          pendingRestart[f] = 0;
+          boundCount[f]++;
+          slots[b].initBound[i] /= 10.0;
+          cold[i] = zero;
+          if (d->hasOffset) cold[i][0] = specs[f].beta0[0];
+          doneAt[f] = 0.0;
+          lastIter[f] = 0;
+        }
+        rc = uploadStart(b, mask, [&](int f) { return cold[f - fit0[b]].data(); });
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(dslots + b, &slots[b], sizeof(Slot<R>), hipMemcpyHostToDevice, s));
+        active[0] = b;
         HIP_TRY(hipMemcpyAsync(dactive, active.data(), sizeof(int32_t), hipMemcpyHostToDevice, s));
-        rc = slotLaunchInit(model, mixed, dslots, dst, dactive, 1, threads, s, false, setupRanks);
+        rc = slotLaunchInit(model, mixed, dslots, dst, dactive, 1, threads, s, false, setupRanks,
+                            mask);
         if (rc) return rc;
         HIP_TRY(hipStreamSynchronize(s));
-        running[f] = 1;
+        for (int i = 0; i < fitsOf[b]; ++i)
+          if ((mask >> i) & 1u) o.hst[fit0[b] + i].done = 0;
+        busy = true;
       }
+      running[b] = busy;
+      allDone = allDone && !busy && !pend;
     }
-    if (retry) {
-      for (int f = 0; f < F; ++f) active[f] = f;
-      HIP_TRY(hipMemcpyAsync(dactive, active.data(), F * sizeof(int32_t), hipMemcpyHostToDevice, s));
-      continue;
-    }
+    for (int b = 0; b < nSlot; ++b) active[b] = b;
+    HIP_TRY(hipMemcpyAsync(dactive, active.data(), nSlot * sizeof(int32_t), hipMemcpyHostToDevice, s));
     if (allDone) break;
     if (++guard > 100000000) return setError(CCDH_E_STATE, "batch did not terminate");
   }
-  rc = slotLaunchFinish(model, mixed, dslots, dst, dactive, F, threads, s);
+  rc = slotLaunchFinish(model, mixed, dslots, dst, dactive, nSlot, threads, s);
   if (rc) return rc;
-  std::vector<FitStatus> fin(F);
-  HIP_TRY(hipMemcpyAsync(fin.data(), dst, F * sizeof(FitStatus), hipMemcpyDeviceToHost, s));
-  for (int f = 0; f < F; ++f)
-    HIP_TRY(hipMemcpyAsync(betasOut + (size_t)f * J, slots[f].beta, J * sizeof(double),
+  std::vector<FitStatus> fin(nFit);
+  HIP_TRY(hipMemcpyAsync(fin.data(), dst, nFit * sizeof(FitStatus), hipMemcpyDeviceToHost, s));
+  std::vector<double> hb((size_t)J * Fmax);
+  for (int b = 0; b < nSlot; ++b) {
+    const int Fb = fitsOf[b];
+    double* dstB = Fb == 1 ? betasOut + (size_t)fit0[b] * J : hb.data();
+    HIP_TRY(hipMemcpyAsync(dstB, slots[b].beta, (size_t)J * Fb * sizeof(double),
                            hipMemcpyDeviceToHost, s));
+    if (Fb == 1) continue;
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int i = 0; i < Fb; ++i)
+      for (int j = 0; j < J; ++j) betasOut[(size_t)(fit0[b] + i) * J + j] = hb[(size_t)j * Fb + i];
+  }
   HIP_TRY(hipStreamSynchronize(s));
   const double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  for (int f = 0; f < F; ++f) {
+  for (int f = 0; f < nFit; ++f) {
     g_stats.sweeps += fin[f].iteration;
     g_stats.gather_nnz += fin[f].gatherNnz;
     g_stats.gather_runs += fin[f].gatherRuns;
@@ -2112,7 +2446,8 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
     g_stats.long_columns += fin[f].nLarge;
   }
   g_stats.wall_ms = total * 1e3;
-  for (int f = 0; f < F; ++f) {
+  g_stats.bundles = nSlot;
+  for (int f = 0; f < nFit; ++f) {
     ccdh_fit_result& r = results[f];
     r.log_likelihood = fin[f].logLik;
     double lp = 0.0;
@@ -2128,7 +2463,7 @@ int runBatch(ccdh_data* d, int F, const ccdh_fit_spec* specs, double* betasOut,
   // Cox held-out scoring regroups the risk sets under the held-out weights:
   // run it on the per-coordinate engine with the batch's coefficients.
   if (model == CCDH_COX && anyHeld) {
-    for (int f = 0; f < F; ++f) {
+    for (int f = 0; f < nFit; ++f) {
       if (!specs[f].heldout) continue;
       double v = 0.0;
       rc = coxHeldOut(d, specs[f].weights, specs[f].heldout, betasOut + (size_t)f * J, &v);

@@ -330,7 +330,7 @@ class Server final : public ccdh_server {
     }
     char* slotBlock = r->block + r->dataBytes;
     Slot<R> s;
-    rc = slotPrepare<R>(r->data, r->spec, slotBlock, r->layout, &s, &r->st0, admitStream_);
+    rc = slotPrepare<R>(r->data, &r->spec, 1, slotBlock, r->layout, &s, &r->st0, admitStream_);
     if (rc) return rc;
     // set-up kernel for this fit alone, on the admission stream
     char* tail = slotBlock + r->layout.bytes;
@@ -376,6 +376,7 @@ class Server final : public ccdh_server {
         r->slot = slot;
         slotReq_[slot] = r;
         std::memcpy(&hSlots_[slot], r->slotHost.data(), sizeof(Slot<R>));
+        hSlots_[slot].fit0 = slot;  // its status in the server's table
         if (!rc && (hipMemcpyAsync(dSlots_ + slot, &hSlots_[slot], sizeof(Slot<R>),
                                    hipMemcpyHostToDevice, schedStream_) != hipSuccess ||
                     hipMemcpyAsync(dSt_ + slot, &r->st0, sizeof(FitStatus), hipMemcpyHostToDevice,
@@ -460,7 +461,7 @@ class Server final : public ccdh_server {
       if (h.flag == CCDH_ILLCONDITIONED && r->boundCount < r->spec.max_bound_count) {
         r->boundCount++;
         Slot<R>& s = hSlots_[slot];
-        s.ctl.initialBound /= 10.0;
+        s.initBound[0] /= 10.0;
         std::vector<double> hb(r->J, 0.0);
         if (r->hasOffset) hb[0] = r->spec.beta0[0];
         if (hipMemcpyAsync(s.beta, hb.data(), r->J * sizeof(double), hipMemcpyHostToDevice, schedStream_) != hipSuccess ||

@@ -24,6 +24,7 @@ struct ccdh_data;
 namespace ccdamd {
 
 constexpr int M_ANY = -1;  // sweep kernel dispatching on each slot's model
+constexpr int kMaxBundle = 8;  // fits of one slot moving in lock step
 
 struct FitCtl {
   double tolerance;
@@ -34,6 +35,8 @@ struct FitCtl {
   int pad;
 };
 
+// Status of one fit.  A slot's fits have consecutive statuses; the first also
+// carries the slot's sweep position, fault and work counters.
 struct FitStatus {
   double lastObj;
   double logLik;
@@ -75,13 +78,19 @@ struct alignas(16) GrpSt {
   double nw;
 };
 
+// A slot holds F fits over one data set that move in lock step (a bundle;
+// F = 1, 2, 4 or 8, and 1 for Cox): their per-row, per-group and per-coordinate
+// arrays interleave the fits (entry i of fit f at i*F + f), so the F values one
+// nonzero needs share a line.  The coordinates fixed at their start are common.
 template <typename R>
 struct Slot {
   DevX d;
   int model, K, J, N;
+  int F, lF;  // fits, log2 F
+  int fit0;   // status index of the first fit
   RowSt<R>* row;
   double* kw;
-  double* heldout;  // nullptr when the fit has no held-out weights
+  double* heldout;  // nullptr when the fits have no held-out weights
   GrpSt* grp;       // groups (grouped models only)
   double* numer;
   double* numer2;
@@ -98,13 +107,16 @@ struct Slot {
   Records<R> rec;
   int64_t recStride;  // records per buffer; the second buffer follows the first
   FitCtl ctl;
-  double fixedLL;
+  double fixedLL[kMaxBundle];    // per fit: the Poisson -sum w log(y!) term
+  double initBound[kMaxBundle];  // per fit: starting bound (/10 after each ill-conditioned restart)
 };
 
 // Cross-workgroup exchange of one slot's gang: an arrival counter (zeroed
-// before each launch) and two alternating rounds of values per rank: 4 that
-// are summed over the gang, then 4 that only the lead rank publishes.
-constexpr int kXchSum = 4, kXchLead = 4;
+// before each launch) and two alternating rounds of values per rank: kXchSum
+// that are summed over the gang (gradient and Hessian of each fit), then
+// kXchLead that only the lead rank publishes (coefficient and bound of each
+// fit, end of quantum).
+constexpr int kXchSum = 2 * kMaxBundle, kXchLead = 2 * kMaxBundle + 1;
 struct Xch {
   unsigned cnt;
   unsigned pad[63];
@@ -124,17 +136,20 @@ struct SlotLayout {
   size_t bytes;
 };
 
-// Layout for a fit of `model` over K rows, J coordinates, at most nCap groups
+// Layout for F fits of `model` over K rows, J coordinates, at most nCap groups
 // and a longest column of maxLen nonzeros; rs = bytes of a per-row real.
-SlotLayout slotLayout(int model, int K, int J, int nCap, int64_t maxLen, bool heldout, size_t rs);
+SlotLayout slotLayout(int model, int K, int J, int nCap, int64_t maxLen, bool heldout, size_t rs,
+                      int F = 1);
 
-// Host-side preparation of one slot: groups (Cox pids under the fit's weights),
-// the fixed log-likelihood term, and the uploads of coefficients, priors,
-// weights and groupings into the slot's block.  `out` receives the device
-// pointers; `st` its initial status.  Returns a ccdh status code.
+// Host-side preparation of one slot of F fits (specs sp[0..F), same fixed
+// coordinates and controls; Cox: F = 1): groups (Cox pids under the fit's
+// weights), the fixed log-likelihood terms, and the interleaved uploads of
+// coefficients, priors, weights and groupings into the slot's block.  `out`
+// receives the device pointers (fit0 = 0); st[0..F) the initial statuses.
+// Returns a ccdh status code.
 template <typename R>
-int slotPrepare(const ccdh_data* d, const ccdh_fit_spec& sp, char* block, const SlotLayout& L,
-                Slot<R>* out, FitStatus* st, hipStream_t s);
+int slotPrepare(const ccdh_data* d, const ccdh_fit_spec* sp, int F, char* block,
+                const SlotLayout& L, Slot<R>* out, FitStatus* st, hipStream_t s);
 
 // Groups a fit of this data has under weights w (K for independent rows,
 // strata for CLR, tie groups for Cox).
@@ -146,10 +161,11 @@ int64_t maxColumnLength(const ccdh_data* d);
 // full = false: a restart from the starting coefficients that keeps the
 // weight-dependent terms (nW, XjY) already in the slot.  ranks: workgroups per
 // slot for the set-up (each takes a range of row shards); the starting
-// objective follows in one workgroup per slot.
+// objective follows in one workgroup per slot.  mask: the fits of each slot
+// to set up (bit f; a restart sets up one fit of a bundle).
 int slotLaunchInit(int model, bool mixed, const void* slots, FitStatus* st,
                    const int32_t* active, int n, int threads, hipStream_t s, bool full = true,
-                   int ranks = 1);
+                   int ranks = 1, unsigned mask = ~0u);
 // Set-up workgroups for a fit of nnz nonzeros over K rows (a power of two <= kShards).
 int initRanks(int64_t nnz, int K);
 // One sweep launch over `n` workgroups described by `map` (device array).
@@ -181,8 +197,8 @@ int gangSize(int n, int capacity, int64_t nnz, int model, int maxGang, int64_t m
 void assignGangs(std::vector<GangEntry>& want, const std::vector<double>& priority,
                  const std::vector<int64_t>& nnz, const std::vector<int>& model, int capacity,
                  int maxGang, int64_t minNnzPerRank, bool tiers = false);
-// Work of one sweep of a fit in nonzero-equivalents: its nonzeros, plus for Cox
-// an N-pid risk-set scan per coordinate.
+// Work of a fit in nonzero-equivalents for sizing its gang: its nonzeros, plus
+// for Cox two per pid (the two scan passes of a coordinate).
 int64_t slotWork(const ccdh_data* d, int N);
 int slotLaunchFinish(int model, bool mixed, const void* slots, FitStatus* st,
                      const int32_t* active, int n, int threads, hipStream_t s);

@@ -184,6 +184,9 @@ typedef struct {
   int autotune;          /* time pass width x gangs on this workload first; the choice is cached */
   double quantum_ms;     /* with gangs: a launch ends at the first coordinate boundary after
                             this long, and gang sizes are re-dealt between launches (0: 200) */
+  int bundle;            /* fits per slot moving in lock step over interleaved state: 1, 2, 4
+                            or 8 (0: 1; Cox: 1).  Consecutive fits with the same fixed
+                            coordinates and controls share a slot. */
 } ccdh_batch_options;
 
 /* Batched engine: n_fits fits over one data handle, device-resident sweeps.
@@ -218,6 +221,7 @@ typedef struct {
   int32_t tuned_pass_items, tuned_max_gang;
   double tune_ms;
   double tune_rates[4];
+  int32_t bundles;         /* slots the fits were bundled into */
 } ccdh_batch_stats;
 int ccdh_batch_last_stats(ccdh_batch_stats* out);
 

@@ -3,7 +3,8 @@
 Every fit of a batch matches the CPU oracle run alone to 1e-8 relative
 (coefficients, log likelihood, predictive log likelihood) with the same
 iteration count and return flag; a fit's coefficients are bit-identical alone
-and inside a batch of other fits; graph replay and direct launches agree.
+and inside a batch of other fits at the same gang and bundle size (either only
+reorders the per-coordinate sums); graph replay and direct launches agree.
 """
 import numpy as np
 import pytest
@@ -60,14 +61,14 @@ def test_batch_fixtures_match_oracle(name):
 @pytest.mark.parametrize("name", sorted(SYNTH))
 @pytest.mark.parametrize("gang", [1, 4])
 def test_batch_same_alone_and_in_the_mix(name, gang):
-    """Same bits alone and among other fits at the same gang size (a gang-size
-    change reorders the per-coordinate sums; the cap keeps it fixed here)."""
+    """Same bits alone and among other fits at the same gang size, one fit per
+    slot (a gang-size or bundle change reorders the per-coordinate sums)."""
     p, prior, var, maxit = SYNTH[name]()
     specs, _ = cv_specs(p, prior, var, maxit, n_fold=3, grid=(0.1, 1.0, 10.0))
     d = api.Data(p)
-    b_mix, i_mix = api.fit_batch(d, specs, max_gang=gang, gang_min_nnz=1)
+    b_mix, i_mix = api.fit_batch(d, specs, max_gang=gang, gang_min_nnz=1, bundle=1)
     for i in (0, 4, len(specs) - 1):
-        b_one, i_one = api.fit_batch(d, [specs[i]], max_gang=gang, gang_min_nnz=1)
+        b_one, i_one = api.fit_batch(d, [specs[i]], max_gang=gang, gang_min_nnz=1, bundle=1)
         assert np.array_equal(b_one[0], b_mix[i])
         assert i_one[0]["iterations"] == i_mix[i]["iterations"]
         assert i_one[0]["log_likelihood"] == i_mix[i]["log_likelihood"]
@@ -77,8 +78,8 @@ def test_batch_graph_and_direct_launch_agree():
     p, prior, var, maxit = SYNTH["clr"]()
     specs, _ = cv_specs(p, prior, var, maxit, n_fold=2, grid=(1.0,))
     d = api.Data(p)
-    b1, _ = api.fit_batch(d, specs, use_graph=1, sweeps_per_sync=3)
-    b2, _ = api.fit_batch(d, specs, use_graph=0, sweeps_per_sync=1)
+    b1, _ = api.fit_batch(d, specs, use_graph=1, sweeps_per_sync=3, max_gang=1)
+    b2, _ = api.fit_batch(d, specs, use_graph=0, sweeps_per_sync=1, max_gang=1)
     assert np.array_equal(b1, b2)
 
 
@@ -159,6 +160,37 @@ def test_batch_cox_gangs_tiny_tables(name):
     for i, sp in enumerate(specs):
         bc, ic = orc.fit(p, sp)
         check_fit(bg[i], ig[i], bc, ic)
+
+
+@pytest.mark.parametrize("name", ["lr", "poisson", "clr"])
+@pytest.mark.parametrize("bundle,gang", [(2, 1), (4, 1), (8, 1), (8, 4), (4, 16)])
+def test_batch_bundles_match_oracle(name, bundle, gang):
+    """Fits bundled into slots (lanes carry (nonzero, fit) pairs over interleaved
+    state, the fits step in lock step, a converged fit drops out while the others
+    go on) match the oracle, alone and on gangs."""
+    p, prior, var, maxit = SYNTH[name]()
+    specs, held = cv_specs(p, prior, var, maxit, n_fold=4, grid=(0.05, 0.5, 5.0))
+    d = api.Data(p)
+    bg, ig = api.fit_batch(d, specs, heldouts=held, max_gang=gang, gang_min_nnz=1, bundle=bundle)
+    st = api.batch_last_stats()
+    assert st["bundles"] < len(specs)
+    if name != "poisson":  # (the Poisson design runs every fit to its 40-sweep cap)
+        assert len({r["iterations"] for r in ig}) > 1  # fits of a bundle finish apart
+    for i, sp in enumerate(specs):
+        bc, ic = orc.fit(p, sp, heldout=held[i])
+        check_fit(bg[i], ig[i], bc, ic)
+        np.testing.assert_allclose(ig[i]["pred_log_likelihood"], ic["pred_log_likelihood"],
+                                   rtol=1e-8)
+
+
+def test_batch_bundle_size_changes_only_rounding():
+    p, prior, var, maxit = SYNTH["clr"]()
+    specs, _ = cv_specs(p, prior, var, maxit, n_fold=4, grid=(0.5, 1.0))
+    d = api.Data(p)
+    b1, i1 = api.fit_batch(d, specs, bundle=1)
+    b8, i8 = api.fit_batch(d, specs, bundle=8)
+    np.testing.assert_allclose(b8, b1, rtol=1e-10, atol=1e-13)
+    assert [r["iterations"] for r in i8] == [r["iterations"] for r in i1]
 
 
 def test_batch_gang_size_changes_only_rounding():

@@ -63,7 +63,7 @@ def test_serve_matches_batched_engine():
     p, prior, var, maxit = SYNTH["clr"]()
     specs = [default_spec(p, prior=prior, variance=var * s, max_iterations=maxit)
              for s in (0.5, 1.0, 2.0)]
-    bb, ib = api.fit_batch(api.Data(p), specs)
+    bb, ib = api.fit_batch(api.Data(p), specs, bundle=1)  # one fit per slot, as served
     with api.Server() as srv:
         got = [srv.wait(srv.submit(p, sp)) for sp in specs]
     for i, (b, r, _) in enumerate(got):
