@@ -274,7 +274,8 @@ def main():
     if os.path.exists(tpath):
         t = json.load(open(tpath))
         if (t.get("strata") == n_strata and t.get("covariates") == P and t.get("fits") == n_fits
-                and precision == api.FP64 and args.sweeps_per_sync in (0, t.get("sweeps_per_launch"))):
+                and precision == api.FP64 and args.sweeps_per_sync == 0 and args.max_gang == 0
+                and args.bundle in (0, 1)):
             traffic = t["traffic_bytes_per_launch"]
 
     # plain engine and CPU oracle on a spread of the same fits
