@@ -64,7 +64,9 @@ def serve_leg(args, api, synth):
     pool, reqs = synth.request_stream(
         n_requests=args.serve_requests, rate=args.serve_rate,
         K_range=(10_000, args.serve_kmax), P_range=(1_000, args.serve_pmax),
-        per_row_range=(20, 60), cox_K_max=args.serve_cox_kmax, seed=20261208)
+        per_row_range=(20, 60), cox_K_max=args.serve_cox_kmax, seed=20261208,
+        progress=lambda p: log(f"serving pool: design K={p.K} J={p.J} nnz={p.nnz} "
+                               f"after {time.perf_counter() - t:.1f}s"))
     log(f"serving pool: {len(pool)} designs, nnz {min(p.nnz for p in pool)}.."
         f"{max(p.nnz for p in pool)}, generated in {time.perf_counter() - t:.1f}s")
 
@@ -84,11 +86,23 @@ def serve_leg(args, api, synth):
             time.sleep(ta - now)
         ids.append(srv.submit(pool[i], sp))
     log(f"serving: {len(ids)} requests submitted over {time.perf_counter() - t0:.1f}s")
-    results = []
-    for k, rid in enumerate(ids):
-        results.append(srv.wait(rid))
-        if (k + 1) % 50 == 0:
-            log(f"serving: {k + 1} results after {time.perf_counter() - t0:.1f}s")
+    # collect in completion order (a long request does not hold back the others' results)
+    results = [None] * len(ids)
+    pending = list(range(len(ids)))
+    got = 0
+    while pending:
+        left = []
+        for k in pending:
+            if srv.poll(ids[k]):
+                results[k] = srv.wait(ids[k])
+                got += 1
+                if got % max(1, len(ids) // 10) == 0:
+                    log(f"serving: {got} results after {time.perf_counter() - t0:.1f}s")
+            else:
+                left.append(k)
+        if len(left) == len(pending):
+            time.sleep(0.005)
+        pending = left
     span = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
@@ -110,7 +124,8 @@ def serve_leg(args, api, synth):
     plain_t, plain_diff = [], 0.0
     tp = time.perf_counter()
     for k in order:
-        if time.perf_counter() - tp > args.serve_plain_seconds and plain_t:
+        if args.serve_plain_seconds <= 0 or (time.perf_counter() - tp > args.serve_plain_seconds
+                                             and plain_t):
             break
         _, i, _ = reqs[k]
         t1 = time.perf_counter()
@@ -123,7 +138,7 @@ def serve_leg(args, api, synth):
         plain_diff = max(plain_diff, float(np.max(np.where(np.abs(b) > 0, rel, np.abs(bp)))))
         log(f"plain request {k} ({names[pool[i].model]}, K={pool[i].K}, J={pool[i].J}): "
             f"{plain_t[-1]:.2f}s, {ip['iterations']} iterations (served {results[k][1]['iterations']})")
-    plain_fps = len(plain_t) / sum(plain_t)
+    plain_fps = len(plain_t) / sum(plain_t) if plain_t else None
     return {
         "workload": "served fit requests: logistic / Poisson / Cox, Zipf(1.1) designs",
         "requests": len(ids), "arrival_rate_per_s": args.serve_rate,
@@ -138,7 +153,7 @@ def serve_leg(args, api, synth):
         "iterations_median": int(np.median(iters)), "launches": st["launches"],
         "arena_peak_bytes": st["arena_peak"], "failed": st["failed"],
         "plain_sequential_fits_per_sec": plain_fps, "plain_sample": len(plain_t),
-        "speedup_vs_plain_gpu": fps / plain_fps,
+        "speedup_vs_plain_gpu": fps / plain_fps if plain_fps else None,
         "max_rel_coef_diff_plain_vs_served": plain_diff,
     }
 
@@ -167,7 +182,7 @@ def main():
     ap.add_argument("--serve-cox-kmax", type=int, default=20_000, help="largest Cox request rows")
     ap.add_argument("--serve-quantum-ms", type=float, default=20.0)
     ap.add_argument("--serve-plain-seconds", type=float, default=20.0,
-                    help="time budget of the plain sequential sample")
+                    help="time budget of the plain sequential sample (0: none)")
     ap.add_argument("--skip-batch", action="store_true", help="serving leg only (probe)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="tune pass width and gangs on the workload first (inside the timed step)")

@@ -184,12 +184,14 @@ def lr_zipf(K=100_000, P=10_000, per_row=40, s=1.1, n_true=100, sd=0.5, intercep
 
 def request_stream(n_requests=200, rate=50.0, K_range=(10_000, 200_000),
                    P_range=(1_000, 20_000), per_row_range=(20, 60), cox_K_max=None,
-                   pool_per_model=4, models=(LOGISTIC, POISSON, COX), seed=DEFAULT_SEED):
+                   pool_per_model=4, models=(LOGISTIC, POISSON, COX), seed=DEFAULT_SEED,
+                   progress=None):
     """Served fit requests: a pool of designs per model (rows and columns
     log-uniform in the ranges, nonzeros per row uniform, Zipf(1.1) columns) and
     `n_requests` arrivals at Poisson `rate` per second, each naming a pool design
     and its own prior variance (an outcome model / hyperparameter of its own).
-    Returns (pool, requests) with requests = [(arrival_s, pool_index, variance)]."""
+    Returns (pool, requests) with requests = [(arrival_s, pool_index, variance)].
+    `progress(design)` is called after each design is generated."""
     rng = np.random.default_rng(seed)
 
     def logu(lo, hi):
@@ -211,6 +213,8 @@ def request_stream(n_requests=200, rate=50.0, K_range=(10_000, 200_000),
             else:
                 prob = cox_zipf(K=K, P=P, per_row=per_row, n_true=min(100, P // 10), seed=sd)[0]
             pool.append(prob)
+            if progress is not None:
+                progress(prob)
     t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
     idx = rng.integers(0, len(pool), n_requests)
     var = np.exp(rng.uniform(np.log(0.01), np.log(1.0), n_requests))
