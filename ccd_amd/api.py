@@ -4,6 +4,7 @@ The library must have been built (python -m ccd_amd.build); importing this
 module never falls back to any other implementation: a missing library raises.
 """
 import ctypes
+import dataclasses
 import os
 
 import numpy as np
@@ -24,7 +25,7 @@ EXPORTS = [
     "ccdh_remaining_stats", "ccdh_numerator", "ccdh_grad_hess", "ccdh_update_xbeta",
     "ccdh_loglik", "ccdh_gradient_objective", "ccdh_pred_loglik", "ccdh_get_xbeta",
     "ccdh_fit_plain", "ccdh_batch_fit", "ccdh_batch_last_stats", "ccdh_cv_fold_weights",
-    "ccdh_cv_grid_point", "ccdh_server_create", "ccdh_server_submit",
+    "ccdh_cv_grid_point", "ccdh_bootstrap_weights", "ccdh_server_create", "ccdh_server_submit",
     "ccdh_server_submit_resident", "ccdh_server_poll", "ccdh_server_wait",
     "ccdh_server_get_stats", "ccdh_server_destroy",
 ]
@@ -163,6 +164,7 @@ def load_library(path=LIB_PATH):
         "ccdh_batch_last_stats": (ci, [vp]),
         "ccdh_cv_fold_weights": (ci, [vp, ci, ci, ci, ctypes.c_long, ci, vp]),
         "ccdh_cv_grid_point": (cd, [cd, cd, ci, ci]),
+        "ccdh_bootstrap_weights": (ci, [vp, ci, ctypes.c_long, ci, vp, ci, vp]),
         "ccdh_server_create": (ci, [vp, vp]),
         "ccdh_server_submit": (ci, [vp, vp, vp, vp]),
         "ccdh_server_submit_resident": (ci, [vp, vp, vp, vp]),
@@ -301,6 +303,48 @@ def cv_fold_weights(ids, n_fold, seed, task, by_pid=True):
     _check(load_library().ccdh_cv_fold_weights(_p(ids), len(ids), int(by_pid), n_fold, seed,
                                                task, _p(out)))
     return out
+
+
+def bootstrap_weights(ids, seed, replicate, exclude=None):
+    """Weights of bootstrap replicate `replicate`: per row, how many times its
+    unit (ids: stratum / pid of each row) was drawn."""
+    ids = np.ascontiguousarray(ids, np.int32)
+    ex = None if exclude is None else np.ascontiguousarray(exclude, np.float64)
+    out = np.zeros(len(ids))
+    _check(load_library().ccdh_bootstrap_weights(_p(ids), len(ids), seed, replicate, _p(ex),
+                                                 0 if ex is None else len(ex), _p(out)))
+    return out
+
+
+def bootstrap_batch(data: Data, spec: FitSpec, ids, replicates, seed, warm_start=None, **opts):
+    """Bootstrap replicates (BootstrapDriver::drive, drivers/BootstrapDriver.cpp:47-71)
+    as one batch: replicate r refits `spec` under the weights of bootstrap sample r,
+    every replicate starting from `warm_start` (default spec.beta0).  The reference
+    chains its warm starts from replicate to replicate, so its estimates agree
+    with these to the convergence tolerance rather than bit for bit.  Returns
+    (betas [replicates x J], results)."""
+    b0 = np.array(spec.beta0 if warm_start is None else warm_start, dtype=np.float64)
+    specs = [dataclasses.replace(spec, weights=bootstrap_weights(ids, seed, r), beta0=b0.copy())
+             for r in range(replicates)]
+    return fit_batch(data, specs, **opts)
+
+
+def profile_batch(data: Data, spec: FitSpec, index, values, mode_beta, **opts):
+    """Profile log likelihood (OptimizationProfile::objective, CcdInterface.cpp:223-260)
+    at many values of coordinate `index` as one batch: each fit holds beta[index]
+    fixed at a value and starts from the mode with that entry replaced (the
+    reference evaluates the values one after another from the previous one's
+    estimate; the optimum per value is the same).  Returns (log likelihoods,
+    betas, results)."""
+    specs = []
+    for x in values:
+        b0 = np.array(mode_beta, dtype=np.float64)
+        b0[index] = x
+        fx = np.array(spec.fixed, dtype=np.uint8)
+        fx[index] = 1
+        specs.append(dataclasses.replace(spec, beta0=b0, fixed=fx))
+    betas, res = fit_batch(data, specs, **opts)
+    return np.array([r["log_likelihood"] for r in res]), betas, res
 
 
 def cv_grid_point(lower, upper, size, step):

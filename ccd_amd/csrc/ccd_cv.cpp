@@ -7,6 +7,8 @@
 //     BY_PID holds out whole strata, BY_ROW single rows.
 //   GridSearchCrossValidationDriver::computeGridPoint (GridSearchCrossValidationDriver.cpp:38-48):
 //     log-uniform grid between lower and upper.
+//   BootstrapSelector (drivers/BootstrapSelector.cpp:17-95) with BootstrapDriver::drive
+//     (drivers/BootstrapDriver.cpp:47-71): resampling weights of bootstrap replicates.
 #include <algorithm>
 #include <cmath>
 #include <random>
@@ -55,6 +57,43 @@ extern "C" int ccdh_cv_fold_weights(const int32_t* ids, int K, int by_pid, int n
   } else {
     for (int p = start[fold]; p < start[fold + 1]; ++p) weights_out[perm[p]] = 0.0;
   }
+  return CCDH_OK;
+}
+
+// A sample draws as many pids as are included, uniformly with replacement
+// (std::uniform_int_distribution over the included pids, std::mt19937 seeded
+// by `seed`); a row weighs the number of times its pid was drawn.  The
+// selector's constructor draws one sample and the driver one more before each
+// replicate, so replicate r is the (r+2)-th sample of the stream.  Included
+// pids: those i < n_exclude with exclude[i] == 0, or every pid without exclude.
+extern "C" int ccdh_bootstrap_weights(const int32_t* ids, int K, long seed, int replicate,
+                                      const double* exclude, int n_exclude, double* weights_out) {
+  if (!ids || !weights_out || K <= 0 || replicate < 0 || (exclude && n_exclude < 0))
+    return ccdamd::setError(CCDH_E_INVALID, "bad bootstrap arguments");
+  int N = 0;
+  for (int k = 0; k < K; ++k) {
+    if (ids[k] < 0) return ccdamd::setError(CCDH_E_INVALID, "negative id");
+    N = std::max(N, ids[k] + 1);
+  }
+  std::vector<int> included;
+  if (exclude) {
+    for (int i = 0; i < n_exclude; ++i)
+      if (exclude[i] == 0.0) included.push_back(i);
+  } else {
+    included.resize(N);
+    for (int i = 0; i < N; ++i) included[i] = i;
+  }
+  if (included.empty()) return ccdamd::setError(CCDH_E_INVALID, "no pid to draw");
+  const int nIn = (int)included.size();
+  std::vector<int> count(std::max(N, n_exclude), 0);
+  std::mt19937 prng;
+  prng.seed(seed);
+  for (int sample = 0; sample <= replicate + 1; ++sample) {
+    std::uniform_int_distribution<int> uniform(0, nIn - 1);
+    std::fill(count.begin(), count.end(), 0);
+    for (int i = 0; i < nIn; ++i) count[included[uniform(prng)]]++;
+  }
+  for (int k = 0; k < K; ++k) weights_out[k] = ids[k] < (int)count.size() ? count[ids[k]] : 0.0;
   return CCDH_OK;
 }
 

@@ -234,6 +234,15 @@ int ccdh_cv_fold_weights(const int32_t* ids, int K, int by_pid, int n_fold, long
 /* GridSearchCrossValidationDriver::computeGridPoint (GridSearchCrossValidationDriver.cpp:38-48) */
 double ccdh_cv_grid_point(double lower, double upper, int size, int step);
 
+/* ---- bootstrap batch definition ---- */
+/* BootstrapSelector::permute + getWeights as BootstrapDriver::drive calls them
+ * (drivers/BootstrapSelector.cpp:17-95, drivers/BootstrapDriver.cpp:47-71): the
+ * weights of replicate `replicate` (0-based), each row weighing the number of
+ * times its pid (ids[k]) was drawn.  exclude (nullable, n_exclude pids): only
+ * pids with exclude[i] == 0 are drawn (the selector's wtsExclude). */
+int ccdh_bootstrap_weights(const int32_t* ids, int K, long seed, int replicate,
+                           const double* exclude, int n_exclude, double* weights_out);
+
 /* ---- serving: continuous batching of fit requests ----
  * No reference counterpart (Cyclops fits one model per call; its parallel
  * drivers run whole fits on std::threads, AbstractCrossValidationDriver.cpp:65-74).

@@ -793,6 +793,36 @@ int orc_cv_weights(const int32_t* ids, int K, int by_pid, int nfold, long seed, 
   return 0;
 }
 
+// BootstrapSelector (drivers/BootstrapSelector.cpp:17-95): the selector keeps
+// the drawn pids in a std::multiset and a row weighs the multiset count of its
+// pid; the constructor permutes once and BootstrapDriver::drive
+// (drivers/BootstrapDriver.cpp:56-60) permutes again before each replicate.
+int orc_bootstrap_weights(const int32_t* ids, int K, long seed, int replicate,
+                          const double* exclude, int n_exclude, double* weights_out) {
+  int N = 0;
+  for (int k = 0; k < K; ++k) N = std::max(N, ids[k] + 1);
+  std::vector<int> indicesIncluded;
+  if (exclude) {
+    for (int i = 0; i < n_exclude; ++i)
+      if (exclude[i] == 0) indicesIncluded.push_back(i);
+  } else {
+    for (int i = 0; i < N; ++i) indicesIncluded.push_back(i);
+  }
+  std::mt19937 prng;
+  prng.seed(seed);
+  std::multiset<int> selected;
+  auto permute = [&]() {
+    selected.clear();
+    const int nNew = (int)indicesIncluded.size();
+    std::uniform_int_distribution<int> uniform(0, nNew - 1);
+    for (int i = 0; i < nNew; i++) selected.insert(indicesIncluded[uniform(prng)]);
+  };
+  permute();  // constructor
+  for (int r = 0; r <= replicate; ++r) permute();  // one per replicate
+  for (int k = 0; k < K; ++k) weights_out[k] = (double)selected.count(ids[k]);
+  return 0;
+}
+
 // GridSearchCrossValidationDriver::computeGridPoint (log-uniform grid)
 double orc_grid_point(double lower, double upper, int size, int step) {
   if (size == 1) return upper;

@@ -66,6 +66,10 @@ def lib():
                                         ctypes.c_int]
         _lib.orc_cox_pids.restype = ctypes.c_int
         _lib.orc_cox_pids.argtypes = [vp] * 5
+        _lib.orc_bootstrap_weights.restype = ctypes.c_int
+        _lib.orc_bootstrap_weights.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_long,
+                                               ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                               ctypes.c_void_p]
         _lib.orc_cv_weights.restype = ctypes.c_int
         _lib.orc_cv_weights.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_long, ctypes.c_int,
@@ -163,6 +167,15 @@ def cox_pids(p, weights=None):
     w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
     N = lib().orc_cox_pids(ctypes.byref(d), _ptr(w), _ptr(pid), _ptr(reset), ctypes.byref(nres))
     return N, pid, reset[: nres.value].copy()
+
+
+def bootstrap_weights(ids, seed, replicate, exclude=None):
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    ex = None if exclude is None else np.ascontiguousarray(exclude, dtype=np.float64)
+    out = np.zeros(len(ids), dtype=np.float64)
+    lib().orc_bootstrap_weights(_ptr(ids), len(ids), seed, replicate, _ptr(ex),
+                                0 if ex is None else len(ex), _ptr(out))
+    return out
 
 
 def cv_weights(ids, nfold, seed, task, by_pid=True):

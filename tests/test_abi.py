@@ -57,3 +57,34 @@ def test_serving_arena_allocator(tmp_path):
     subprocess.run(["g++", "-O1", "-std=c++17", "-o", str(exe), src], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True)
     assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
+
+
+def test_bootstrap_weights_match_the_selector_restatement():
+    """Host logic of the C ABI (no device): bootstrap weights equal the oracle's
+    multiset restatement of BootstrapSelector for several seeds and replicates,
+    with and without excluded pids, and are multinomial resampling counts."""
+    import numpy as np
+    from oracle import oracle as orc
+    rng = np.random.default_rng(3)
+    sizes = rng.integers(1, 6, 400)  # uneven strata
+    ids = np.repeat(np.arange(len(sizes)), sizes).astype(np.int32)
+    N = len(sizes)
+    exclude = (rng.random(N) < 0.2).astype(float)
+    for seed in (1, 123, 99991):
+        prev = None
+        for r in range(4):
+            for ex in (None, exclude):
+                w = api.bootstrap_weights(ids, seed, r, exclude=ex)
+                assert np.array_equal(w, orc.bootstrap_weights(ids, seed, r, exclude=ex))
+                per_pid = np.bincount(ids, weights=w, minlength=N) / sizes
+                assert np.array_equal(per_pid, np.round(per_pid)) and per_pid.min() >= 0
+                drawn = N if ex is None else int((ex == 0).sum())
+                assert per_pid.sum() == drawn  # one draw per included pid
+                if ex is not None:
+                    assert np.all(per_pid[ex != 0] == 0)
+                assert per_pid.max() > 1  # drawn with replacement
+            if prev is not None:
+                assert not np.array_equal(w, prev)
+            prev = w
+    with pytest.raises(api.CcdhError):
+        api.bootstrap_weights(np.array([-1], np.int32), 1, 0)

@@ -276,3 +276,47 @@ def test_batch_autotune_matches_oracle_and_caches():
     b2, _ = api.fit_batch(d, specs, heldouts=held, autotune=1, gang_min_nnz=1)
     assert api.batch_last_stats()["tune_ms"] == 0.0  # the cached choice is reused
     np.testing.assert_allclose(b2, bg, rtol=1e-10, atol=1e-13)
+
+
+@pytest.mark.parametrize("name", ["lr", "clr", "cox"])
+def test_batch_bootstrap_replicates_match_oracle(name):
+    """Bootstrap replicates (resampling weights per replicate, all warm-started from
+    the point estimate) run as one batch; each matches the oracle fit of the same
+    weights and start."""
+    p, prior, var, maxit = SYNTH[name]()
+    spec = default_spec(p, prior=prior, variance=var, max_iterations=maxit)
+    b_hat, _ = orc.fit(p, spec)
+    ids = p.strata if p.model == CONDITIONAL_LOGISTIC else np.arange(p.K, dtype=np.int32)
+    d = api.Data(p)
+    bg, ig = api.bootstrap_batch(d, spec, ids, replicates=6, seed=2024, warm_start=b_hat)
+    for r in range(6):
+        w = orc.bootstrap_weights(ids, 2024, r)
+        sp = default_spec(p, prior=prior, variance=var, max_iterations=maxit, weights=w)
+        sp.beta0 = b_hat.copy()
+        bc, ic = orc.fit(p, sp)
+        check_fit(bg[r], ig[r], bc, ic)
+
+
+def test_batch_profile_likelihood_matches_oracle():
+    """Profile log likelihood of one coordinate at seven values as one batch of
+    fits with that coordinate fixed (warm start at the mode); each matches the
+    oracle, and the profile peaks at the unpenalized estimate."""
+    p, _, _, _ = SYNTH["lr"]()
+    spec = default_spec(p, max_iterations=1000)  # no prior: the profile of the MLE
+    b_hat, i_hat = orc.fit(p, spec)
+    j = int(np.argmax(np.abs(b_hat[1:]))) + 1
+    values = b_hat[j] + np.linspace(-0.6, 0.6, 7)
+    d = api.Data(p)
+    ll, bg, ig = api.profile_batch(d, spec, j, values, b_hat)
+    for k, x in enumerate(values):
+        sp = default_spec(p, max_iterations=1000)
+        sp.fixed = sp.fixed.copy()
+        sp.fixed[j] = 1
+        sp.beta0 = b_hat.copy()
+        sp.beta0[j] = x
+        bc, ic = orc.fit(p, sp)
+        check_fit(bg[k], ig[k], bc, ic)
+        assert bg[k][j] == x
+    assert np.argmax(ll) == 3
+    np.testing.assert_allclose(ll[3], i_hat["log_likelihood"], rtol=1e-8)
+    assert np.all(np.diff(ll[:4]) > 0) and np.all(np.diff(ll[3:]) < 0)
