@@ -25,7 +25,8 @@ EXPORTS = [
     "ccdh_remaining_stats", "ccdh_numerator", "ccdh_grad_hess", "ccdh_update_xbeta",
     "ccdh_loglik", "ccdh_gradient_objective", "ccdh_pred_loglik", "ccdh_get_xbeta",
     "ccdh_fit_plain", "ccdh_batch_fit", "ccdh_batch_last_stats", "ccdh_cv_fold_weights",
-    "ccdh_cv_grid_point", "ccdh_bootstrap_weights", "ccdh_server_create", "ccdh_server_submit",
+    "ccdh_cv_grid_point", "ccdh_bootstrap_weights", "ccdh_search_step", "ccdh_search_quadratic",
+    "ccdh_server_create", "ccdh_server_submit",
     "ccdh_server_submit_resident", "ccdh_server_poll", "ccdh_server_wait",
     "ccdh_server_get_stats", "ccdh_server_destroy",
 ]
@@ -165,6 +166,8 @@ def load_library(path=LIB_PATH):
         "ccdh_cv_fold_weights": (ci, [vp, ci, ci, ci, ctypes.c_long, ci, vp]),
         "ccdh_cv_grid_point": (cd, [cd, cd, ci, ci]),
         "ccdh_bootstrap_weights": (ci, [vp, ci, ctypes.c_long, ci, vp, ci, vp]),
+        "ccdh_search_step": (ci, [vp, vp, vp, ci, cd, cd, cd, cd, vp, vp, vp]),
+        "ccdh_search_quadratic": (ci, [vp, vp, vp, ci, vp]),
         "ccdh_server_create": (ci, [vp, vp]),
         "ccdh_server_submit": (ci, [vp, vp, vp, vp]),
         "ccdh_server_submit_resident": (ci, [vp, vp, vp, vp]),
@@ -345,6 +348,58 @@ def profile_batch(data: Data, spec: FitSpec, index, values, mode_beta, **opts):
         specs.append(dataclasses.replace(spec, beta0=b0, fixed=fx))
     betas, res = fit_batch(data, specs, **opts)
     return np.array([r["log_likelihood"] for r in res]), betas, res
+
+
+def search_step(tried, stdstep=10.0, stop_by_y=0.01, stop_by_x=np.log(1.5), first_cut=1.0):
+    """Next point of the unimodal hyperparameter search over `tried` = [(x, mean,
+    stddev), ...] in the order tried: (continue, next x, expected value)."""
+    t = np.asarray(tried, dtype=np.float64).reshape(-1, 3)
+    xs, ys, sd = (np.ascontiguousarray(t[:, i]) for i in range(3))
+    cont, nxt, exp_ = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+    _check(load_library().ccdh_search_step(_p(xs), _p(ys), _p(sd), len(t), stdstep, stop_by_y,
+                                           stop_by_x, first_cut, ctypes.byref(cont),
+                                           ctypes.byref(nxt), ctypes.byref(exp_)))
+    return bool(cont.value), nxt.value, exp_.value
+
+
+def search_quadratic(tried):
+    t = np.asarray(tried, dtype=np.float64).reshape(-1, 3)
+    xs, ys, sd = (np.ascontiguousarray(t[:, i]) for i in range(3))
+    c = np.zeros(3)
+    _check(load_library().ccdh_search_quadratic(_p(xs), _p(ys), _p(sd), len(t), _p(c)))
+    return c
+
+
+def auto_search_cv(data: Data, spec: FitSpec, ids, n_fold=10, seed=123, start_variance=1.0,
+                   max_steps=10, by_pid=True, **opts):
+    """Auto-search cross-validation over one prior variance
+    (AutoSearchCrossValidationDriver::doCrossValidationLoop,
+    drivers/AutoSearchCrossValidationDriver.cpp:63-165, nDim = 1): each step fits
+    the n_fold folds at the current variance as one batch (cold starts), scores
+    each by its held-out predictive log likelihood (a fold that does not converge
+    scores NaN and is left out), and the unimodal search (UniModalSearch(10, 0.01,
+    log 1.5)) proposes the next variance, until it converges or max_steps.
+    Returns (variance, expected score, trace [(variance, mean, stddev)])."""
+    weights = [cv_fold_weights(ids, n_fold, seed, task, by_pid=by_pid) for task in range(n_fold)]
+    var = float(start_variance)
+    tried = []
+    expected = float("nan")
+    for _ in range(max_steps):
+        specs = []
+        for w in weights:
+            pv = np.where(np.asarray(spec.prior_type) != 0, var, np.asarray(spec.prior_var))
+            specs.append(dataclasses.replace(spec, prior_var=pv, weights=w))
+        _, res = fit_batch(data, specs, heldouts=[1.0 - w for w in weights], **opts)
+        pred = np.array([r["pred_log_likelihood"] if r["return_flag"] == 0 else np.nan
+                         for r in res])
+        ok = pred[pred == pred]
+        mean = float(ok.mean())
+        sd = float(np.sqrt(np.mean(ok * ok) - mean * mean))
+        tried.append((var, mean, sd))
+        cont, var, expected = search_step(tried)
+        if not cont:
+            break
+    return var, expected, tried
 
 
 def cv_grid_point(lower, upper, size, step):

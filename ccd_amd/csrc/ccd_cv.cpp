@@ -9,8 +9,12 @@
 //     log-uniform grid between lower and upper.
 //   BootstrapSelector (drivers/BootstrapSelector.cpp:17-95) with BootstrapDriver::drive
 //     (drivers/BootstrapDriver.cpp:47-71): resampling weights of bootstrap replicates.
+//   UniModalSearch (utils/HParSearch.h:30-65, utils/HParSearch.cpp:22-120): the next
+//     prior variance of auto-search cross-validation.
 #include <algorithm>
 #include <cmath>
+#include <limits>
+#include <map>
 #include <random>
 #include <string>
 #include <vector>
@@ -94,6 +98,126 @@ extern "C" int ccdh_bootstrap_weights(const int32_t* ids, int K, long seed, int 
     for (int i = 0; i < nIn; ++i) count[included[uniform(prng)]]++;
   }
   for (int k = 0; k < K; ++k) weights_out[k] = ids[k] < (int)count.size() ? count[ids[k]] : 0.0;
+  return CCDH_OK;
+}
+
+namespace {
+
+// Weighted least squares of y on (1, log x, log^2 x) with weights 1/stddev
+// (100/smallest positive stddev where a stddev is 0; all 1 when none is
+// positive), solved by Gaussian elimination with complete pivoting.
+void quadrLogFit(const std::map<double, std::pair<double, double>>& pts, double c[3]) {
+  double minSd = std::numeric_limits<double>::max();
+  for (const auto& p : pts)
+    if (0 < p.second.second && p.second.second < minSd) minSd = p.second.second;
+  const double zeroAdjust = minSd < std::numeric_limits<double>::max() ? 100.0 / minSd : 1.0;
+  double A[3][3] = {{0}}, b[3] = {0};
+  for (const auto& p : pts) {
+    const double w = p.second.second > 0 ? 1.0 / p.second.second : zeroAdjust;
+    const double lx = std::log(p.first);
+    const double row[3] = {1.0, lx, lx * lx};
+    for (int r = 0; r < 3; ++r) {
+      for (int q = 0; q < 3; ++q) A[r][q] += w * row[r] * row[q];
+      b[r] += w * row[r] * p.second.first;
+    }
+  }
+  int perm[3] = {0, 1, 2};  // column order after pivoting
+  for (int k = 0; k < 3; ++k) {
+    int pr = k, pc = k;
+    for (int r = k; r < 3; ++r)
+      for (int q = k; q < 3; ++q)
+        if (std::fabs(A[r][q]) > std::fabs(A[pr][pc])) {
+          pr = r;
+          pc = q;
+        }
+    for (int q = 0; q < 3; ++q) std::swap(A[k][q], A[pr][q]);
+    std::swap(b[k], b[pr]);
+    for (int r = 0; r < 3; ++r) std::swap(A[r][k], A[r][pc]);
+    std::swap(perm[k], perm[pc]);
+    for (int r = k + 1; r < 3; ++r) {
+      const double m = A[k][k] != 0.0 ? A[r][k] / A[k][k] : 0.0;
+      for (int q = k; q < 3; ++q) A[r][q] -= m * A[k][q];
+      b[r] -= m * b[k];
+    }
+  }
+  double z[3];
+  for (int k = 2; k >= 0; --k) {
+    double t = b[k];
+    for (int q = k + 1; q < 3; ++q) t -= A[k][q] * z[q];
+    z[k] = A[k][k] != 0.0 ? t / A[k][k] : 0.0;
+  }
+  for (int k = 0; k < 3; ++k) c[perm[k]] = z[k];
+}
+
+}  // namespace
+
+// The search keeps every tried (x, mean, stddev) in a map by x (a repeated x
+// replaces its entry) and the best entry (the first, then any later one with a
+// larger mean); with 0 points it proposes 1, with 1 it steps by stdstep (up
+// when x < first_cut, else down), with 2 it steps beyond the better end, with
+// 3+ it steps beyond an end that is best, else proposes the argmax of the
+// quadratic fit in log x and stops when that gains less than stop_by_y
+// (relative) or moves less than stop_by_x in log x.
+extern "C" int ccdh_search_step(const double* xs, const double* ys, const double* sds, int n,
+                                double stdstep, double stop_by_y, double stop_by_x,
+                                double first_cut, int* cont, double* next, double* expected) {
+  if (n < 0 || (n > 0 && (!xs || !ys)) || !cont || !next || !expected)
+    return ccdamd::setError(CCDH_E_INVALID, "bad search arguments");
+  std::map<double, std::pair<double, double>> pts;
+  double bestX = 0.0;
+  for (int i = 0; i < n; ++i) {
+    pts[xs[i]] = {ys[i], sds ? sds[i] : 0.0};
+    if (pts.size() == 1 || ys[i] > pts[bestX].first) bestX = xs[i];
+  }
+  *cont = 1;
+  *next = 0.0;
+  *expected = 0.0;
+  if (pts.empty()) {
+    *next = 1.0;
+  } else if (pts.size() == 1) {
+    const double x = pts.begin()->first;
+    *next = x < first_cut ? x * stdstep : x / stdstep;
+  } else if (pts.size() == 2) {
+    if (pts.begin()->second.first > pts.rbegin()->second.first)
+      *next = pts.begin()->first / stdstep;
+    else
+      *next = pts.rbegin()->first * stdstep;
+  } else {
+    const double bestY = pts[bestX].first;
+    if (pts.begin()->first == bestX) {
+      *next = bestX / stdstep;
+      if (!(bestX > std::numeric_limits<double>::denorm_min())) {
+        *cont = 0;
+        *expected = bestY;
+      }
+    } else if (pts.rbegin()->first == bestX) {
+      *next = bestX * stdstep;
+      if (!(bestX < std::numeric_limits<double>::infinity())) {
+        *cont = 0;
+        *expected = bestY;
+      }
+    } else {
+      double c[3];
+      quadrLogFit(pts, c);
+      const double logArgmax = -c[1] / c[2] / 2;
+      const double expMax = -c[1] * c[1] / c[2] / 4 + c[0];
+      if (bestY == 0.0) *cont = 0;
+      else if ((expMax - bestY) / std::fabs(bestY) < stop_by_y) *cont = 0;
+      else if (std::fabs(logArgmax - std::log(bestX)) < stop_by_x) *cont = 0;
+      *next = std::exp(logArgmax);
+      *expected = expMax;
+    }
+  }
+  return CCDH_OK;
+}
+
+// QuadrLogFit's coefficients (c0 + c1 log x + c2 log^2 x), for tests and callers.
+extern "C" int ccdh_search_quadratic(const double* xs, const double* ys, const double* sds, int n,
+                                     double* c_out) {
+  if (n < 3 || !xs || !ys || !c_out) return ccdamd::setError(CCDH_E_INVALID, "bad fit arguments");
+  std::map<double, std::pair<double, double>> pts;
+  for (int i = 0; i < n; ++i) pts[xs[i]] = {ys[i], sds ? sds[i] : 0.0};
+  quadrLogFit(pts, c_out);
   return CCDH_OK;
 }
 

@@ -243,6 +243,20 @@ double ccdh_cv_grid_point(double lower, double upper, int size, int step);
 int ccdh_bootstrap_weights(const int32_t* ids, int K, long seed, int replicate,
                            const double* exclude, int n_exclude, double* weights_out);
 
+/* ---- auto-search cross-validation ---- */
+/* UniModalSearch::tried for each (xs[i], ys[i], sds[i]) in order, then ::step
+ * (utils/HParSearch.h:30-65, utils/HParSearch.cpp:22-120) as
+ * AutoSearchCrossValidationDriver::doCrossValidationLoop uses it
+ * (drivers/AutoSearchCrossValidationDriver.cpp:63-165, UniModalSearch(10, 0.01,
+ * log 1.5)): *cont = 0 when the search has converged, *next the next prior
+ * variance, *expected the fitted value there.  sds may be null (all 0). */
+int ccdh_search_step(const double* xs, const double* ys, const double* sds, int n, double stdstep,
+                     double stop_by_y, double stop_by_x, double first_cut, int* cont, double* next,
+                     double* expected);
+/* QuadrLogFit (utils/HParSearch.cpp:22-63): c0 + c1 log x + c2 log^2 x, n >= 3. */
+int ccdh_search_quadratic(const double* xs, const double* ys, const double* sds, int n,
+                          double* c_out);
+
 /* ---- serving: continuous batching of fit requests ----
  * No reference counterpart (Cyclops fits one model per call; its parallel
  * drivers run whole fits on std::threads, AbstractCrossValidationDriver.cpp:65-74).

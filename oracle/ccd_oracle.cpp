@@ -40,6 +40,7 @@
 #include <limits>
 #include <random>
 #include <set>
+#include <map>
 #include <vector>
 
 namespace {
@@ -820,6 +821,105 @@ int orc_bootstrap_weights(const int32_t* ids, int K, long seed, int replicate,
   permute();  // constructor
   for (int r = 0; r <= replicate; ++r) permute();  // one per replicate
   for (int k = 0; k < K; ++k) weights_out[k] = (double)selected.count(ids[k]);
+  return 0;
+}
+
+// UniModalSearch (utils/HParSearch.h:30-65, utils/HParSearch.cpp:22-120) as a
+// small class: tried() keeps a map x -> (mean, stddev) and the best entry;
+// step() proposes the next x.  The quadratic fit in log x solves its 3x3 normal
+// equations by Cramer's rule.
+struct OrcSearch {
+  std::map<double, std::pair<double, double>> pts;
+  std::map<double, std::pair<double, double>>::iterator best;
+  void tried(double x, double y, double sd) {
+    pts[x] = std::make_pair(y, sd);
+    if (pts.size() == 1) best = pts.begin();
+    else if (y > best->second.first) best = pts.find(x);
+  }
+  static double det3(const double m[3][3]) {
+    return m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+           m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+           m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+  }
+  void fit(double c[3]) const {
+    double minSd = std::numeric_limits<double>::max();
+    for (auto& p : pts)
+      if (p.second.second > 0 && p.second.second < minSd) minSd = p.second.second;
+    const double adj = minSd < std::numeric_limits<double>::max() ? 100.0 / minSd : 1.0;
+    double M[3][3] = {{0}}, v[3] = {0};
+    for (auto& p : pts) {
+      const double w = p.second.second > 0 ? 1.0 / p.second.second : adj;
+      const double l = std::log(p.first), f[3] = {1.0, l, l * l};
+      for (int a = 0; a < 3; ++a) {
+        v[a] += w * f[a] * p.second.first;
+        for (int b = 0; b < 3; ++b) M[a][b] += w * f[a] * f[b];
+      }
+    }
+    const double D = det3(M);
+    for (int k = 0; k < 3; ++k) {
+      double Mk[3][3];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) Mk[a][b] = b == k ? v[a] : M[a][b];
+      c[k] = det3(Mk) / D;
+    }
+  }
+  void step(double stdstep, double stopY, double stopX, double firstCut, int* cont, double* next,
+            double* expected) const {
+    *cont = 1;
+    *next = 0.0;
+    *expected = 0.0;
+    switch (pts.size()) {
+      case 0: *next = 1.0; break;
+      case 1: {
+        const double x = pts.begin()->first;
+        *next = x < firstCut ? x * stdstep : x / stdstep;
+        break;
+      }
+      case 2:
+        *next = pts.begin()->second.first > pts.rbegin()->second.first ? pts.begin()->first / stdstep
+                                                                       : pts.rbegin()->first * stdstep;
+        break;
+      default:
+        if (pts.begin()->first == best->first) {
+          *next = best->first / stdstep;
+          if (!(best->first > std::numeric_limits<double>::denorm_min())) {
+            *cont = 0;
+            *expected = best->second.first;
+          }
+        } else if (pts.rbegin()->first == best->first) {
+          *next = best->first * stdstep;
+          if (!(best->first < std::numeric_limits<double>::infinity())) {
+            *cont = 0;
+            *expected = best->second.first;
+          }
+        } else {
+          double c[3];
+          fit(c);
+          const double la = -c[1] / c[2] / 2, em = -c[1] * c[1] / c[2] / 4 + c[0];
+          const double mv = best->second.first;
+          if (mv == 0) *cont = 0;
+          else if ((em - mv) / std::fabs(mv) < stopY) *cont = 0;
+          else if (std::fabs(la - std::log(best->first)) < stopX) *cont = 0;
+          *next = std::exp(la);
+          *expected = em;
+        }
+    }
+  }
+};
+
+int orc_search_step(const double* xs, const double* ys, const double* sds, int n, double stdstep,
+                    double stopY, double stopX, double firstCut, int* cont, double* next,
+                    double* expected) {
+  OrcSearch s;
+  for (int i = 0; i < n; ++i) s.tried(xs[i], ys[i], sds ? sds[i] : 0.0);
+  s.step(stdstep, stopY, stopX, firstCut, cont, next, expected);
+  return 0;
+}
+
+int orc_search_quadratic(const double* xs, const double* ys, const double* sds, int n, double* c) {
+  OrcSearch s;
+  for (int i = 0; i < n; ++i) s.tried(xs[i], ys[i], sds ? sds[i] : 0.0);
+  s.fit(c);
   return 0;
 }
 

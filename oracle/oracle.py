@@ -66,6 +66,10 @@ def lib():
                                         ctypes.c_int]
         _lib.orc_cox_pids.restype = ctypes.c_int
         _lib.orc_cox_pids.argtypes = [vp] * 5
+        _lib.orc_search_step.restype = ctypes.c_int
+        _lib.orc_search_step.argtypes = [vp, vp, vp, ctypes.c_int] + [ctypes.c_double] * 4 + [vp] * 3
+        _lib.orc_search_quadratic.restype = ctypes.c_int
+        _lib.orc_search_quadratic.argtypes = [vp, vp, vp, ctypes.c_int, vp]
         _lib.orc_bootstrap_weights.restype = ctypes.c_int
         _lib.orc_bootstrap_weights.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_long,
                                                ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
@@ -167,6 +171,23 @@ def cox_pids(p, weights=None):
     w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
     N = lib().orc_cox_pids(ctypes.byref(d), _ptr(w), _ptr(pid), _ptr(reset), ctypes.byref(nres))
     return N, pid, reset[: nres.value].copy()
+
+
+def search_step(tried, stdstep=10.0, stop_by_y=0.01, stop_by_x=np.log(1.5), first_cut=1.0):
+    t = np.asarray(tried, dtype=np.float64).reshape(-1, 3)
+    xs, ys, sd = (np.ascontiguousarray(t[:, i]) for i in range(3))
+    cont, nxt, exp_ = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+    lib().orc_search_step(_ptr(xs), _ptr(ys), _ptr(sd), len(t), stdstep, stop_by_y, stop_by_x,
+                          first_cut, ctypes.byref(cont), ctypes.byref(nxt), ctypes.byref(exp_))
+    return bool(cont.value), nxt.value, exp_.value
+
+
+def search_quadratic(tried):
+    t = np.asarray(tried, dtype=np.float64).reshape(-1, 3)
+    xs, ys, sd = (np.ascontiguousarray(t[:, i]) for i in range(3))
+    c = np.zeros(3)
+    lib().orc_search_quadratic(_ptr(xs), _ptr(ys), _ptr(sd), len(t), _ptr(c))
+    return c
 
 
 def bootstrap_weights(ids, seed, replicate, exclude=None):

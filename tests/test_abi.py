@@ -88,3 +88,48 @@ def test_bootstrap_weights_match_the_selector_restatement():
             prev = w
     with pytest.raises(api.CcdhError):
         api.bootstrap_weights(np.array([-1], np.int32), 1, 0)
+
+
+# UniModalSearch's own unit example (utils/HParSearch.cpp, UNITTEST_ block): ten
+# (variance, log likelihood) points with R's quadratic fit in log variance and
+# the search's next step.
+_SEARCH_KAT = [(2.000000e+04, -408.199), (2.002884e+03, -328.581), (2.000000e+02, -281.788),
+               (2.002884e+01, -229.966), (2.000000e+00, -184.908), (2.002884e-01, -150.792),
+               (2.000000e-02, -132.081), (2.002884e-03, -127.708), (2.000000e-04, -146.397),
+               (2.002884e-05, -183.208)]
+
+
+def test_search_matches_the_reference_example_and_the_oracle():
+    """Host logic of the C ABI: the unimodal search reproduces the reference's
+    unit example (R: c0 -171.030466, log coefficient -12.662184, log^2 coefficient
+    -1.186039, next step 0.004805411 with the default step 100), and agrees with
+    the oracle's restatement (Cramer's rule instead of pivoted elimination) on
+    every branch."""
+    import numpy as np
+    from oracle import oracle as orc
+    tried = [(x, y, 0.0) for x, y in _SEARCH_KAT]
+    c = api.search_quadratic(tried)
+    np.testing.assert_allclose(c, [-171.030466, -12.662184, -1.186039], rtol=2e-6)
+    cont, nxt, _ = api.search_step(tried, stdstep=100.0)
+    np.testing.assert_allclose(nxt, 0.004805411, rtol=2e-6)
+    assert not cont  # the fitted maximum (-137.2) is below the best point tried (-127.7)
+    rng = np.random.default_rng(8)
+    cases = [[], [(0.5, -10.0, 1.0)], [(2.0, -10.0, 1.0)], [(1.0, -9.0, 0.1), (10.0, -8.0, 0.2)],
+             [(1.0, -8.0, 0.1), (10.0, -9.0, 0.2)]]
+    for _ in range(40):
+        n = int(rng.integers(3, 9))
+        xs = np.exp(rng.uniform(-8, 2, n))
+        ys = -(np.log(xs) + 3.0) ** 2 + rng.normal(0, 0.3, n) - 50
+        sds = np.where(rng.random(n) < 0.2, 0.0, rng.uniform(0.1, 2.0, n))
+        cases.append(list(zip(xs, ys, sds)))
+    cases.append([(1.0, -5.0, 0.1), (0.1, -6.0, 0.1), (0.01, -7.0, 0.1)])  # best at the right end
+    cases.append([(1.0, -7.0, 0.1), (0.1, -6.0, 0.1), (0.01, -5.0, 0.1)])  # best at the left end
+    cases.append([(1.0, -5.0, 0.1), (1.0, -7.0, 0.1), (0.1, -6.0, 0.0), (0.01, -8.0, 0.0)])
+    for t in cases:
+        a, o = api.search_step(t), orc.search_step(t)
+        assert a[0] == o[0]
+        # (normal equations in log x reach (log x)^4: the two solvers agree to ~1e-9)
+        np.testing.assert_allclose(a[1:], o[1:], rtol=1e-7, atol=1e-12)
+        if len({x for x, _, _ in t}) >= 3:
+            np.testing.assert_allclose(api.search_quadratic(t), orc.search_quadratic(t),
+                                       rtol=1e-7, atol=1e-9)

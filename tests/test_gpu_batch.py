@@ -320,3 +320,35 @@ def test_batch_profile_likelihood_matches_oracle():
     assert np.argmax(ll) == 3
     np.testing.assert_allclose(ll[3], i_hat["log_likelihood"], rtol=1e-8)
     assert np.all(np.diff(ll[:4]) > 0) and np.all(np.diff(ll[3:]) < 0)
+
+
+def test_batch_auto_search_cv_matches_oracle_search():
+    """Auto-search cross-validation with each step's folds batched proposes the
+    same variances, scores and end point as the same search driven by oracle fits."""
+    p, prior, var, maxit = SYNTH["clr"]()
+    spec = default_spec(p, prior=prior, variance=var, max_iterations=maxit)
+    d = api.Data(p)
+    v_gpu, e_gpu, trace = api.auto_search_cv(d, spec, p.strata, n_fold=5, seed=77,
+                                             start_variance=0.05, max_steps=8)
+    assert len(trace) >= 3
+    # the same loop over oracle fits
+    weights = [orc.cv_weights(p.strata, 5, 77, t, by_pid=True) for t in range(5)]
+    vv, tried = 0.05, []
+    for k in range(len(trace)):
+        preds = []
+        for w in weights:
+            sp = default_spec(p, prior=prior, variance=vv, weights=w, max_iterations=maxit)
+            _, ic = orc.fit(p, sp, heldout=1.0 - w)
+            preds.append(ic["pred_log_likelihood"] if ic["return_flag"] == 0 else np.nan)
+        pr = np.array(preds)
+        pr = pr[pr == pr]
+        mean = pr.mean()
+        tried.append((vv, mean, np.sqrt(np.mean(pr * pr) - mean * mean)))
+        np.testing.assert_allclose(trace[k][:2], tried[-1][:2], rtol=1e-7)
+        # (the spread is sqrt(E[x^2] - mean^2) as the reference computes it: cancellation)
+        np.testing.assert_allclose(trace[k][2], tried[-1][2], rtol=1e-4)
+        cont, vv, ev = orc.search_step(tried)
+        if not cont:
+            break
+    assert len(tried) == len(trace)
+    np.testing.assert_allclose([v_gpu, e_gpu], [vv, ev], rtol=1e-6)
