@@ -162,6 +162,7 @@ __device__ __forceinline__ bool quantumOver(long long deadline) {
 // records back in one coalesced hop.
 constexpr int kIT = 4;
 constexpr int kITLDefault = 8;  // long-column pass width (nonzeros per lane); 4 is the alternative
+constexpr int kBlkIT = 2;       // nonzeros per lane of a block column (one pass of <= 512)
 
 template <int NMAX>
 struct PassLds {
@@ -625,6 +626,109 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
       break;
     }
     if (s.fixed[j]) continue;
+    // A block of short columns with pairwise disjoint rows (strata): each column
+    // is gathered on its own, then one reduction and one exchange serve the
+    // whole block, then each column takes its step and its update.  No column's
+    // update changes another's gradient, so this is the sequential result.
+    const int B = (F == 1 && MODEL != M_COX && d.blk != nullptr) ? d.blk[j] : 1;
+    if (B > 1) {
+      static_assert(kBlkIT * 256 >= kBlockNnz, "a block column must fit one pass");
+      const long long t0 = wall_clock64();
+      Lanes<kBlkIT, R> L[kBlockCols];
+      double v[2 * kBlockCols], bj[kBlockCols], bnd[kBlockCols], delta[kBlockCols];
+      int64_t nAll[kBlockCols], nMine[kBlockCols];
+      bool live[kBlockCols], stg[kBlockCols];
+      long long runsC[kBlockCols];
+      bool stop = false;
+#pragma unroll
+      for (int c = 0; c < kBlockCols; ++c) {
+        const int jj = j + c;
+        v[2 * c] = v[2 * c + 1] = 0.0;
+        bj[c] = bnd[c] = delta[c] = 0.0;
+        nAll[c] = nMine[c] = 0;
+        runsC[c] = 0;
+        stg[c] = false;
+        live[c] = c < B && !s.fixed[jj];  // uniform
+        if (!live[c]) continue;
+        const int64_t cb = d.colPtr[jj], ce = d.colPtr[jj + 1];
+        int64_t b = cb, e = ce;
+        if (multi) {
+          const int32_t* off = d.colShard + (size_t)jj * (d.nShards + 1);
+          b = cb + off[gg.s0];
+          e = cb + off[gg.s1];
+        }
+        nAll[c] = ce - cb;
+        nMine[c] = e - b;
+        const int fmtc = d.fmt[jj];
+        bj[c] = s.beta[jj];
+        bnd[c] = s.bound[jj];
+        runsC[c] = MODEL == M_CLR ? (long long)(d.segPtr[jj + 1] - d.segPtr[jj]) : nAll[c];
+        stg[c] = MODEL == M_CLR && runsC[c] < nAll[c];
+        if (threadIdx.x == 0) S.cValid[0] = 0;
+        passLoad<MODEL, kBlkIT, R>(s, fmtc, cb, e, b, useW, true, L[c]);
+        double vv[2] = {0.0, 0.0};
+        passGather<MODEL, kBlkIT, R>(fmtc, S, (int)nMine[c], stg[c], L[c], useW, 1, true, vv);
+        v[2 * c] = vv[0];
+        v[2 * c + 1] = vv[1];
+      }
+      blockSumFits<2 * kBlockCols>(v, 1);
+      if (multi) {
+        if (threadIdx.x == 0) {
+          for (int c = 0; c < B; ++c) {
+            xs[2 * c] = v[2 * c];
+            xs[2 * c + 1] = v[2 * c + 1];
+            xl[2 * c] = bj[c];
+            xl[2 * c + 1] = bnd[c];
+          }
+          xl[2 * B] = (deadline > 0 && wall_clock64() >= deadline) ? 1.0 : 0.0;
+        }
+        if (!gangExchangeN(gg, xs, 2 * B, xl, 2 * B + 1)) return -1;
+#pragma unroll
+        for (int c = 0; c < kBlockCols; ++c) {
+          if (c >= B) continue;
+          v[2 * c] = xs[2 * c];
+          v[2 * c + 1] = xs[2 * c + 1];
+          bj[c] = xl[2 * c];
+          bnd[c] = xl[2 * c + 1];
+        }
+        stop = xl[2 * B] != 0.0;
+      }
+      bool moved = false;
+#pragma unroll
+      for (int c = 0; c < kBlockCols; ++c) {
+        if (!live[c]) continue;
+        const int jj = j + c;
+        double g = nAll[c] > 0 ? v[2 * c] - s.xjy[jj] : 0.0, h = nAll[c] > 0 ? v[2 * c + 1] : 0.0;
+        if (h < 0.0) {
+          g = 0.0;
+          h = 0.0;
+        }
+        delta[c] = applyBounds(priorDelta(s.ptype[jj], s.pvar[jj], g, h, bj[c]), &bnd[c]);
+        if (threadIdx.x == 0 && lead) {
+          s.bound[jj] = bnd[c];
+          if (delta[c] != 0.0) s.beta[jj] = bj[c] + delta[c];
+        }
+        moved = moved || delta[c] != 0.0;
+        cG += nMine[c];
+        cGR += MODEL == M_CLR && lead ? runsC[c] : 0;
+      }
+#pragma unroll
+      for (int c = 0; c < kBlockCols; ++c) {
+        if (!live[c] || delta[c] == 0.0 || nMine[c] == 0) continue;
+        if (threadIdx.x == 0) S.cValid[0] = 0;
+        passUpdate<MODEL, kBlkIT, R>(s, S, (int)nMine[c], stg[c], L[c], delta[c], true);
+        cU += nMine[c];
+        cUR += MODEL == M_CLR && lead ? runsC[c] : 0;
+      }
+      if (moved) __syncthreads();
+      tS += wall_clock64() - t0;
+      if (stop) {
+        jEnd = j + B;
+        break;
+      }
+      j += B - 1;
+      continue;
+    }
     Records<R> rec = s.rec;
     if (multi && (recSel++ & 1u)) {
       rec.row += s.recStride;

@@ -43,9 +43,15 @@ struct DevX {
   int nShards;
   const int32_t* shardRow;  // [nShards+1]
   const int32_t* colShard;  // [J * (nShards+1)]
+  // column blocks: blk[j] = B > 1 starts a block of B consecutive short columns
+  // whose rows (CLR: strata) are pairwise disjoint; 1 otherwise, 0 inside a
+  // block; nullptr for Cox
+  const int8_t* blk;        // [J]
 };
 
-constexpr int kShards = 64;  // row shards per data set (largest gang)
+constexpr int kShards = 64;      // row shards per data set (largest gang)
+constexpr int kBlockCols = 4;    // columns of one block at most
+constexpr int kBlockNnz = 512;   // nonzeros of a block column at most
 
 template <typename R>
 __device__ __forceinline__ R colValue(const DevX& d, int fmt, int64_t p);

@@ -541,6 +541,7 @@ DevX ccdh_data::view() const {
   x.nShards = kShards;
   x.shardRow = shardRow;
   x.colShard = colShard;
+  x.blk = blk;
   return x;
 }
 
@@ -587,7 +588,48 @@ int64_t dataFootprint(const ccdh_data_desc* desc) {
   if (desc->model == CCDH_CONDITIONAL_LOGISTIC)
     t += r(nnz * 4) + r((J + 1) * 8) + 2 * r(nnz * 4) + r((nStrata + 1) * 4);
   t += r((kShards + 1) * 4) + r(J * (int64_t)(kShards + 1) * 4);
+  if (desc->model != CCDH_COX) t += r(J);
   return t;
+}
+
+// Column blocks (see DevX::blk): greedy from each block start, a column joins
+// while it is short (1..kBlockNnz nonzeros, not dense) and touches no row
+// (CLR: stratum) of the block's earlier columns, up to kBlockCols.  Updating a
+// column then cannot change another block column's gradient, so the block's
+// columns may share one gather, reduction and exchange and still give the
+// sequential results.
+static void buildBlocks(const ccdh_data_desc* desc, const std::vector<int32_t>& strata,
+                        std::vector<int8_t>& blk) {
+  const int J = desc->J, K = desc->K;
+  const bool clr = desc->model == CCDH_CONDITIONAL_LOGISTIC;
+  blk.assign(J, 1);
+  int nUnit = K;
+  if (clr)
+    for (int k = 0; k < K; ++k) nUnit = std::max(nUnit, strata[k] + 1);
+  std::vector<int32_t> stamp(nUnit, -1);
+  auto isShort = [&](int c) {
+    const int64_t n = desc->col_ptr[c + 1] - desc->col_ptr[c];
+    return desc->fmt[c] != CCDH_DENSE && desc->fmt[c] != CCDH_INTERCEPT && n > 0 && n <= kBlockNnz;
+  };
+  auto unit = [&](int64_t p) { return clr ? strata[desc->rows[p]] : desc->rows[p]; };
+  for (int j = 0; j < J;) {
+    int B = 0;
+    for (int c = j; c < J && B < kBlockCols && isShort(c); ++c) {
+      bool clash = false;
+      for (int64_t p = desc->col_ptr[c]; p < desc->col_ptr[c + 1] && !clash; ++p)
+        clash = stamp[unit(p)] == j;
+      if (clash) break;
+      for (int64_t p = desc->col_ptr[c]; p < desc->col_ptr[c + 1]; ++p) stamp[unit(p)] = j;
+      ++B;
+    }
+    if (B < 2) {
+      ++j;
+      continue;
+    }
+    blk[j] = (int8_t)B;
+    for (int c = j + 1; c < j + B; ++c) blk[c] = 0;
+    j += B;
+  }
 }
 
 // Row shards for workgroup gangs: kShards near-equal row ranges whose
@@ -773,6 +815,11 @@ int dataCreate(const ccdh_data_desc* desc, ccdh_data** out, Bump* place) {
     buildShards(desc, d->hStrata, d->hShardRow, colShard);
     rc = devUpload(&d->shardRow, d->hShardRow.data(), d->hShardRow.size(), &acc, place);
     rc = rc ? rc : devUpload(&d->colShard, colShard.data(), colShard.size(), &acc, place);
+    if (!rc && model != CCDH_COX) {
+      std::vector<int8_t> blk;
+      buildBlocks(desc, d->hStrata, blk);
+      rc = devUpload(&d->blk, blk.data(), blk.size(), &acc, place);
+    }
   }
   if (rc) {
     dataDestroy(d);
@@ -786,7 +833,7 @@ void dataDestroy(ccdh_data* d) {
   if (!d) return;
   void* ptrs[] = {d->colPtr, d->rows, d->vals, d->valsF, d->fmt, d->y, d->strata,
                   d->segPtr, d->segOff, d->segKey, d->stratStart, d->nzKey, d->shardRow,
-                  d->colShard};
+                  d->colShard, d->blk};
   if (!d->placed)
     for (void* p : ptrs)
       if (p) (void)hipFree(p);

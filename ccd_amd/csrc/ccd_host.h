@@ -30,6 +30,7 @@ struct ccdh_data {
   int32_t* nzKey = nullptr;
   int32_t* shardRow = nullptr;
   int32_t* colShard = nullptr;
+  int8_t* blk = nullptr;
   bool placed = false;  // device arrays carved from a caller's block (not freed here)
   // host copies of the small per-column / per-row vectors
   std::vector<int64_t> hColPtr, hSegPtr;
