@@ -192,7 +192,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
-              "independent fits would shard across GPUs by fit, see DESIGN.md", file=sys.stderr)
+              "independent fits would shard across GPUs by fit (each GPU its own fits over its "
+              "own copy of the data, nothing exchanged)", file=sys.stderr)
         sys.exit(2)
 
     from ccd_amd import api, synth
