@@ -26,6 +26,7 @@ EXPORTS = [
     "ccdh_loglik", "ccdh_gradient_objective", "ccdh_pred_loglik", "ccdh_get_xbeta",
     "ccdh_fit_plain", "ccdh_batch_fit", "ccdh_batch_last_stats", "ccdh_cv_fold_weights",
     "ccdh_cv_grid_point", "ccdh_bootstrap_weights", "ccdh_search_step", "ccdh_search_quadratic",
+    "ccdh_cox_groups",
     "ccdh_server_create", "ccdh_server_submit",
     "ccdh_server_submit_resident", "ccdh_server_poll", "ccdh_server_wait",
     "ccdh_server_get_stats", "ccdh_server_destroy",
@@ -168,6 +169,7 @@ def load_library(path=LIB_PATH):
         "ccdh_bootstrap_weights": (ci, [vp, ci, ctypes.c_long, ci, vp, ci, vp]),
         "ccdh_search_step": (ci, [vp, vp, vp, ci, cd, cd, cd, cd, vp, vp, vp]),
         "ccdh_search_quadratic": (ci, [vp, vp, vp, ci, vp]),
+        "ccdh_cox_groups": (ci, [vp, vp, vp, vp, vp, vp, vp]),
         "ccdh_server_create": (ci, [vp, vp]),
         "ccdh_server_submit": (ci, [vp, vp, vp, vp]),
         "ccdh_server_submit_resident": (ci, [vp, vp, vp, vp]),
@@ -348,6 +350,22 @@ def profile_batch(data: Data, spec: FitSpec, index, values, mode_beta, **opts):
         specs.append(dataclasses.replace(spec, beta0=b0, fixed=fx))
     betas, res = fit_batch(data, specs, **opts)
     return np.array([r["log_likelihood"] for r in res]), betas, res
+
+
+def cox_groups(data: Data, weights=None):
+    """Risk-set groups of a Cox data set under `weights`, built on the device:
+    (pid [K], reset [N+1], first row [N+1], first group of each row shard [65], N)."""
+    K = data.problem.K
+    w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+    pid = np.zeros(K, np.int32)
+    reset = np.zeros(K + 1, np.uint8)
+    first = np.zeros(K + 1, np.int32)
+    shard = np.zeros(65, np.int32)
+    n = ctypes.c_int()
+    _check(load_library().ccdh_cox_groups(data.handle, _p(w), _p(pid), _p(reset), _p(first),
+                                          _p(shard), ctypes.byref(n)))
+    N = n.value
+    return pid, reset[:N + 1], first[:N + 1], shard, N
 
 
 def search_step(tried, stdstep=10.0, stop_by_y=0.01, stop_by_x=np.log(1.5), first_cut=1.0):

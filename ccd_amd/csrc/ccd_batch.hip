@@ -1596,6 +1596,112 @@ __global__ void __launch_bounds__(BLOCK)
   }
 }
 
+// Risk-set groups of a Cox fit under its weights, built on the device
+// (setPidForAccumulation, ModelSpecifics.hpp:1809-1868): walking the rows of
+// non-zero weight in their (stratum, -time, y) order, a row starts a new group
+// (pid) unless it is an event at the same time as the previous such row of the
+// same stratum, and a new stratum also marks a reset; zero-weight rows get pid
+// N.  One workgroup per listed slot, three passes over per-thread contiguous
+// row blocks: the last weighted row of each block (an exclusive max-scan gives
+// every thread the weighted row before its block), the groups each block
+// starts (an exclusive sum-scan gives its first pid), then pid, first row and
+// reset flag of every group and cleared numerators.  Also writes the slot's N
+// and the first pid of each row shard (the Cox gang ranks' boundaries).
+template <typename R>
+__global__ void k_cox_groups(Slot<R>* slots, FitStatus* st, const int32_t* active) {
+  Slot<R>& s = slots[active[blockIdx.x]];
+  const DevX& d = s.d;
+  const int K = s.K;
+  const double* w = s.kw;
+  __shared__ int sLast[kMaxBlockThreads];
+  __shared__ int sCnt[kMaxBlockThreads];
+  __shared__ int sN;
+  int a, z;
+  pidBlock(0, K, &a, &z);
+  int last = -1;
+  for (int k = a; k < z; ++k)
+    if (w[k] != 0.0) last = k;
+  sLast[threadIdx.x] = last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = -1;
+    for (int t = 0; t < (int)blockDim.x; ++t) {
+      const int v = sLast[t];
+      sLast[t] = run;
+      run = max(run, v);
+    }
+  }
+  __syncthreads();
+  const int carry = sLast[threadIdx.x];
+  auto starts = [&](int prev, int k) {
+    return prev < 0 || d.strata[k] != d.strata[prev] ||
+           !(d.y[prev] == 1.0 && d.time[prev] == d.time[k] && d.y[k] == 1.0);
+  };
+  int cnt = 0;
+  int prev = carry;
+  for (int k = a; k < z; ++k) {
+    if (w[k] == 0.0) continue;
+    cnt += starts(prev, k) ? 1 : 0;
+    prev = k;
+  }
+  sCnt[threadIdx.x] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int t = 0; t < (int)blockDim.x; ++t) {
+      const int v = sCnt[t];
+      sCnt[t] = run;
+      run += v;
+    }
+    sN = run;
+  }
+  __syncthreads();
+  const int N = sN;
+  int p = sCnt[threadIdx.x] - 1;
+  prev = carry;
+  for (int k = a; k < z; ++k) {
+    if (w[k] == 0.0) {
+      s.pid[k] = N;
+      continue;
+    }
+    if (starts(prev, k)) {
+      ++p;
+      s.pidFirst[p] = k;
+      s.reset[p] = (prev >= 0 && d.strata[k] != d.strata[prev]) ? 1 : 0;
+      s.numer[p] = 0.0;
+      s.numer2[p] = 0.0;
+    }
+    s.pid[k] = p;
+    prev = k;
+  }
+  if (threadIdx.x == 0) {
+    s.pidFirst[N] = K;
+    s.reset[N] = 0;
+    s.numer[N] = 0.0;
+    s.numer2[N] = 0.0;
+    s.N = N;
+    st[s.fit0].N = N;
+  }
+  __syncthreads();
+  // the first pid whose first row is at or after each shard's first row
+  for (int sh = threadIdx.x; sh <= d.nShards; sh += blockDim.x) {
+    int v = N;
+    if (sh == 0) {
+      v = 0;
+    } else if (sh < d.nShards) {
+      const int row = d.shardRow[sh];
+      int lo = 0, hi = N;
+      while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (s.pidFirst[mid] < row) lo = mid + 1;
+        else hi = mid;
+      }
+      v = lo;
+    }
+    s.pidShard[sh] = v;
+  }
+}
+
 // Zero the exchange counters of the gangs of the next launch.
 __global__ void k_gang_reset(const GangEntry* map, int n, Xch* xch) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1642,6 +1748,18 @@ int slotLaunchInit(int /*model*/, bool mixed, const void* slots, FitStatus* st,
                                                       full ? 1 : 0, ranks, mask);
     k_slot_objective<double><<<n, threads, 0, s>>>((const Slot<double>*)slots, st, active, mask);
   }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int slotLaunchCoxGroups(bool mixed, void* slots, FitStatus* st, const int32_t* active, int n,
+                        int threads, hipStream_t s) {
+  if (n <= 0) return 0;
+  threads = clampThreads(threads);
+  if (mixed)
+    k_cox_groups<float><<<n, threads, 0, s>>>((Slot<float>*)slots, st, active);
+  else
+    k_cox_groups<double><<<n, threads, 0, s>>>((Slot<double>*)slots, st, active);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1895,14 +2013,9 @@ SlotLayout slotLayout(int model, int K, int J, int nCap, int64_t maxLen, bool he
   return L;
 }
 
-int slotGroups(const ccdh_data* d, const double* w) {
+int slotGroups(const ccdh_data* d, const double* /*w*/) {
   if (d->model == CCDH_CONDITIONAL_LOGISTIC) return d->nStrata;
-  if (d->model != CCDH_COX) return d->K;
-  std::vector<int32_t> pid, pidFirst;
-  std::vector<uint8_t> reset;
-  int N = 0;
-  coxPids(d, w, pid, &N, reset, pidFirst);
-  return N;
+  return d->K;  // Cox: at most K groups (built on the device, k_cox_groups)
 }
 
 int64_t maxColumnLength(const ccdh_data* d) {
@@ -1957,36 +2070,8 @@ int slotPrepare(const ccdh_data* d, const ccdh_fit_spec* sps, int F, char* block
   x.recStride = L.recStride;
   x.ctl = FitCtl{sp.tolerance, sp.initial_bound, sp.max_iterations, sp.convergence_type,
                  sp.weights != nullptr, 0};
-  // groups
-  int N = d->model == CCDH_CONDITIONAL_LOGISTIC ? d->nStrata : K;
-  if (d->model == CCDH_COX) {
-    std::vector<int32_t> pid, pidFirst;
-    std::vector<uint8_t> reset;
-    coxPids(d, sp.weights, pid, &N, reset, pidFirst);
-    HIP_TRY(hipMemcpyAsync(x.pid, pid.data(), K * sizeof(int32_t), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(x.pidFirst, pidFirst.data(), (N + 1) * sizeof(int32_t),
-                           hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(x.reset, reset.data(), N + 1, hipMemcpyHostToDevice, s));
-    // first pid at or after each shard's first row (pids grow with the row
-    // among rows of non-zero weight; shards start at tie groups)
-    std::vector<int32_t> pidShard(kShards + 1, N);
-    {
-      int k = K;
-      int next = N;
-      for (int sh = kShards; sh >= 0; --sh) {
-        const int lo = sh == kShards ? K : d->hShardRow[sh];
-        for (--k; k >= lo; --k)
-          if (pid[k] < N) next = pid[k];
-        k = lo;
-        pidShard[sh] = sh == kShards ? N : (sh == 0 ? 0 : next);
-      }
-    }
-    HIP_TRY(hipMemcpyAsync(x.pidShard, pidShard.data(), (kShards + 1) * sizeof(int32_t),
-                           hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemsetAsync(x.numer, 0, (N + 1) * sizeof(double), s));
-    HIP_TRY(hipMemsetAsync(x.numer2, 0, (N + 1) * sizeof(double), s));
-    HIP_TRY(hipStreamSynchronize(s));
-  }
+  // groups (Cox: K until k_cox_groups has built them under the fit's weights)
+  const int N = d->model == CCDH_CONDITIONAL_LOGISTIC ? d->nStrata : K;
   x.N = N;
   // per fit: fixed term of the Poisson log likelihood (sum of w * -log(y!)) and starting bound
   for (int f = 0; f < F; ++f) {
@@ -2273,6 +2358,13 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
   HIP_TRY(hipMemcpyAsync(dslots, slots.data(), nSlot * sizeof(Slot<R>), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(dst, st.data(), nFit * sizeof(FitStatus), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(dactive, active.data(), nSlot * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  if (model == CCDH_COX) {  // risk-set groups of every fit under its weights, then their counts
+    rc = slotLaunchCoxGroups(mixed, dslots, dst, dactive, nSlot, threads, s);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(st.data(), dst, nFit * sizeof(FitStatus), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int b = 0; b < nSlot; ++b) slots[b].N = st[fit0[b]].N;
+  }
   const int setupRanks = initRanks(d->nnz, d->K);
   rc = slotLaunchInit(model, mixed, dslots, dst, dactive, nSlot, threads, s, true, setupRanks);
   if (rc) return rc;
@@ -2613,6 +2705,72 @@ int validateSpec(const ccdh_data* d, const ccdh_fit_spec& sp) {
 }
 
 }  // namespace ccdamd
+
+namespace {
+
+template <typename R>
+int coxGroupsOnDevice(ccdh_data* d, const double* weights, int32_t* pidOut, uint8_t* resetOut,
+                      int32_t* pidFirstOut, int32_t* pidShardOut, int* nOut) {
+  const int K = d->K, J = d->J;
+  std::vector<int32_t> pt(J, CCDH_PRIOR_NONE);
+  std::vector<double> pv(J, 1.0), b0(J, 0.0);
+  std::vector<uint8_t> fx(J, 0);
+  ccdh_fit_spec sp{pt.data(), pv.data(), fx.data(), b0.data(), weights, nullptr, 1e-6, 1, 0, 2.0, 1};
+  Owned o;
+  HIP_TRY(hipStreamCreateWithFlags(&o.stream, hipStreamNonBlocking));
+  const SlotLayout L = slotLayout(CCDH_COX, K, J, K, 1, false, sizeof(R), 1);
+  char* block = nullptr;
+  Slot<R>* dslot = nullptr;
+  FitStatus* dst = nullptr;
+  int32_t* dact = nullptr;
+  int rc = devAllocOwned(&block, L.bytes, o.ptrs);
+  rc = rc ? rc : devAllocOwned(&dslot, 1, o.ptrs);
+  rc = rc ? rc : devAllocOwned(&dst, 1, o.ptrs);
+  rc = rc ? rc : devAllocOwned(&dact, 1, o.ptrs);
+  if (rc) return rc;
+  Slot<R> x;
+  FitStatus st;
+  rc = slotPrepare<R>(d, &sp, 1, block, L, &x, &st, o.stream);
+  if (rc) return rc;
+  const int32_t zero = 0;
+  HIP_TRY(hipMemcpyAsync(dslot, &x, sizeof(x), hipMemcpyHostToDevice, o.stream));
+  HIP_TRY(hipMemcpyAsync(dst, &st, sizeof(st), hipMemcpyHostToDevice, o.stream));
+  HIP_TRY(hipMemcpyAsync(dact, &zero, sizeof(zero), hipMemcpyHostToDevice, o.stream));
+  rc = slotLaunchCoxGroups(std::is_same<R, float>::value, dslot, dst, dact, 1, 256, o.stream);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(&st, dst, sizeof(st), hipMemcpyDeviceToHost, o.stream));
+  HIP_TRY(hipStreamSynchronize(o.stream));
+  const int N = st.N;
+  HIP_TRY(hipMemcpyAsync(pidOut, x.pid, (size_t)K * sizeof(int32_t), hipMemcpyDeviceToHost, o.stream));
+  if (resetOut) HIP_TRY(hipMemcpyAsync(resetOut, x.reset, N + 1, hipMemcpyDeviceToHost, o.stream));
+  if (pidFirstOut)
+    HIP_TRY(hipMemcpyAsync(pidFirstOut, x.pidFirst, (size_t)(N + 1) * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, o.stream));
+  if (pidShardOut)
+    HIP_TRY(hipMemcpyAsync(pidShardOut, x.pidShard, (kShards + 1) * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, o.stream));
+  HIP_TRY(hipStreamSynchronize(o.stream));
+  *nOut = N;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int ccdh_cox_groups(ccdh_data_t data, const double* weights, int32_t* pid_out,
+                               uint8_t* reset_out, int32_t* pid_first_out,
+                               int32_t* shard_pid_out, int* n_out) {
+  if (!data || !pid_out || !n_out) return setError(CCDH_E_INVALID, "null argument");
+  if (data->model != CCDH_COX) return setError(CCDH_E_INVALID, "not a Cox data set");
+  try {
+    return data->precision == CCDH_MIXED
+               ? coxGroupsOnDevice<float>(data, weights, pid_out, reset_out, pid_first_out,
+                                          shard_pid_out, n_out)
+               : coxGroupsOnDevice<double>(data, weights, pid_out, reset_out, pid_first_out,
+                                           shard_pid_out, n_out);
+  } catch (const std::exception& e) {
+    return setError(CCDH_E_DEVICE, e.what());
+  }
+}
 
 extern "C" int ccdh_batch_last_stats(ccdh_batch_stats* out) {
   if (!out) return setError(CCDH_E_INVALID, "null argument");

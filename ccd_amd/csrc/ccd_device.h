@@ -30,6 +30,7 @@ struct DevX {
   const float* valsF;       // [nnz] FP32 copy (mixed precision only)
   const int8_t* fmt;        // [J]
   const double* y;          // [K]
+  const double* time;       // [K] Cox event / censoring times (nullptr otherwise)
   const int32_t* strata;    // [K]
   // grouped models: runs of equal stratum inside each column
   const int64_t* segPtr;    // [J+1] run range of column j

@@ -532,6 +532,7 @@ DevX ccdh_data::view() const {
   x.valsF = valsF;
   x.fmt = fmt;
   x.y = y;
+  x.time = time;
   x.strata = strata;
   x.segPtr = segPtr;
   x.segOff = segOff;
@@ -589,6 +590,7 @@ int64_t dataFootprint(const ccdh_data_desc* desc) {
     t += r(nnz * 4) + r((J + 1) * 8) + 2 * r(nnz * 4) + r((nStrata + 1) * 4);
   t += r((kShards + 1) * 4) + r(J * (int64_t)(kShards + 1) * 4);
   if (desc->model != CCDH_COX) t += r(J);
+  if (desc->model == CCDH_COX) t += r(K * 8);  // times
   return t;
 }
 
@@ -761,6 +763,7 @@ int dataCreate(const ccdh_data_desc* desc, ccdh_data** out, Bump* place) {
   }
   rc = rc ? rc : devUpload(&d->fmt, desc->fmt, J, &acc, place);
   rc = rc ? rc : devUpload(&d->y, desc->y, K, &acc, place);
+  if (model == CCDH_COX) rc = rc ? rc : devUpload(&d->time, desc->offs, K, &acc, place);
   rc = rc ? rc : devUpload(&d->strata, d->hStrata.data(), K, &acc, place);
   if (!rc && model == CCDH_CONDITIONAL_LOGISTIC) {
     // runs of equal stratum inside every column (rows are sorted by stratum)
@@ -831,7 +834,7 @@ int dataCreate(const ccdh_data_desc* desc, ccdh_data** out, Bump* place) {
 
 void dataDestroy(ccdh_data* d) {
   if (!d) return;
-  void* ptrs[] = {d->colPtr, d->rows, d->vals, d->valsF, d->fmt, d->y, d->strata,
+  void* ptrs[] = {d->colPtr, d->rows, d->vals, d->valsF, d->fmt, d->y, d->time, d->strata,
                   d->segPtr, d->segOff, d->segKey, d->stratStart, d->nzKey, d->shardRow,
                   d->colShard, d->blk};
   if (!d->placed)

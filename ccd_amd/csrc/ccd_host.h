@@ -22,6 +22,7 @@ struct ccdh_data {
   float* valsF = nullptr;
   int8_t* fmt = nullptr;
   double* y = nullptr;
+  double* time = nullptr;  // Cox only
   int32_t* strata = nullptr;
   int64_t* segPtr = nullptr;
   int32_t* segOff = nullptr;

@@ -343,6 +343,10 @@ class Server final : public ccdh_server {
         hipMemcpyAsync(dF, &r->st0, sizeof(FitStatus), hipMemcpyHostToDevice, admitStream_) != hipSuccess ||
         hipMemcpyAsync(dA, &zero, sizeof(zero), hipMemcpyHostToDevice, admitStream_) != hipSuccess)
       return setError(CCDH_E_DEVICE, "request set-up copies");
+    if (r->model == CCDH_COX) {  // risk-set groups under the request's weights
+      rc = slotLaunchCoxGroups(kMixed, dS, dF, dA, 1, threads_, admitStream_);
+      if (rc) return rc;
+    }
     rc = slotLaunchInit(M_ANY, kMixed, dS, dF, dA, 1, threads_, admitStream_, true,
                         initRanks(r->data->nnz, r->K));
     if (rc) return rc;
@@ -350,6 +354,7 @@ class Server final : public ccdh_server {
       return setError(CCDH_E_DEVICE, "request set-up status");
     const hipError_t e = hipStreamSynchronize(admitStream_);
     if (e != hipSuccess) return setError(CCDH_E_DEVICE, std::string("request set-up: ") + hipGetErrorString(e));
+    s.N = r->st0.N;  // Cox: as the device built it
     r->slotHost.resize(sizeof(Slot<R>));
     std::memcpy(r->slotHost.data(), &s, sizeof(s));
     r->beta.assign(r->J, 0.0);

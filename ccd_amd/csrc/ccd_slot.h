@@ -168,6 +168,12 @@ int slotLaunchInit(int model, bool mixed, const void* slots, FitStatus* st,
                    int ranks = 1, unsigned mask = ~0u);
 // Set-up workgroups for a fit of nnz nonzeros over K rows (a power of two <= kShards).
 int initRanks(int64_t nnz, int K);
+// Cox slots listed in `active`: their risk-set groups under their weights
+// (pid, first row and reset flag per group, rank boundaries, cleared
+// numerators), the slot's N and the first fit's FitStatus::N, on the device.
+// Runs before the set-up.
+int slotLaunchCoxGroups(bool mixed, void* slots, FitStatus* st, const int32_t* active, int n,
+                        int threads, hipStream_t s);
 // One sweep launch over `n` workgroups described by `map` (device array).
 // quantumTicks > 0: a slot stops at the first coordinate boundary after that
 // many device clock ticks (100 MHz) and resumes there in the next launch.

@@ -243,6 +243,16 @@ double ccdh_cv_grid_point(double lower, double upper, int size, int step);
 int ccdh_bootstrap_weights(const int32_t* ids, int K, long seed, int replicate,
                            const double* exclude, int n_exclude, double* weights_out);
 
+/* ---- device-side ingestion ---- */
+/* Cox risk-set groups under `weights` (NULL: all 1), built on the device as the
+ * batched engine and the server build them for every fit
+ * (setPidForAccumulation, engine/ModelSpecifics.hpp:1809-1868): pid_out[K] (N
+ * for zero-weight rows), reset_out[N+1] (1 where a stratum starts), pid_first_out
+ * [N+1] (first row of each group, K at N), shard_pid_out[65] (first group of each
+ * row shard), *n_out = N.  Output pointers other than pid_out may be NULL. */
+int ccdh_cox_groups(ccdh_data_t data, const double* weights, int32_t* pid_out, uint8_t* reset_out,
+                    int32_t* pid_first_out, int32_t* shard_pid_out, int* n_out);
+
 /* ---- auto-search cross-validation ---- */
 /* UniModalSearch::tried for each (xs[i], ys[i], sds[i]) in order, then ::step
  * (utils/HParSearch.h:30-65, utils/HParSearch.cpp:22-120) as

@@ -352,3 +352,33 @@ def test_batch_auto_search_cv_matches_oracle_search():
             break
     assert len(tried) == len(trace)
     np.testing.assert_allclose([v_gpu, e_gpu], [vv, ev], rtol=1e-6)
+
+
+def test_device_cox_groups_match_oracle():
+    """Risk-set groups built on the device (as every batched and served Cox fit
+    builds them) equal the oracle's grouping exactly: tied and untied times,
+    strata, zero-weight rows (leading ones included), CV and bootstrap weights."""
+    probs = [fixtures.load(n)[0] for n in fixtures.names() if n.startswith("cox")]
+    probs += [SYNTH["cox"]()[0], _stratified_cox()]
+    rng = np.random.default_rng(12)
+    for p in probs:
+        d = api.Data(p)
+        lead = np.ones(p.K)
+        lead[: max(1, p.K // 10)] = 0.0
+        ws = [None, lead, rng.choice([0.0, 1.0, 2.0], p.K),
+              orc.cv_weights(np.zeros(p.K, np.int32), 4, 5, 1, by_pid=False),
+              orc.bootstrap_weights(np.arange(p.K, dtype=np.int32), 9, 0)]
+        for w in ws:
+            pid, reset, first, shard, N = api.cox_groups(d, w)
+            No, pido, acc = orc.cox_pids(p, w)
+            assert N == No
+            assert np.array_equal(pid, pido)
+            want = np.zeros(N + 1, np.uint8)
+            want[[r for r in acc if r < N]] = 1
+            assert np.array_equal(reset, want)
+            rows = np.arange(p.K)
+            exp_first = np.full(N + 1, p.K)
+            np.minimum.at(exp_first, pid[pid < N], rows[pid < N])
+            assert np.array_equal(first, exp_first)
+            assert shard[0] == 0 and shard[-1] == N and np.all(np.diff(shard) >= 0)
+        d.close()
