@@ -24,7 +24,7 @@ EXPORTS = [
     "ccdh_fixed_terms_grad_hess", "ccdh_zero_xbeta", "ccdh_axpy_xbeta", "ccdh_compute_xbeta",
     "ccdh_remaining_stats", "ccdh_numerator", "ccdh_grad_hess", "ccdh_update_xbeta",
     "ccdh_loglik", "ccdh_gradient_objective", "ccdh_pred_loglik", "ccdh_get_xbeta",
-    "ccdh_fit_plain", "ccdh_batch_fit", "ccdh_batch_last_stats", "ccdh_cv_fold_weights",
+    "ccdh_mm_grad_hess", "ccdh_fit_plain", "ccdh_fit_mm", "ccdh_batch_fit", "ccdh_batch_last_stats", "ccdh_cv_fold_weights",
     "ccdh_cv_grid_point", "ccdh_bootstrap_weights", "ccdh_search_step", "ccdh_search_quadratic",
     "ccdh_cox_groups",
     "ccdh_server_create", "ccdh_server_submit",
@@ -35,6 +35,16 @@ EXPORTS = [
 
 class CcdhError(RuntimeError):
     pass
+
+
+class NonIncreasing(CcdhError):
+    """An MM fit whose log posterior decreased; carries the coefficients reached
+    and the iteration."""
+
+    def __init__(self, msg, beta, iterations):
+        super().__init__(msg)
+        self.beta = beta
+        self.iterations = iterations
 
 
 class DataDesc(ctypes.Structure):
@@ -161,7 +171,9 @@ def load_library(path=LIB_PATH):
         "ccdh_gradient_objective": (ci, [vp, ci, vp]),
         "ccdh_pred_loglik": (ci, [vp, vp, vp]),
         "ccdh_get_xbeta": (ci, [vp, vp]),
+        "ccdh_mm_grad_hess": (ci, [vp, vp, ci, vp]),
         "ccdh_fit_plain": (ci, [vp, vp, vp, vp]),
+        "ccdh_fit_mm": (ci, [vp, vp, vp, vp]),
         "ccdh_batch_fit": (ci, [vp, ci, vp, vp, vp, vp]),
         "ccdh_batch_last_stats": (ci, [vp]),
         "ccdh_cv_fold_weights": (ci, [vp, ci, ci, ci, ctypes.c_long, ci, vp]),
@@ -266,14 +278,21 @@ def _spec_struct(spec: FitSpec, keep, heldout=None):
                     spec.convergence_type, spec.initial_bound, spec.max_bound_count)
 
 
-def fit_plain(data: Data, spec: FitSpec, heldout=None):
-    """One fit with the host CCD driving per-coordinate kernels."""
+def fit_plain(data: Data, spec: FitSpec, heldout=None, algorithm="ccd"):
+    """One fit with the host driver: "ccd" steps one coordinate at a time through
+    per-coordinate kernels; "mm" steps every coordinate at once from the MM
+    gradient (logistic and conditional logistic models) and raises NonIncreasing
+    when the log posterior decreases."""
     lib = load_library()
     keep = []
     s = _spec_struct(spec, keep, heldout)
     beta = np.zeros(data.problem.J)
     res = FitResult()
-    _check(lib.ccdh_fit_plain(data.handle, ctypes.byref(s), _p(beta), ctypes.byref(res)))
+    fn = {"ccd": lib.ccdh_fit_plain, "mm": lib.ccdh_fit_mm}[algorithm]
+    rc = fn(data.handle, ctypes.byref(s), _p(beta), ctypes.byref(res))
+    if rc == -5:
+        raise NonIncreasing(lib.ccdh_last_error().decode(), beta, res.iterations)
+    _check(rc)
     return beta, res.as_dict()
 
 
@@ -448,6 +467,14 @@ class FitHandle:
 
     def remaining_stats(self, use_w=False):
         _check(self.lib.ccdh_remaining_stats(self.handle, int(use_w)))
+
+    def mm_grad_hess(self, fixed=None, use_w=False):
+        """(gradients, Hessians) of every column from the MM increments."""
+        J = self.data.problem.J
+        fx = np.zeros(J, np.uint8) if fixed is None else np.ascontiguousarray(fixed, np.uint8)
+        gh = np.zeros(2 * J)
+        _check(self.lib.ccdh_mm_grad_hess(self.handle, _p(fx), int(use_w), _p(gh)))
+        return gh[0::2].copy(), gh[1::2].copy()
 
     def grad_hess(self, j, use_w=False):
         _check(self.lib.ccdh_numerator(self.handle, j, int(use_w)))

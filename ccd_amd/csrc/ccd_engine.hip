@@ -518,7 +518,190 @@ __global__ void k_groups_log(int N, const double* nw, const double* denom, doubl
   gridFinish<2>(v, partials, ticket, out);
 }
 
+// ------------------------------------------- transpose and the MM step
+// The row-major copy follows CompressedDataMatrix::transpose
+// (CompressedDataMatrix.cpp:151-208): with INDICATOR columns and no DENSE one
+// every row is an INDICATOR row, whose entries count 1 whatever the column's
+// value; otherwise entries keep their values.  Zero values are left out (an
+// exact zero added to a row sum changes nothing for finite coefficients).
+
+// entries per row; one workgroup per column (grid-stride over columns)
+__global__ void k_t_count(DevX d, int32_t* cnt) {
+  for (int j = blockIdx.x; j < d.J; j += gridDim.x) {
+    const int fmt = d.fmt[j];
+    const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
+    for (int64_t p = b + threadIdx.x; p < e; p += blockDim.x) {
+      if (colValue<double>(d, fmt, p) == 0.0) continue;
+      atomicAdd(&cnt[colRow(d, fmt, p, b)], 1);
+    }
+  }
+}
+
+// exclusive scan of cnt[0..K) into ptr[0..K], one workgroup: each thread sums a
+// contiguous run, the run totals are scanned in LDS, then each run is written
+__global__ void k_scan_rows(int K, const int32_t* cnt, int64_t* ptr) {
+  __shared__ int64_t part[kMaxBlockThreads];
+  const int T = blockDim.x, t = threadIdx.x;
+  const int64_t per = ((int64_t)K + T - 1) / T;
+  const int64_t b = std::min<int64_t>(K, t * per), e = std::min<int64_t>(K, b + per);
+  int64_t s = 0;
+  for (int64_t i = b; i < e; ++i) s += cnt[i];
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int64_t acc = 0;
+    for (int i = 0; i < T; ++i) {
+      const int64_t v = part[i];
+      part[i] = acc;
+      acc += v;
+    }
+    ptr[K] = acc;
+  }
+  __syncthreads();
+  int64_t acc = part[t];
+  for (int64_t i = b; i < e; ++i) {
+    ptr[i] = acc;
+    acc += cnt[i];
+  }
+}
+
+// scatter every entry to its row (slot order within a row is arbitrary here)
+__global__ void k_t_fill(DevX d, unsigned long long* cursor, int32_t* tCol, double* tVal) {
+  for (int j = blockIdx.x; j < d.J; j += gridDim.x) {
+    const int fmt = d.fmt[j];
+    const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
+    for (int64_t p = b + threadIdx.x; p < e; p += blockDim.x) {
+      const double v = colValue<double>(d, fmt, p);
+      if (v == 0.0) continue;
+      const unsigned long long slot = atomicAdd(&cursor[colRow(d, fmt, p, b)], 1ull);
+      tCol[slot] = j;
+      if (tVal) tVal[slot] = v;
+    }
+  }
+}
+
+// sort each row's entries by column (rows are short: insertion sort per thread)
+__global__ void k_t_sort(int K, const int64_t* ptr, int32_t* tCol, double* tVal) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < K;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = ptr[k], e = ptr[k + 1];
+    for (int64_t q = b + 1; q < e; ++q) {
+      const int32_t c = tCol[q];
+      const double v = tVal ? tVal[q] : 0.0;
+      int64_t r = q - 1;
+      while (r >= b && tCol[r] > c) {
+        tCol[r + 1] = tCol[r];
+        if (tVal) tVal[r + 1] = tVal[r];
+        --r;
+      }
+      tCol[r + 1] = c;
+      if (tVal) tVal[r + 1] = v;
+    }
+  }
+}
+
+// ModelSpecifics::computeXBetaImpl (ModelSpecifics.hpp:346-364): each row's sum
+// over its entries in column order
+template <typename R>
+__global__ void k_xbeta_rows(int K, const int64_t* ptr, const int32_t* tCol, const double* tVal,
+                             const double* beta, R* xb) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < K;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    double sum = 0.0;
+    for (int64_t q = ptr[k]; q < ptr[k + 1]; ++q)
+      sum += (tVal ? tVal[q] : 1.0) * beta[tCol[q]];
+    xb[k] = (R)sum;
+  }
+}
+
+// MM increments (ModelSpecifics.h:500-517, 571-584) factor into a per-row value
+// r_k = nW_g * oeb_k / denom_g over the row's group g (logistic: nW_k = w_k and
+// denom_k = 1 + oeb_k); a column's gradient is sum r_k x_k and its Hessian
+// sum r_k x_k^2.
+template <typename R>
+__global__ void k_mm_rows(int model, int K, const R* oeb, const double* kw, const double* nw,
+                          const double* denom, const int32_t* strata, double* r) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < K;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const double o = (double)oeb[k];
+    if (model == M_LOGISTIC) {
+      r[k] = kw[k] * o / (1.0 + o);
+    } else {
+      const int s = strata[k];
+      r[k] = nw[s] * o / denom[s];
+    }
+  }
+}
+
+constexpr int kMmChunk = 2048;  // nonzeros of one column chunk (one wave)
+
+// one wave per column chunk: partial (gradient, Hessian) sums
+__global__ void k_mm_chunks(DevX d, const double* r, const int32_t* chunk0,
+                            const int32_t* chunkCol, int nChunks, const uint8_t* fixed,
+                            double* part) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int c = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (c >= nChunks) return;  // uniform over the wave
+  const int j = chunkCol[c];
+  double g = 0.0, h = 0.0;
+  if (!fixed[j]) {
+    const int fmt = d.fmt[j];
+    const int64_t cb = d.colPtr[j];
+    const int64_t b = cb + (int64_t)(c - chunk0[j]) * kMmChunk;
+    const int64_t e = std::min<int64_t>(d.colPtr[j + 1], b + kMmChunk);
+    if (isIndicator(fmt)) {
+      for (int64_t p = b + lane; p < e; p += kWave) g += r[colRow(d, fmt, p, cb)];
+      h = g;
+    } else {
+      for (int64_t p = b + lane; p < e; p += kWave) {
+        const double t = r[colRow(d, fmt, p, cb)], x = d.vals[p];
+        g += t * x;
+        h += t * x * x;
+      }
+    }
+  }
+  g = waveSum(g);
+  h = waveSum(h);
+  if (lane == 0) {
+    part[2 * c] = g;
+    part[2 * c + 1] = h;
+  }
+}
+
+// per column: chunk sums in chunk order, minus XjY; fixed columns (0, 0)
+__global__ void k_mm_finish(int J, const int32_t* chunk0, const double* part, const double* xjy,
+                            const uint8_t* fixed, double* gh) {
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < J; j += gridDim.x * blockDim.x) {
+    double g = 0.0, h = 0.0;
+    if (!fixed[j]) {
+      for (int c = chunk0[j]; c < chunk0[j + 1]; ++c) {
+        g += part[2 * c];
+        h += part[2 * c + 1];
+      }
+      g -= xjy[j];
+    }
+    gh[2 * j] = g;
+    gh[2 * j + 1] = h;
+  }
+}
+
 }  // namespace
+
+namespace ccdamd {
+void dataDestroy(ccdh_data* d) {
+  if (!d) return;
+  void* ptrs[] = {d->colPtr, d->rows, d->vals, d->valsF, d->fmt, d->y, d->time, d->strata,
+                  d->segPtr, d->segOff, d->segKey, d->stratStart, d->nzKey, d->shardRow,
+                  d->colShard, d->blk};
+  if (!d->placed)
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+  void* lazy[] = {d->tRowPtr, d->tCol, d->tVal, d->mmChunk0, d->mmChunkCol};
+  for (void* p : lazy)
+    if (p) (void)hipFree(p);
+  delete d;
+}
+}  // namespace ccdamd
 
 // ================================================================ host side
 DevX ccdh_data::view() const {
@@ -832,16 +1015,7 @@ int dataCreate(const ccdh_data_desc* desc, ccdh_data** out, Bump* place) {
   return 0;
 }
 
-void dataDestroy(ccdh_data* d) {
-  if (!d) return;
-  void* ptrs[] = {d->colPtr, d->rows, d->vals, d->valsF, d->fmt, d->y, d->time, d->strata,
-                  d->segPtr, d->segOff, d->segKey, d->stratStart, d->nzKey, d->shardRow,
-                  d->colShard, d->blk};
-  if (!d->placed)
-    for (void* p : ptrs)
-      if (p) (void)hipFree(p);
-  delete d;
-}
+
 
 // ---------------------------------------------------------------- fit state
 static int ensureGroups(ccdh_fit* f, int N) {
@@ -985,7 +1159,8 @@ void fitDestroy(ccdh_fit* f) {
   if (f->stream) (void)hipStreamSynchronize(f->stream);
   void* ptrs[] = {f->xb,  f->oeb,       f->kw,      f->w2,      f->xjy,     f->partials,
                   f->ticket, f->dres,   f->denom,   f->nw,      f->numer,   f->numer2,
-                  f->accTmp, f->pid,    f->pidFirst, f->resetFlag, f->tileAgg, f->tileCarry};
+                  f->accTmp, f->pid,    f->pidFirst, f->resetFlag, f->tileAgg, f->tileCarry,
+                  f->mmR,    f->mmPart, f->mmGH,     f->mmBeta,    f->mmFixed};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (f->hres) (void)hipHostFree(f->hres);
@@ -1320,6 +1495,134 @@ int getXBeta(ccdh_fit* f, double* out) {
   return 0;
 }
 
+// Row-major copy of X and the MM column chunks, built once per data set on the
+// device: entries per row, an exclusive scan, a scatter, a per-row sort.
+static int ensureTranspose(ccdh_data* d, hipStream_t s) {
+  if (d->tRowPtr) return 0;
+  const int K = d->K, J = d->J;
+  bool anyDense = false, anyIndicator = false;
+  for (int j = 0; j < J; ++j) {
+    anyDense = anyDense || d->hFmt[j] == CCDH_DENSE;
+    anyIndicator = anyIndicator || d->hFmt[j] == CCDH_INDICATOR;
+  }
+  const bool positionsOnly = anyIndicator && !anyDense;
+  int64_t* ptr = nullptr;
+  int32_t *cnt = nullptr, *col = nullptr, *c0 = nullptr, *cc = nullptr;
+  unsigned long long* cursor = nullptr;
+  double* val = nullptr;
+  auto fail = [&](hipError_t e, const char* what) {
+    void* all[] = {ptr, cnt, col, c0, cc, cursor, val};
+    for (void* p : all)
+      if (p) (void)hipFree(p);
+    return setError(CCDH_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+  };
+#define T_TRY(expr)                            \
+  do {                                         \
+    hipError_t e_ = (expr);                    \
+    if (e_ != hipSuccess) return fail(e_, #expr); \
+  } while (0)
+  T_TRY(hipMalloc(&ptr, (size_t)(K + 1) * sizeof(int64_t)));
+  T_TRY(hipMalloc(&cnt, (size_t)K * sizeof(int32_t)));
+  T_TRY(hipMemsetAsync(cnt, 0, (size_t)K * sizeof(int32_t), s));
+  const int colGrid = std::max(1, std::min(J, 65535));
+  k_t_count<<<colGrid, kThreads, 0, s>>>(d->view(), cnt);
+  T_TRY(hipGetLastError());
+  k_scan_rows<<<1, kMaxBlockThreads, 0, s>>>(K, cnt, ptr);
+  T_TRY(hipGetLastError());
+  int64_t total = 0;
+  T_TRY(hipMemcpyAsync(&total, ptr + K, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  T_TRY(hipStreamSynchronize(s));
+  T_TRY(hipMalloc(&col, (size_t)std::max<int64_t>(total, 1) * sizeof(int32_t)));
+  if (!positionsOnly) T_TRY(hipMalloc(&val, (size_t)std::max<int64_t>(total, 1) * sizeof(double)));
+  T_TRY(hipMalloc(&cursor, (size_t)K * sizeof(unsigned long long)));
+  T_TRY(hipMemcpyAsync(cursor, ptr, (size_t)K * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+  k_t_fill<<<colGrid, kThreads, 0, s>>>(d->view(), cursor, col, val);
+  T_TRY(hipGetLastError());
+  k_t_sort<<<gridFor(K, 1), kThreads, 0, s>>>(K, ptr, col, val);
+  T_TRY(hipGetLastError());
+  std::vector<int32_t> h0(J + 1, 0), hc;
+  for (int j = 0; j < J; ++j) {
+    const int64_t n = d->hColPtr[j + 1] - d->hColPtr[j];
+    const int nc = (int)((n + kMmChunk - 1) / kMmChunk);
+    h0[j + 1] = h0[j] + nc;
+    for (int i = 0; i < nc; ++i) hc.push_back(j);
+  }
+  T_TRY(hipMalloc(&c0, (size_t)(J + 1) * sizeof(int32_t)));
+  T_TRY(hipMalloc(&cc, std::max<size_t>(hc.size(), 1) * sizeof(int32_t)));
+  T_TRY(hipMemcpyAsync(c0, h0.data(), h0.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  if (!hc.empty())
+    T_TRY(hipMemcpyAsync(cc, hc.data(), hc.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  T_TRY(hipStreamSynchronize(s));
+#undef T_TRY
+  (void)hipFree(cnt);
+  (void)hipFree(cursor);
+  d->tRowPtr = ptr;
+  d->tCol = col;
+  d->tVal = val;
+  d->mmChunk0 = c0;
+  d->mmChunkCol = cc;
+  d->nChunks = (int)hc.size();
+  d->devBytes += (int64_t)(K + 1) * 8 + total * (positionsOnly ? 4 : 12) +
+                 (int64_t)(J + 1 + hc.size()) * 4;
+  return 0;
+}
+
+static int ensureMm(ccdh_fit* f) {
+  ccdh_data* d = f->d;
+  int rc = ensureTranspose(d, f->stream);
+  if (!rc && !f->mmPart) rc = devAlloc(&f->mmPart, 2 * (size_t)std::max(d->nChunks, 1), nullptr);
+  if (!rc && !f->mmGH) rc = devAlloc(&f->mmGH, 2 * (size_t)d->J, nullptr);
+  if (!rc && !f->mmBeta) rc = devAlloc(&f->mmBeta, (size_t)d->J, nullptr);
+  if (!rc && !f->mmFixed) rc = devAlloc(&f->mmFixed, (size_t)d->J, nullptr);
+  if (!rc && !f->mmR) rc = devAlloc(&f->mmR, (size_t)d->K, nullptr);
+  return rc;
+}
+
+// ModelSpecifics::computeXBeta(beta) (ModelSpecifics.hpp:313-364): xBeta from
+// the row-major copy, one launch
+int computeXBetaRows(ccdh_fit* f, const double* beta) {
+  if (!beta) return setError(CCDH_E_INVALID, "beta required");
+  int rc = ensureMm(f);
+  if (rc) return rc;
+  ccdh_data* d = f->d;
+  HIP_TRY(hipMemcpyAsync(f->mmBeta, beta, d->J * sizeof(double), hipMemcpyHostToDevice,
+                         f->stream));
+  LAUNCH_R(k_xbeta_rows, gridFor(d->K, 1), d->K, d->tRowPtr, d->tCol, d->tVal, f->mmBeta,
+           XB(T_));
+  f->statsKnown = false;
+  return 0;
+}
+
+// computeMMGradientAndHessian (ModelSpecifics.hpp:776-889) for every column:
+// per-row factors, one wave per column chunk, then chunk sums per column
+int mmGradHess(ccdh_fit* f, const uint8_t* fixed, double* gh) {
+  ccdh_data* d = f->d;
+  if (d->model != CCDH_LOGISTIC && d->model != CCDH_CONDITIONAL_LOGISTIC)
+    return setError(CCDH_E_INVALID,
+                    "MM gradient: not model-specific (logistic and conditional logistic only)");
+  if (!fixed || !gh) return setError(CCDH_E_INVALID, "null argument");
+  if (!f->weightsSet || !f->statsKnown)
+    return setError(CCDH_E_STATE, "MM gradient before weights and statistics");
+  int rc = ensureMm(f);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(f->mmFixed, fixed, d->J, hipMemcpyHostToDevice, f->stream));
+  LAUNCH_R(k_mm_rows, gridFor(d->K), d->model, d->K, OEB(T_), f->kw, f->nw, f->denom, d->strata,
+           f->mmR);
+  const int wavesPerBlock = kThreads / kWave;
+  if (d->nChunks > 0) {
+    k_mm_chunks<<<(d->nChunks + wavesPerBlock - 1) / wavesPerBlock, kThreads, 0, f->stream>>>(
+        d->view(), f->mmR, d->mmChunk0, d->mmChunkCol, d->nChunks, f->mmFixed, f->mmPart);
+    HIP_TRY(hipGetLastError());
+  }
+  k_mm_finish<<<gridFor(d->J, 1), kThreads, 0, f->stream>>>(d->J, d->mmChunk0, f->mmPart, f->xjy,
+                                                            f->mmFixed, f->mmGH);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(gh, f->mmGH, 2 * (size_t)d->J * sizeof(double), hipMemcpyDeviceToHost,
+                         f->stream));
+  HIP_TRY(hipStreamSynchronize(f->stream));
+  return 0;
+}
+
 }  // namespace ccdamd
 
 // =================================================================== C ABI
@@ -1415,9 +1718,13 @@ int ccdh_axpy_xbeta(ccdh_fit_t fit, double beta, int j) {
 
 int ccdh_compute_xbeta(ccdh_fit_t fit, const double* beta) {
   GUARD(fit);
-  int rc = zeroXBeta(fit);
-  for (int j = 0; !rc && j < fit->d->J; ++j) rc = axpyXBeta(fit, beta[j], j);
-  return rc;
+  return computeXBetaRows(fit, beta);
+}
+
+int ccdh_mm_grad_hess(ccdh_fit_t fit, const uint8_t* fixed, int use_weights, double* gh_out) {
+  GUARD(fit);
+  (void)use_weights;  // the MM increments weigh by nW whatever the flag
+  return mmGradHess(fit, fixed, gh_out);
 }
 
 int ccdh_remaining_stats(ccdh_fit_t fit, int use_weights) {

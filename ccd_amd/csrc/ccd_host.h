@@ -32,6 +32,14 @@ struct ccdh_data {
   int32_t* shardRow = nullptr;
   int32_t* colShard = nullptr;
   int8_t* blk = nullptr;
+  // row-major copy (ModelSpecifics::computeXBeta over the transpose) and the MM
+  // step's column chunks; built on first use, never carved from `place`
+  int64_t* tRowPtr = nullptr;   // [K+1]
+  int32_t* tCol = nullptr;      // [entries] columns ascending within a row
+  double* tVal = nullptr;       // [entries]; nullptr: every entry counts 1
+  int32_t* mmChunk0 = nullptr;  // [J+1] first chunk of each column
+  int32_t* mmChunkCol = nullptr;// [nChunks] column of each chunk
+  int nChunks = 0;
   bool placed = false;  // device arrays carved from a caller's block (not freed here)
   // host copies of the small per-column / per-row vectors
   std::vector<int64_t> hColPtr, hSegPtr;
@@ -68,6 +76,12 @@ struct ccdh_fit {
   unsigned* ticket = nullptr;
   double* dres = nullptr;
   double* hres = nullptr;  // pinned
+  // MM step: per-row factor, per-chunk partial sums, results, coefficients, flags
+  double* mmR = nullptr;
+  double* mmPart = nullptr;
+  double* mmGH = nullptr;
+  double* mmBeta = nullptr;
+  uint8_t* mmFixed = nullptr;
   double fixedLL = 0.0;
   bool weightsSet = false, statsKnown = false;
   std::vector<double> hKw;
@@ -102,6 +116,8 @@ int logLik(ccdh_fit* f, int useCV, double* out);
 int gradientObjective(ccdh_fit* f, int useCV, double* out);
 int predLogLik(ccdh_fit* f, const double* w, double* out);
 int getXBeta(ccdh_fit* f, double* out);
+int computeXBetaRows(ccdh_fit* f, const double* beta);
+int mmGradHess(ccdh_fit* f, const uint8_t* fixed, double* gh);
 
 // batched / served engine helpers
 int validateSpec(const ccdh_data* d, const ccdh_fit_spec& sp);

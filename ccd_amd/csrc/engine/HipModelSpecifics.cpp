@@ -36,9 +36,15 @@ void HipModelSpecifics::computeGradientAndHessian(int index, double* ogradient,
   check(ccdh_grad_hess(fit, index, useWeights, ogradient, ohessian));
 }
 
-void HipModelSpecifics::computeMMGradientAndHessian(std::vector<GradientHessian>&,
-                                                    const std::vector<bool>&, bool) {
-  throw std::logic_error("MM algorithm is not implemented by the HIP engine");
+void HipModelSpecifics::computeMMGradientAndHessian(std::vector<GradientHessian>& gh,
+                                                    const std::vector<bool>& fixBeta,
+                                                    bool useWeights) {
+  std::vector<uint8_t> fixed(J);
+  for (int j = 0; j < J; ++j) fixed[j] = fixBeta[j] ? 1 : 0;
+  std::vector<double> flat(2 * (size_t)J);
+  check(ccdh_mm_grad_hess(fit, fixed.data(), useWeights, flat.data()));
+  gh.resize(J);
+  for (int j = 0; j < J; ++j) gh[j] = GradientHessian(flat[2 * j], flat[2 * j + 1]);
 }
 
 void HipModelSpecifics::computeNumeratorForGradient(int index, bool useWeights) {

@@ -7,6 +7,7 @@
 //   applyBounds() ........ :1503-1520
 //   performCheckConvergence() :839-906 with criterion :1498-1501
 //   getNewPredictiveLogLikelihood() :319-355
+//   MM algorithm: cycle :974-1005, check :1026-1048, mmUpdateAllBeta() :1354-1391
 // With HipModelSpecifics underneath, every coordinate is one gradient/Hessian
 // launch, one 16-byte device-to-host copy for the host prior step, and one
 // update launch: the reference's per-coordinate host round trip, kept on purpose
@@ -25,11 +26,16 @@ namespace ccdamd {
 
 int setError(int code, const std::string& msg);
 
+// The MM check's error (CyclicCoordinateDescent.cpp:1036-1042)
+struct NonIncreasing : std::runtime_error {
+  NonIncreasing() : std::runtime_error("Non-increasing!") {}
+};
+
 class HostCyclicCoordinateDescent {
  public:
   HostCyclicCoordinateDescent(AbstractModelSpecifics& ms, int J, bool hasOffset,
-                              const ccdh_fit_spec& spec, int K)
-      : ms(ms), J(J), K(K), hasOffset(hasOffset), spec(spec) {
+                              const ccdh_fit_spec& spec, int K, bool mm = false)
+      : ms(ms), J(J), K(K), hasOffset(hasOffset), mm(mm), spec(spec) {
     beta.assign(spec.beta0, spec.beta0 + J);
     fixBeta.assign(J, false);
     for (int j = 0; j < J; ++j) fixBeta[j] = spec.fixed[j] != 0;
@@ -75,6 +81,7 @@ class HostCyclicCoordinateDescent {
 
   std::vector<double> beta;
   int lastIterationCount = 0, lastReturnFlag = CCDH_SUCCESS, boundCount = 0;
+  int iterationReached = 0;  // the iteration an MM fit stopped at with NonIncreasing
 
  private:
   void computeXBeta() {
@@ -136,9 +143,25 @@ class HostCyclicCoordinateDescent {
     for (int j = 0; j < J; ++j) bound[j] = initialBound;
     int iteration = 0;
     double lastObj = objective();
+    double lastLogPosterior = 0.0;
     bool done = false;
     while (!done) {
-      for (int j = 0; j < J; ++j) {
+      if (mm) {
+        mmUpdateAllBeta();
+        for (int j = 0; j < J; ++j)
+          if (!fixBeta[j] && allDelta[j] != 0.0) beta[j] += allDelta[j];
+        ms.computeXBeta(beta.data(), useCV);
+        ms.computeRemainingStatistics(useCV);
+        sufficientStatisticsKnown = true;
+        iteration++;
+        const double logPosterior = logLikelihood() + logPrior();
+        if (iteration > 1 && logPosterior - lastLogPosterior < 0.0) {
+          iterationReached = iteration;
+          throw NonIncreasing();
+        }
+        lastLogPosterior = logPosterior;
+      }
+      for (int j = 0; !mm && j < J; ++j) {
         if (fixBeta[j]) continue;
         // ccdUpdateBeta
         ms.computeNumeratorForGradient(j, useCV);
@@ -155,7 +178,7 @@ class HostCyclicCoordinateDescent {
           ms.updateXBeta(delta, j, useCV);
         }
       }
-      iteration++;
+      if (!mm) iteration++;
       // performCheckConvergence
       double conv;
       bool ill = false;
@@ -178,9 +201,26 @@ class HostCyclicCoordinateDescent {
     lastIterationCount = iteration;
   }
 
+  // every free coordinate's prior step from one set of statistics; no bounds
+  void mmUpdateAllBeta() {
+    ms.computeMMGradientAndHessian(gh, fixBeta, useCV);
+    allDelta.assign(J, 0.0);
+    for (int j = 0; j < J; ++j) {
+      if (fixBeta[j]) continue;
+      double g = gh[j].first, h = gh[j].second;
+      if (h < 0.0) {
+        g = 0.0;
+        h = 0.0;
+      }
+      allDelta[j] = priorDelta(spec.prior_type[j], spec.prior_var[j], g, h, beta[j]);
+    }
+  }
+
   AbstractModelSpecifics& ms;
   int J, K;
-  bool hasOffset;
+  bool hasOffset, mm;
+  std::vector<GradientHessian> gh;
+  std::vector<double> allDelta;
   const ccdh_fit_spec& spec;
   std::vector<bool> fixBeta;
   std::vector<double> bound, weights;
@@ -192,8 +232,8 @@ class HostCyclicCoordinateDescent {
 
 using namespace ccdamd;
 
-extern "C" int ccdh_fit_plain(ccdh_data_t data, const ccdh_fit_spec* spec, double* beta_out,
-                              ccdh_fit_result* result) {
+static int fitHost(ccdh_data_t data, const ccdh_fit_spec* spec, double* beta_out,
+                   ccdh_fit_result* result, bool mm) {
   if (!data || !spec || !beta_out || !result) return setError(CCDH_E_INVALID, "null argument");
   if (!spec->prior_type || !spec->prior_var || !spec->fixed || !spec->beta0)
     return setError(CCDH_E_INVALID, "fit spec arrays required");
@@ -204,8 +244,14 @@ extern "C" int ccdh_fit_plain(ccdh_data_t data, const ccdh_fit_spec* spec, doubl
     auto shared = std::make_shared<SharedData>();
     shared->handle = data;
     HipModelSpecifics ms(shared, model, K, J);
-    HostCyclicCoordinateDescent ccd(ms, J, hasOffset != 0, *spec, K);
-    ccd.update();
+    HostCyclicCoordinateDescent ccd(ms, J, hasOffset != 0, *spec, K, mm);
+    try {
+      ccd.update();
+    } catch (const NonIncreasing& e) {
+      for (int j = 0; j < J; ++j) beta_out[j] = ccd.beta[j];
+      result->iterations = ccd.iterationReached;
+      return setError(CCDH_E_FIT, e.what());
+    }
     for (int j = 0; j < J; ++j) beta_out[j] = ccd.beta[j];
     result->log_likelihood = ccd.logLikelihood();
     result->log_prior = ccd.logPrior();
@@ -220,4 +266,21 @@ extern "C" int ccdh_fit_plain(ccdh_data_t data, const ccdh_fit_spec* spec, doubl
     return setError(CCDH_E_DEVICE, e.what());
   }
   return CCDH_OK;
+}
+
+extern "C" int ccdh_fit_plain(ccdh_data_t data, const ccdh_fit_spec* spec, double* beta_out,
+                              ccdh_fit_result* result) {
+  return fitHost(data, spec, beta_out, result, false);
+}
+
+extern "C" int ccdh_fit_mm(ccdh_data_t data, const ccdh_fit_spec* spec, double* beta_out,
+                           ccdh_fit_result* result) {
+  if (data) {
+    int model = 0, K = 0, J = 0, hasOffset = 0;
+    ccdh_data_shape(data, &model, &K, &J, &hasOffset);
+    if (model != CCDH_LOGISTIC && model != CCDH_CONDITIONAL_LOGISTIC)
+      return setError(CCDH_E_INVALID,
+                      "MM algorithm: not model-specific (logistic and conditional logistic only)");
+  }
+  return fitHost(data, spec, beta_out, result, true);
 }

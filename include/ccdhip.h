@@ -39,6 +39,7 @@ extern "C" {
 #define CCDH_E_DEVICE (-2)    /* HIP runtime error */
 #define CCDH_E_NOMEM (-3)     /* device allocation failed */
 #define CCDH_E_STATE (-4)     /* call out of order (e.g. gradient before statistics) */
+#define CCDH_E_FIT (-5)       /* the optimizer raised an error (MM: "Non-increasing!") */
 
 /* column formats: FormatType, CompressedDataMatrix.h:36-38 */
 #define CCDH_DENSE 0
@@ -146,10 +147,20 @@ int ccdh_set_weights(ccdh_fit_t fit, const double* weights, int use_cv);
 int ccdh_fixed_terms_loglik(ccdh_fit_t fit, int use_cv);
 /* computeFixedTermsInGradientAndHessian(bool) (:85), i.e. XjY */
 int ccdh_fixed_terms_grad_hess(ccdh_fit_t fit, int use_cv);
-/* zeroXBeta() / axpyXBeta(double, int) / computeXBeta(double*, bool) (:79,120,122) */
+/* zeroXBeta() / axpyXBeta(double, int) (:120,122) */
 int ccdh_zero_xbeta(ccdh_fit_t fit);
 int ccdh_axpy_xbeta(ccdh_fit_t fit, double beta, int j);
+/* computeXBeta(double*, bool) (:79; ModelSpecifics.hpp:313-364): xBeta from the
+ * transpose of X (CompressedDataMatrix.cpp:151-208), each row summed in column
+ * order.  As there, a matrix with INDICATOR columns and no DENSE column gives an
+ * INDICATOR transpose, in which every entry counts 1.  Built on the first call. */
 int ccdh_compute_xbeta(ccdh_fit_t fit, const double* beta);
+/* computeMMGradientAndHessian(std::vector<GradientHessian>&, const std::vector<bool>&,
+ * bool) (:70; ModelSpecifics.hpp:776-889): every column's MM gradient and Hessian
+ * from the current statistics into gh_out[2j], gh_out[2j+1]; (0, 0) where fixed[j].
+ * The increments weigh by nW whatever use_weights says (ModelSpecifics.h:500-517,
+ * 571-584).  Logistic and conditional logistic models only (CCDH_E_INVALID otherwise). */
+int ccdh_mm_grad_hess(ccdh_fit_t fit, const uint8_t* fixed, int use_weights, double* gh_out);
 /* computeRemainingStatistics(bool) (:81) */
 int ccdh_remaining_stats(ccdh_fit_t fit, int use_weights);
 /* computeNumeratorForGradient(int, bool) (:72) */
@@ -171,6 +182,14 @@ int ccdh_get_xbeta(ccdh_fit_t fit, double* out);
 /* Plain engine: the host CCD (findMode) calling the per-coordinate ops. */
 int ccdh_fit_plain(ccdh_data_t data, const ccdh_fit_spec* spec, double* beta_out,
                    ccdh_fit_result* result);
+/* The same driver with the MM algorithm (createControl(algorithm = "mm"), R/ModelFit.R:513-524;
+ * CyclicCoordinateDescent.cpp:958-1005, 1026-1048, 1354-1391): each iteration steps every
+ * free coordinate from one set of statistics (no bounding box), then recomputes xBeta
+ * from the transpose and the statistics.  A decrease of the log posterior stops the fit
+ * with CCDH_E_FIT ("Non-increasing!"); beta_out then holds the coefficients reached and
+ * result->iterations the iteration.  Logistic and conditional logistic models only. */
+int ccdh_fit_mm(ccdh_data_t data, const ccdh_fit_spec* spec, double* beta_out,
+                ccdh_fit_result* result);
 
 /* Batched engine options; zero fields pick the defaults. */
 typedef struct {

@@ -28,6 +28,11 @@
 //   model increments ................ engine/ModelSpecifics.h:540-568 (LR), 820-837 (CLR),
 //                                     1185-1202 (Breslow Cox), 1508-1543 (Poisson)
 //   priors .......................... priors/CovariatePrior.h:186-188, 221-292, 404-415
+//   MM algorithm .................... CyclicCoordinateDescent.cpp:958-1005 (cycle), 1026-1048
+//                                     (check), 1354-1391 (mmUpdateAllBeta);
+//                                     engine/ModelSpecifics.hpp:776-889 (MM gradient/Hessian),
+//                                     313-364 (computeXBeta over the transpose);
+//                                     CompressedDataMatrix.cpp:151-208 (transpose)
 //   CV fold weights ................. drivers/CrossValidationSelector.cpp:21-116, 138-143
 //   grid points ..................... drivers/GridSearchCrossValidationDriver.cpp:38-48
 //
@@ -450,6 +455,62 @@ struct Engine {
     return c;
   }
 
+  // Row-major copy of X as CompressedDataMatrix::transpose builds it
+  // (CompressedDataMatrix.cpp:151-208; add_data CompressedDataMatrix.h:159-188).
+  // Every row gets one format: INDICATOR when the matrix has INDICATOR columns
+  // and no DENSE one, so an entry keeps only its position (a SPARSE value counts
+  // as 1); otherwise the values themselves.  Zero values are dropped (SPARSE /
+  // INDICATOR add_data); a DENSE row keeps them, which adds exact zeros to the
+  // row sums below and so changes nothing for finite coefficients.
+  std::vector<std::vector<std::pair<int, double>>> xt;
+  void buildTranspose() {
+    bool anyDense = false, anyIndicator = false;
+    for (int j = 0; j < J; ++j) {
+      anyDense = anyDense || d.fmt[j] == DENSE;
+      anyIndicator = anyIndicator || d.fmt[j] == INDICATOR;
+    }
+    const bool positionsOnly = anyIndicator && !anyDense;
+    xt.assign(K, {});
+    for (int j = 0; j < J; ++j) {
+      forColumn(j, [&](int k, double x) {
+        if (x == 0.0) return;
+        xt[k].push_back({j, positionsOnly ? 1.0 : x});
+      });
+    }
+  }
+
+  // ModelSpecifics::computeXBeta(beta) (ModelSpecifics.hpp:313-364): per row, the
+  // sum over its entries in column order.
+  void mmComputeXBeta(const double* beta) {
+    if (xt.empty()) buildTranspose();
+    for (int k = 0; k < K; ++k) {
+      double sum = 0.0;
+      for (const auto& e : xt[k]) sum += e.second * beta[e.first];
+      xb[k] = sum;
+    }
+  }
+
+  // computeMMGradientAndHessianImpl (ModelSpecifics.hpp:848-889) with the
+  // increments of ModelSpecifics.h:500-517 (conditional models) and 571-584
+  // (logistic): weight nW and denominator of the row's group, no 1-t factor.
+  void mmGradHess(int j, double* og, double* oh) const {
+    const bool ind = isIndicatorFmt(d.fmt[j]);
+    double g = 0.0, h = 0.0;
+    forColumn(j, [&](int k, double x) {
+      const int i = group(k);
+      if (ind) {
+        g += nw[i] * oeb[k] / denom[i];
+        h += nw[i] * oeb[k] / denom[i];
+      } else {
+        g += nw[i] * oeb[k] * x / denom[i];
+        h += nw[i] * oeb[k] * x * x / denom[i];
+      }
+    });
+    g -= xjy[j];
+    *og = g;
+    *oh = h;
+  }
+
   // ModelSpecifics.hpp:652-690 (the CCD recomputes xb and statistics first)
   double predictiveLogLikelihood(const double* w) {
     std::vector<double> saved;
@@ -544,6 +605,7 @@ typedef struct {
   double initial_bound;
   int max_bound_count;
   int has_offset;  // column 0 is an offset covariate (kept by resetBeta)
+  int algorithm;   // 0: cyclic coordinate descent, 1: MM (AlgorithmType, Types.h)
 } orc_control;
 
 typedef struct {
@@ -561,6 +623,10 @@ typedef struct {
 //   weights   nullable; non-null means useCrossValidation=true
 //   heldout   nullable; when non-null, the predictive log likelihood of these weights
 //             is evaluated after the fit (CV scoring)
+// Returns 0; with the MM algorithm -1 for a model without MM increments (Poisson:
+// ModelSpecifics.h:1546-1552 throws; Cox: not offered, its MM increments divide by
+// tie-group sums rather than risk-set sums) and -2 when the log posterior
+// decreased ("Non-increasing!", CyclicCoordinateDescent.cpp:1033-1046).
 int orc_fit(const orc_data* od, const int* prior_type, const double* prior_var,
             const uint8_t* fixed, double* beta, const double* weights, const double* heldout,
             const orc_control* ctl, orc_result* out) {
@@ -570,6 +636,9 @@ int orc_fit(const orc_data* od, const int* prior_type, const double* prior_var,
   Engine e(d);
   const bool useCV = weights != nullptr;
   const bool hasOffset = ctl->has_offset != 0;  // CyclicCoordinateDescent.cpp:256-264
+  const bool mm = ctl->algorithm == 1;
+  if (mm && d.model != M_LOGISTIC && d.model != M_CLR) return -1;
+  std::vector<double> allDelta(J, 0.0);
 
   std::vector<double> b(beta, beta + J), bound(J, ctl->initial_bound);
   double initialBound = ctl->initial_bound;
@@ -619,9 +688,36 @@ int orc_fit(const orc_data* od, const int* prior_type, const double* prior_var,
     std::fill(bound.begin(), bound.end(), initialBound);
     int iteration = 0;
     double lastObj = objective();
+    double lastLogPosterior = 0.0;
     bool fmDone = false;
     while (!fmDone) {
-      for (int j = 0; j < J; ++j) {
+      if (mm) {
+        // mmUpdateAllBeta: every free coordinate's step from one set of statistics
+        for (int j = 0; j < J; ++j) {
+          allDelta[j] = 0.0;
+          if (fixed[j]) continue;
+          double g, h;
+          e.mmGradHess(j, &g, &h);
+          if (h < 0.0) {
+            g = 0.0;
+            h = 0.0;
+          }
+          allDelta[j] = priorDelta(prior_type[j], prior_var[j], g, h, b[j]);
+        }
+        for (int j = 0; j < J; ++j)
+          if (!fixed[j] && allDelta[j] != 0.0) b[j] += allDelta[j];
+        e.mmComputeXBeta(b.data());
+        e.computeRemainingStatistics(true);
+        iteration++;
+        const double logPosterior = e.logLikelihood(useCV) + logPrior();
+        if (iteration > 1 && logPosterior - lastLogPosterior < 0.0) {
+          for (int j = 0; j < J; ++j) beta[j] = b[j];
+          out->iterations = iteration;
+          return -2;
+        }
+        lastLogPosterior = logPosterior;
+      }
+      for (int j = 0; !mm && j < J; ++j) {
         if (fixed[j]) continue;
         e.computeNumeratorForGradient(j, useCV);
         double g, h;
@@ -644,7 +740,7 @@ int orc_fit(const orc_data* od, const int* prior_type, const double* prior_var,
           e.updateXBeta(delta, j, useCV);
         }
       }
-      iteration++;
+      if (!mm) iteration++;
       // performCheckConvergence
       double conv;
       bool ill = false;
@@ -731,6 +827,34 @@ int orc_eval_cols(const orc_data* od, const double* beta, const double* weights,
     e.computeNumeratorForGradient(cols[i], useCV);
     e.gradHess(cols[i], useCV, grad + i, hess + i);
   }
+  return 0;
+}
+
+// MM gradient and Hessian of every column at a given beta (fresh statistics;
+// fixed columns included).
+int orc_mm_eval(const orc_data* od, const double* beta, const double* weights, double* g,
+                double* h) {
+  Data d{od->K, od->J, od->col_ptr, od->rows, od->vals, od->fmt, od->y, od->offs, od->strata,
+         od->model};
+  if (d.model != M_LOGISTIC && d.model != M_CLR) return -1;
+  Engine e(d);
+  const bool useCV = weights != nullptr;
+  e.setWeights(weights, useCV);
+  e.computeXjY(useCV);
+  e.zeroXBeta();
+  for (int j = 0; j < d.J; ++j) e.axpyXBeta(beta[j], j);
+  e.computeRemainingStatistics(true);
+  for (int j = 0; j < d.J; ++j) e.mmGradHess(j, &g[j], &h[j]);
+  return 0;
+}
+
+// xBeta as ModelSpecifics::computeXBeta(beta) forms it from the transpose.
+int orc_mm_xbeta(const orc_data* od, const double* beta, double* xb_out) {
+  Data d{od->K, od->J, od->col_ptr, od->rows, od->vals, od->fmt, od->y, od->offs, od->strata,
+         od->model};
+  Engine e(d);
+  e.mmComputeXBeta(beta);
+  for (int k = 0; k < d.K; ++k) xb_out[k] = e.xb[k];
   return 0;
 }
 

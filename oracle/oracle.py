@@ -28,6 +28,7 @@ class OrcControl(ctypes.Structure):
         ("tolerance", ctypes.c_double), ("max_iterations", ctypes.c_int),
         ("convergence_type", ctypes.c_int), ("initial_bound", ctypes.c_double),
         ("max_bound_count", ctypes.c_int), ("has_offset", ctypes.c_int),
+        ("algorithm", ctypes.c_int),
     ]
 
 
@@ -64,6 +65,10 @@ def lib():
         _lib.orc_grid_point.restype = ctypes.c_double
         _lib.orc_grid_point.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                         ctypes.c_int]
+        _lib.orc_mm_eval.restype = ctypes.c_int
+        _lib.orc_mm_eval.argtypes = [vp] * 5
+        _lib.orc_mm_xbeta.restype = ctypes.c_int
+        _lib.orc_mm_xbeta.argtypes = [vp] * 3
         _lib.orc_cox_pids.restype = ctypes.c_int
         _lib.orc_cox_pids.argtypes = [vp] * 5
         _lib.orc_search_step.restype = ctypes.c_int
@@ -101,9 +106,10 @@ def _data_struct(p, keep):
                    int(p.model))
 
 
-def fit(p, spec, heldout=None):
-    """Run the oracle fit.  `p` is a ccd_amd.data.CcdProblem, `spec` a FitSpec.
-    Returns (beta, dict of diagnostics)."""
+def fit(p, spec, heldout=None, algorithm="ccd"):
+    """Run the oracle fit.  `p` is a ccd_amd.data.CcdProblem, `spec` a FitSpec,
+    `algorithm` "ccd" or "mm".  Returns (beta, dict of diagnostics); the dict's
+    "status" is 0, -1 (no MM for this model) or -2 (MM log posterior decreased)."""
     keep = []
     d = _data_struct(p, keep)
     pt = np.ascontiguousarray(spec.prior_type, dtype=np.int32)
@@ -113,13 +119,15 @@ def fit(p, spec, heldout=None):
     w = None if spec.weights is None else np.ascontiguousarray(spec.weights, dtype=np.float64)
     ho = None if heldout is None else np.ascontiguousarray(heldout, dtype=np.float64)
     ctl = OrcControl(spec.tolerance, spec.max_iterations, spec.convergence_type,
-                     spec.initial_bound, spec.max_bound_count, int(p.has_offset))
+                     spec.initial_bound, spec.max_bound_count, int(p.has_offset),
+                     {"ccd": 0, "mm": 1}[algorithm])
     res = OrcResult()
-    lib().orc_fit(ctypes.byref(d), _ptr(pt), _ptr(pv), _ptr(fx), _ptr(beta), _ptr(w), _ptr(ho),
+    status = lib().orc_fit(ctypes.byref(d), _ptr(pt), _ptr(pv), _ptr(fx), _ptr(beta), _ptr(w), _ptr(ho),
                   ctypes.byref(ctl), ctypes.byref(res))
     return beta, dict(log_likelihood=res.log_likelihood, log_prior=res.log_prior,
                       iterations=res.iterations, return_flag=res.return_flag,
-                      bound_count=res.bound_count, pred_log_likelihood=res.pred_log_likelihood)
+                      bound_count=res.bound_count, pred_log_likelihood=res.pred_log_likelihood,
+                      status=status)
 
 
 def evaluate(p, beta, j=-1, weights=None):
@@ -148,6 +156,30 @@ def evaluate_cols(p, beta, cols, weights=None):
     lib().orc_eval_cols(ctypes.byref(d), _ptr(b), _ptr(w), _ptr(c), len(c), ctypes.byref(ll),
                         _ptr(g), _ptr(h))
     return ll.value, g, h
+
+
+def mm_evaluate(p, beta, weights=None):
+    """MM gradients and Hessians of every column at `beta` (fresh statistics)."""
+    keep = []
+    d = _data_struct(p, keep)
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    g = np.zeros(p.J)
+    h = np.zeros(p.J)
+    rc = lib().orc_mm_eval(ctypes.byref(d), _ptr(b), _ptr(w), _ptr(g), _ptr(h))
+    if rc != 0:
+        raise ValueError("no MM increments for this model")
+    return g, h
+
+
+def mm_xbeta(p, beta):
+    """xBeta formed from the transpose, as the MM step forms it."""
+    keep = []
+    d = _data_struct(p, keep)
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    out = np.zeros(p.K)
+    lib().orc_mm_xbeta(ctypes.byref(d), _ptr(b), _ptr(out))
+    return out
 
 
 def prior_delta(ptype, var, g, h, beta):
