@@ -20,10 +20,15 @@ log-uniform, 20-60 nonzeros per row, each request with its own prior variance)
 served by the continuous-batching server; fits/s over the stream, p50/p99
 submit-to-result latency, and plain sequential GPU fits of sampled requests.
 
+MM leg ("mm"): one of the CV fits (variance grid point 3, fold 0) with the MM
+algorithm (every coordinate stepped at once from the MM gradient), run to its
+stopping rule on the GPU, and the oracle's first MM iterations for the CPU rate.
+
   python bench.py [--steps 1] [--warmup 0] [--scale 1.0] [--plain-fits 2] [--cpu-fits 1]
                   [--serve-requests 200] [--serve-rate 50]
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -158,6 +163,39 @@ def serve_leg(args, api, synth):
     }
 
 
+def mm_leg(args, api, prob, data, spec, beta_ccd):
+    """The MM algorithm on one CV fit of the workload: the GPU fit to its stopping
+    rule, and the oracle's MM iterations on a bounded sample for the CPU rate."""
+    t = time.perf_counter()
+    status = "converged"
+    try:
+        bm, im = api.fit_plain(data, spec, algorithm="mm")
+        iters, flag = im["iterations"], im["return_flag"]
+    except api.NonIncreasing as e:
+        bm, iters, flag, status = e.beta, e.iterations, None, "non-increasing"
+    gpu_s = time.perf_counter() - t
+    scale = max(1.0, float(np.max(np.abs(beta_ccd))))
+    out = {"workload": "MM algorithm, one CV fit of the conditional logistic workload "
+                       f"(fold 0, Laplace var {spec.prior_var[-1]:.3g})",
+           "status": status, "iterations": iters, "return_flag": flag, "seconds": gpu_s,
+           "ms_per_iteration": gpu_s / max(1, iters) * 1e3,
+           "max_rel_coef_diff_vs_ccd": float(np.max(np.abs(bm - beta_ccd)) / scale)}
+    log(f"mm: {status} after {iters} iterations in {gpu_s:.2f}s "
+        f"({out['ms_per_iteration']:.2f} ms/iteration), vs CCD {out['max_rel_coef_diff_vs_ccd']:.2e}")
+    if args.mm_cpu_iters > 0:
+        from oracle import oracle as orc
+        sample = dataclasses.replace(spec, max_iterations=args.mm_cpu_iters)
+        t = time.perf_counter()
+        orc.fit(prob, sample, algorithm="mm")
+        cpu_s = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": args.mm_cpu_iters / cpu_s, "unit": "iterations/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"the first {args.mm_cpu_iters} MM iterations of the same "
+                                         "fit, single-threaded FP64 oracle (transpose build included)"}
+        log(f"mm cpu oracle: {args.mm_cpu_iters} iterations in {cpu_s:.2f}s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +227,9 @@ def main():
     ap.add_argument("--pass-items", type=int, default=0, help="long-column pass width without tuning")
     ap.add_argument("--max-gang", type=int, default=0, help="workgroups per fit at most (0: 64)")
     ap.add_argument("--bundle", type=int, default=0, help="fits per slot in lock step (0: default)")
+    ap.add_argument("--mm", type=int, default=1, help="MM leg on one CV fit (0: skip)")
+    ap.add_argument("--mm-cpu-iters", type=int, default=3,
+                    help="MM leg: oracle iterations timed for the CPU rate (0: none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -324,6 +365,11 @@ def main():
                "sample": f"{len(cpu_t)} of the {n_fits} CV fits (indices {sample[:args.cpu_fits].tolist()}), "
                          "single-threaded FP64 oracle"}
 
+    mm = None
+    if args.mm and precision == api.FP64 and not args.limit:
+        idx = 3 * folds  # variance grid point 3, fold 0
+        mm = mm_leg(args, api, prob, data, specs[idx], betas[idx])
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -354,6 +400,7 @@ def main():
         "cpu_baseline": cpu,
         "tuning": tuning,
         "serving": serving,
+        "mm": mm,
     }
     print(json.dumps(out), flush=True)
 

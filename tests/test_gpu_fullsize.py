@@ -9,7 +9,9 @@
           32 spread over the rest; the others fixed at 0) fitted to convergence, since
           each coordinate costs an O(N) risk-set scan on the CPU
   clr     200,000 strata (~1.3e6 rows) x 20,000 columns, 10-fold BY_PID CV weights,
-          Laplace var 1e-4 and 1e-3 — two of the 160 CV fits, with held-out scores
+          Laplace var 1e-4 and 1e-3 — two of the 160 CV fits, with held-out scores;
+          and the MM algorithm on one of them, its first five iterations (the
+          oracle takes ~1.5 s per MM iteration here)
 
 FP64: coefficients and log likelihood within 1e-8 relative of the oracle (atol
 1e-10), same iterations and return flag; per-coordinate gradient and Hessian at the
@@ -50,6 +52,12 @@ def cox_full():
     sp.fixed[free] = 0
     bc, ic = orc.fit(p, sp)
     return p, sp, bc, ic
+
+
+@pytest.fixture(scope="module")
+def clr_full():
+    p, _ = synth.clr_strata(seed=SEED + 4)
+    return p
 
 
 def _probe_columns(p, beta):
@@ -147,8 +155,8 @@ def test_full_cox_mixed_precision(cox_full):
     d.close()
 
 
-def test_full_clr_cv_batched():
-    p, _ = synth.clr_strata(seed=SEED + 4)
+def test_full_clr_cv_batched(clr_full):
+    p = clr_full
     assert p.K > 1_200_000
     specs, held = [], []
     for fold, step in ((0, 0), (1, 3)):
@@ -163,4 +171,17 @@ def test_full_clr_cv_batched():
         check_fit(bb[i], ib[i], bc, ic)
         np.testing.assert_allclose(ib[i]["pred_log_likelihood"], ic["pred_log_likelihood"],
                                    rtol=1e-8)
+    d.close()
+
+
+def test_full_clr_mm_trajectory(clr_full):
+    p = clr_full
+    w = orc.cv_weights(p.strata, 10, 123, 0, by_pid=True)
+    var = orc.grid_point(1e-4, 1.0, 13, 3)
+    sp = default_spec(p, prior=PRIOR_LAPLACE, variance=var, weights=w, max_iterations=5)
+    bc, ic = orc.fit(p, sp, algorithm="mm")
+    assert ic["status"] == 0
+    d = api.Data(p)
+    bg, ig = api.fit_plain(d, sp, algorithm="mm")
+    check_fit(bg, ig, bc, ic)
     d.close()
