@@ -601,16 +601,34 @@ __global__ void k_t_sort(int K, const int64_t* ptr, int32_t* tCol, double* tVal)
 }
 
 // ModelSpecifics::computeXBetaImpl (ModelSpecifics.hpp:346-364): each row's sum
-// over its entries in column order
+// over its entries in column order.  A workgroup takes kThreads consecutive rows,
+// whose entries are contiguous; it stages them through LDS in tiles with
+// coalesced loads, and each thread adds its own row's part of every tile in
+// order, so a row is still summed entry by entry in column order.
+constexpr int kXbTile = 2048;
 template <typename R>
-__global__ void k_xbeta_rows(int K, const int64_t* ptr, const int32_t* tCol, const double* tVal,
-                             const double* beta, R* xb) {
-  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < K;
-       k += (int64_t)gridDim.x * blockDim.x) {
+__global__ __launch_bounds__(kThreads) void k_xbeta_rows(int K, const int64_t* ptr,
+                                                         const int32_t* tCol, const double* tVal,
+                                                         const double* beta, R* xb) {
+  __shared__ double sv[kXbTile], sb[kXbTile];
+  for (int64_t k0 = (int64_t)blockIdx.x * blockDim.x; k0 < K;
+       k0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = k0 + threadIdx.x;
+    const int64_t e0 = ptr[k0], e1 = ptr[std::min<int64_t>(K, k0 + blockDim.x)];
+    const int64_t rb = k < K ? ptr[k] : e1, re = k < K ? ptr[k + 1] : e1;
     double sum = 0.0;
-    for (int64_t q = ptr[k]; q < ptr[k + 1]; ++q)
-      sum += (tVal ? tVal[q] : 1.0) * beta[tCol[q]];
-    xb[k] = (R)sum;
+    for (int64_t t0 = e0; t0 < e1; t0 += kXbTile) {
+      const int64_t t1 = std::min<int64_t>(e1, t0 + kXbTile);
+      __syncthreads();  // the previous tile is consumed
+      for (int64_t q = t0 + threadIdx.x; q < t1; q += blockDim.x) {
+        sv[q - t0] = tVal ? tVal[q] : 1.0;
+        sb[q - t0] = beta[tCol[q]];
+      }
+      __syncthreads();
+      const int64_t a = std::max(rb, t0), b = std::min(re, t1);
+      for (int64_t q = a; q < b; ++q) sum += sv[q - t0] * sb[q - t0];
+    }
+    if (k < K) xb[k] = (R)sum;
   }
 }
 
@@ -668,20 +686,27 @@ __global__ void k_mm_chunks(DevX d, const double* r, const int32_t* chunk0,
   }
 }
 
-// per column: chunk sums in chunk order, minus XjY; fixed columns (0, 0)
+// one wave per column: its chunk sums (lane-strided, then a fixed wave tree),
+// minus XjY; fixed columns (0, 0)
 __global__ void k_mm_finish(int J, const int32_t* chunk0, const double* part, const double* xjy,
                             const uint8_t* fixed, double* gh) {
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < J; j += gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int waves = blockDim.x / kWave;
+  for (int j = blockIdx.x * waves + threadIdx.x / kWave; j < J; j += gridDim.x * waves) {
+    const bool fx = fixed[j] != 0;
     double g = 0.0, h = 0.0;
-    if (!fixed[j]) {
-      for (int c = chunk0[j]; c < chunk0[j + 1]; ++c) {
+    if (!fx) {
+      for (int c = chunk0[j] + lane; c < chunk0[j + 1]; c += kWave) {
         g += part[2 * c];
         h += part[2 * c + 1];
       }
-      g -= xjy[j];
     }
-    gh[2 * j] = g;
-    gh[2 * j + 1] = h;
+    g = waveSum(g);
+    h = waveSum(h);
+    if (lane == 0) {
+      gh[2 * j] = fx ? 0.0 : g - xjy[j];
+      gh[2 * j + 1] = fx ? 0.0 : h;
+    }
   }
 }
 
@@ -1614,8 +1639,9 @@ int mmGradHess(ccdh_fit* f, const uint8_t* fixed, double* gh) {
         d->view(), f->mmR, d->mmChunk0, d->mmChunkCol, d->nChunks, f->mmFixed, f->mmPart);
     HIP_TRY(hipGetLastError());
   }
-  k_mm_finish<<<gridFor(d->J, 1), kThreads, 0, f->stream>>>(d->J, d->mmChunk0, f->mmPart, f->xjy,
-                                                            f->mmFixed, f->mmGH);
+  const int finishBlocks = std::min((d->J + wavesPerBlock - 1) / wavesPerBlock, 16384);
+  k_mm_finish<<<finishBlocks, kThreads, 0, f->stream>>>(d->J, d->mmChunk0, f->mmPart, f->xjy,
+                                                        f->mmFixed, f->mmGH);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(gh, f->mmGH, 2 * (size_t)d->J * sizeof(double), hipMemcpyDeviceToHost,
                          f->stream));
