@@ -18,7 +18,8 @@ FP64, MIXED = 0, 1
 
 # every symbol include/ccdhip.h declares
 EXPORTS = [
-    "ccdh_last_error", "ccdh_device_info", "ccdh_data_create", "ccdh_data_destroy",
+    "ccdh_last_error", "ccdh_device_info", "ccdh_data_create", "ccdh_data_create_long",
+    "ccdh_data_get_csc", "ccdh_data_destroy",
     "ccdh_data_device_bytes", "ccdh_data_shape", "ccdh_fit_create", "ccdh_fit_clone",
     "ccdh_fit_destroy", "ccdh_set_weights", "ccdh_fixed_terms_loglik",
     "ccdh_fixed_terms_grad_hess", "ccdh_zero_xbeta", "ccdh_axpy_xbeta", "ccdh_compute_xbeta",
@@ -53,6 +54,16 @@ class DataDesc(ctypes.Structure):
         ("J", ctypes.c_int), ("col_ptr", ctypes.c_void_p), ("rows", ctypes.c_void_p),
         ("vals", ctypes.c_void_p), ("fmt", ctypes.c_void_p), ("y", ctypes.c_void_p),
         ("offs", ctypes.c_void_p), ("strata", ctypes.c_void_p), ("has_offset", ctypes.c_int),
+    ]
+
+
+class LongDesc(ctypes.Structure):
+    _fields_ = [
+        ("model", ctypes.c_int), ("precision", ctypes.c_int), ("n_rows", ctypes.c_int64),
+        ("row_id", ctypes.c_void_p), ("stratum_id", ctypes.c_void_p), ("y", ctypes.c_void_p),
+        ("time", ctypes.c_void_p), ("n_entries", ctypes.c_int64),
+        ("covariate_id", ctypes.c_void_p), ("entry_row_id", ctypes.c_void_p),
+        ("value", ctypes.c_void_p), ("add_intercept", ctypes.c_int),
     ]
 
 
@@ -151,6 +162,8 @@ def load_library(path=LIB_PATH):
         "ccdh_last_error": (ctypes.c_char_p, []),
         "ccdh_device_info": (ci, [vp, ctypes.c_size_t, vp, vp]),
         "ccdh_data_create": (ci, [vp, vp]),
+        "ccdh_data_create_long": (ci, [vp, vp]),
+        "ccdh_data_get_csc": (ci, [vp, vp, vp, vp, vp, vp, vp, vp]),
         "ccdh_data_destroy": (ci, [vp]),
         "ccdh_data_device_bytes": (ctypes.c_int64, [vp]),
         "ccdh_data_shape": (ci, [vp, vp, vp, vp, vp]),
@@ -246,6 +259,49 @@ class Data:
         _check(lib.ccdh_data_create(ctypes.byref(desc), ctypes.byref(h)))
         self.handle = h
         self._arrs = None  # the library copied everything
+
+    @classmethod
+    def from_long(cls, model, row_id, y, covariate_id, entry_row_id, value=None, stratum_id=None,
+                  time=None, add_intercept=True, precision=FP64):
+        """A data set from long-format outcomes and covariate entries, sorted and
+        mapped on the device (ccdh_data_create_long)."""
+        lib = load_library()
+        a = dict(row_id=np.ascontiguousarray(row_id, np.int64),
+                 y=np.ascontiguousarray(y, np.float64),
+                 cov=np.ascontiguousarray(covariate_id, np.int64),
+                 erow=np.ascontiguousarray(entry_row_id, np.int64),
+                 val=None if value is None else np.ascontiguousarray(value, np.float64),
+                 strata=None if stratum_id is None else np.ascontiguousarray(stratum_id, np.int64),
+                 time=None if time is None else np.ascontiguousarray(time, np.float64))
+        desc = LongDesc(model, precision, len(a["row_id"]), _p(a["row_id"]), _p(a["strata"]),
+                        _p(a["y"]), _p(a["time"]), len(a["cov"]), _p(a["cov"]), _p(a["erow"]),
+                        _p(a["val"]), int(add_intercept))
+        h = ctypes.c_void_p()
+        _check(lib.ccdh_data_create_long(ctypes.byref(desc), ctypes.byref(h)))
+        self = cls.__new__(cls)
+        self.handle = h
+        self.precision = precision
+        self._arrs = None
+        self.problem = self.to_problem()
+        return self
+
+    def to_problem(self):
+        """The resident data set's compressed columns as a CcdProblem."""
+        lib = load_library()
+        m, K, J, off = (ctypes.c_int() for _ in range(4))
+        _check(lib.ccdh_data_shape(self.handle, ctypes.byref(m), ctypes.byref(K), ctypes.byref(J),
+                                   ctypes.byref(off)))
+        col_ptr = np.zeros(J.value + 1, np.int64)
+        _check(lib.ccdh_data_get_csc(self.handle, _p(col_ptr), None, None, None, None, None, None))
+        nnz = int(col_ptr[-1])
+        rows, vals = np.zeros(nnz, np.int32), np.zeros(nnz)
+        fmt = np.zeros(J.value, np.int8)
+        y, offs, strata = np.zeros(K.value), np.zeros(K.value), np.zeros(K.value, np.int32)
+        _check(lib.ccdh_data_get_csc(self.handle, None, _p(rows), _p(vals), _p(fmt), _p(y),
+                                     _p(offs), _p(strata)))
+        return CcdProblem(model=m.value, K=K.value, J=J.value, col_ptr=col_ptr, rows=rows,
+                          vals=vals, fmt=fmt, y=y, offs=offs, strata=strata,
+                          has_offset=bool(off.value))
 
     @property
     def device_bytes(self):

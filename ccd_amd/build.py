@@ -15,6 +15,7 @@ BUILD_DIR = os.path.join(ROOT, "_build")
 SOURCES = [
     "ccd_engine.hip",
     "ccd_batch.hip",
+    "ccd_ingest.hip",
     "ccd_cv.cpp",
     "ccd_serve.cpp",
     "engine/HipModelSpecifics.cpp",

@@ -1732,6 +1732,27 @@ int ccdh_fixed_terms_grad_hess(ccdh_fit_t fit, int use_cv) {
   return fixedTermsGradHess(fit, use_cv);
 }
 
+int ccdh_data_get_csc(ccdh_data_t data, int64_t* col_ptr, int32_t* rows, double* vals,
+                      int8_t* fmt, double* y, double* offs, int32_t* strata) {
+  if (!data) return setError(CCDH_E_INVALID, "null handle");
+  const int K = data->K, J = data->J;
+  const int64_t nnz = data->hColPtr[J];
+  if (col_ptr) std::copy(data->hColPtr.begin(), data->hColPtr.end(), col_ptr);
+  if (fmt) std::copy(data->hFmt.begin(), data->hFmt.end(), fmt);
+  if (y) std::copy(data->hY.begin(), data->hY.end(), y);
+  if (offs) std::copy(data->hOffs.begin(), data->hOffs.end(), offs);
+  if (strata) std::copy(data->hStrata.begin(), data->hStrata.end(), strata);
+  if (rows && nnz > 0) HIP_TRY(hipMemcpy(rows, data->rows, nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (vals && nnz > 0) {
+    if (data->vals)
+      HIP_TRY(hipMemcpy(vals, data->vals, nnz * sizeof(double), hipMemcpyDeviceToHost));
+    else
+      std::fill(vals, vals + nnz, 1.0);
+  }
+  (void)K;
+  return 0;
+}
+
 int ccdh_zero_xbeta(ccdh_fit_t fit) {
   GUARD(fit);
   return zeroXBeta(fit);

@@ -135,6 +135,40 @@ int64_t ccdh_data_device_bytes(ccdh_data_t data);
 /* model, rows, columns and offset flag of a data handle */
 int ccdh_data_shape(ccdh_data_t data, int* model, int* K, int* J, int* has_offset);
 
+/* Long-format input, as convertToCyclopsData takes it (R/NewDataConversion.R:95-235):
+ * one outcome row per row id, covariate entries (covariate id, row id, value).  The
+ * library sorts the rows (LR, Poisson: row id; CLR: stratum, row id; Cox: stratum,
+ * time descending, y, row id), numbers the strata in that order (loadY,
+ * ModelData.cpp:69-146), maps row ids to rows and covariate ids (ascending) to
+ * columns, leaves out entries whose row id is not an outcome row and zero values,
+ * refuses a repeated (row, covariate) entry, and makes a column SPARSE when a value
+ * other than 0 and 1 occurs, INDICATOR otherwise (loadMultipleX, ModelData.cpp:148-243).
+ * LR and Poisson get an intercept column first when add_intercept is set; Poisson
+ * gets log(time) as the offset column before it (finalizeSqlCyclopsData with
+ * useOffsetCovariate = -1; RcppModelData.cpp:482-556).  The sorts and maps run on
+ * the device. */
+typedef struct {
+  int model;                    /* CCDH_POISSON ... CCDH_COX */
+  int precision;                /* CCDH_FP64 or CCDH_MIXED */
+  int64_t n_rows;               /* outcome rows */
+  const int64_t* row_id;        /* [n_rows] distinct */
+  const int64_t* stratum_id;    /* [n_rows]; required for CLR, optional for Cox, else ignored */
+  const double* y;              /* [n_rows] */
+  const double* time;           /* [n_rows]: Cox event time, Poisson exposure (> 0); else NULL */
+  int64_t n_entries;            /* covariate entries */
+  const int64_t* covariate_id;  /* [n_entries] */
+  const int64_t* entry_row_id;  /* [n_entries] */
+  const double* value;          /* [n_entries] or NULL: every entry 1 */
+  int add_intercept;            /* LR / Poisson */
+} ccdh_long_desc;
+
+/* convertToCyclopsData + loadY / loadMultipleX / finalizeSqlCyclopsData (above). */
+int ccdh_data_create_long(const ccdh_long_desc* desc, ccdh_data_t* out);
+/* Copies a data set's compressed columns back: col_ptr [J+1], rows and vals [nnz]
+ * (vals may be NULL), fmt [J], y, offs and strata [K]; any output may be NULL. */
+int ccdh_data_get_csc(ccdh_data_t data, int64_t* col_ptr, int32_t* rows, double* vals,
+                      int8_t* fmt, double* y, double* offs, int32_t* strata);
+
 /* ---- per-coordinate engine (AbstractModelSpecifics virtuals) ---- */
 /* ModelSpecifics ctor + initialize(N,K,J,...) (ModelSpecifics.hpp:1899-1952) */
 int ccdh_fit_create(ccdh_data_t data, ccdh_fit_t* out);
