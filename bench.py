@@ -24,6 +24,11 @@ MM leg ("mm"): one of the CV fits (variance grid point 3, fold 0) with the MM
 algorithm (every coordinate stepped at once from the MM gradient), run to its
 stopping rule on the GPU, and the oracle's first MM iterations for the CPU rate.
 
+Ingestion leg ("ingest"): the CV data set written out in long format (outcome
+rows and covariate entries shuffled, random ids) and ingested on the device
+(ccdh_data_create_long), checked identical to the source; the oracle ingests the
+first quarter of the strata for the CPU rate.
+
   python bench.py [--steps 1] [--warmup 0] [--scale 1.0] [--plain-fits 2] [--cpu-fits 1]
                   [--serve-requests 200] [--serve-rate 50]
 """
@@ -196,6 +201,48 @@ def mm_leg(args, api, prob, data, spec, beta_ccd):
     return out
 
 
+def ingest_leg(args, api, synth, prob):
+    """The CV data set written out in long format (rows and entries shuffled) and
+    ingested on the device; the oracle ingests a bounded sample for the CPU rate."""
+    long = synth.to_long(prob)
+    m = len(long["covariate_id"])
+    t = time.perf_counter()
+    d = api.Data.from_long(**long)
+    gpu_s = time.perf_counter() - t
+    q = d.problem
+    d.close()
+    identical = bool(np.array_equal(q.col_ptr, prob.col_ptr) and np.array_equal(q.rows, prob.rows)
+                     and np.array_equal(q.y, prob.y) and np.array_equal(q.strata, prob.strata))
+    t = time.perf_counter()
+    d = api.Data(prob)
+    csc_s = time.perf_counter() - t
+    d.close()
+    out = {"workload": f"long-format ingestion of the CV data set ({m} covariate entries, "
+                       f"{prob.K} rows, rows and entries shuffled)",
+           "seconds": gpu_s, "entries_per_sec": m / gpu_s, "from_csc_seconds": csc_s,
+           "identical_to_source": identical}
+    log(f"ingest: {m} entries in {gpu_s:.2f}s ({m / gpu_s / 1e6:.1f} M entries/s; from a host "
+        f"CSC {csc_s:.2f}s), identical {identical}")
+    if args.ingest_cpu_fraction > 0:
+        from oracle.ingest import ingest
+        n_s = max(1, int((prob.strata.max() + 1) * args.ingest_cpu_fraction))
+        sid = long["stratum_id"]
+        keep_rows = sid < 5 + 3 * n_s
+        ids = long["row_id"][keep_rows]
+        keep = np.isin(long["entry_row_id"], ids)
+        sample = dict(long, row_id=ids, y=long["y"][keep_rows], stratum_id=sid[keep_rows],
+                      covariate_id=long["covariate_id"][keep],
+                      entry_row_id=long["entry_row_id"][keep], value=long["value"][keep])
+        t = time.perf_counter()
+        ingest(**sample)
+        cpu_s = time.perf_counter() - t
+        ms = int(keep.sum())
+        out["cpu_baseline"] = {"value": ms / cpu_s, "unit": "entries/s", "cores": 1, "kind": "port",
+                               "sample": f"the first {n_s} strata ({ms} entries), numpy oracle"}
+        log(f"ingest cpu oracle: {ms} entries in {cpu_s:.2f}s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,6 +275,10 @@ def main():
     ap.add_argument("--max-gang", type=int, default=0, help="workgroups per fit at most (0: 64)")
     ap.add_argument("--bundle", type=int, default=0, help="fits per slot in lock step (0: default)")
     ap.add_argument("--mm", type=int, default=1, help="MM leg on one CV fit (0: skip)")
+    ap.add_argument("--ingest", type=int, default=1,
+                    help="long-format ingestion leg on the CV data set (0: skip)")
+    ap.add_argument("--ingest-cpu-fraction", type=float, default=0.25,
+                    help="ingestion leg: fraction of the strata the oracle ingests (0: none)")
     ap.add_argument("--mm-cpu-iters", type=int, default=3,
                     help="MM leg: oracle iterations timed for the CPU rate (0: none)")
     args = ap.parse_args()
@@ -370,6 +421,10 @@ def main():
         idx = 3 * folds  # variance grid point 3, fold 0
         mm = mm_leg(args, api, prob, data, specs[idx], betas[idx])
 
+    ingest_out = None
+    if args.ingest and not args.limit:
+        ingest_out = ingest_leg(args, api, synth, prob)
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -401,6 +456,7 @@ def main():
         "tuning": tuning,
         "serving": serving,
         "mm": mm,
+        "ingest": ingest_out,
     }
     print(json.dumps(out), flush=True)
 

@@ -182,6 +182,45 @@ def lr_zipf(K=100_000, P=10_000, per_row=40, s=1.1, n_true=100, sd=0.5, intercep
     return prob, dict(beta_cols=bc, beta_vals=bv)
 
 
+def to_long(p, seed=41):
+    """Problem p's covariate columns (INDICATOR / SPARSE) as long-format input, as
+    convertToCyclopsData takes it: distinct row ids rising with the row and
+    covariate ids rising with the column (so every model's row and column order is
+    p's own), outcome rows and entries in random order; an empty column is written
+    as one zero-valued entry, which creates the column and stores nothing.  Returns the keyword
+    arguments of api.Data.from_long; Cox / Poisson times come from p.offs / the
+    exponentiated offset column."""
+    from .data import DENSE, INTERCEPT, SPARSE
+    rng = np.random.default_rng(seed)
+    K = p.K
+    row_id = 1000 + np.cumsum(rng.integers(1, 5, K))
+    cols = np.flatnonzero(np.isin(p.fmt, (INDICATOR, SPARSE)))
+    cov_id = 10 * np.cumsum(rng.integers(1, 7, len(cols)))
+    counts = np.diff(p.col_ptr)[cols]
+    take = np.concatenate([np.arange(p.col_ptr[j], p.col_ptr[j + 1]) for j in cols]) \
+        if len(cols) else np.zeros(0, np.int64)
+    cov = np.repeat(cov_id, counts)
+    erow = row_id[p.rows[take]]
+    val = np.where(np.repeat(p.fmt[cols] == SPARSE, counts), p.vals[take], 1.0)
+    empty = cov_id[counts == 0]
+    cov = np.concatenate([cov, empty])
+    erow = np.concatenate([erow, np.full(len(empty), row_id[0])])
+    val = np.concatenate([val, np.zeros(len(empty))])
+    e = rng.permutation(len(cov))
+    o = rng.permutation(K)
+    out = dict(model=p.model, row_id=row_id[o], y=p.y[o], covariate_id=cov[e],
+               entry_row_id=erow[e], value=val[e],
+               add_intercept=bool(np.any(p.fmt == INTERCEPT)))
+    if p.model in (CONDITIONAL_LOGISTIC, COX):
+        sid = 5 + 3 * np.arange(int(p.strata.max()) + 1)
+        out["stratum_id"] = sid[p.strata][o]
+    if p.model == COX:
+        out["time"] = p.offs[o]
+    if p.model == POISSON and p.has_offset and p.fmt[0] == DENSE:
+        out["time"] = np.exp(p.vals[p.col_ptr[0]:p.col_ptr[1]])[o]
+    return out
+
+
 def request_stream(n_requests=200, rate=50.0, K_range=(10_000, 200_000),
                    P_range=(1_000, 20_000), per_row_range=(20, 60), cox_K_max=None,
                    pool_per_model=4, models=(LOGISTIC, POISSON, COX), seed=DEFAULT_SEED,

@@ -11,7 +11,9 @@
   clr     200,000 strata (~1.3e6 rows) x 20,000 columns, 10-fold BY_PID CV weights,
           Laplace var 1e-4 and 1e-3 — two of the 160 CV fits, with held-out scores;
           and the MM algorithm on one of them, its first five iterations (the
-          oracle takes ~1.5 s per MM iteration here)
+          oracle takes ~1.5 s per MM iteration here); and the same data set
+          written out in long format (3.2e7 covariate entries, rows and entries
+          shuffled) and ingested on the device: identical to the original
 
 FP64: coefficients and log likelihood within 1e-8 relative of the oracle (atol
 1e-10), same iterations and return flag; per-coordinate gradient and Hessian at the
@@ -185,3 +187,12 @@ def test_full_clr_mm_trajectory(clr_full):
     bg, ig = api.fit_plain(d, sp, algorithm="mm")
     check_fit(bg, ig, bc, ic)
     d.close()
+
+
+def test_full_clr_long_ingest(clr_full):
+    from tests.test_ingest import assert_same_problem
+    p = clr_full
+    d = api.Data.from_long(**synth.to_long(p))
+    q = d.problem
+    d.close()
+    assert_same_problem(q, p)
