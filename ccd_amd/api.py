@@ -20,7 +20,7 @@ FP64, MIXED = 0, 1
 EXPORTS = [
     "ccdh_last_error", "ccdh_device_info", "ccdh_data_create", "ccdh_data_create_long",
     "ccdh_data_get_csc", "ccdh_data_destroy",
-    "ccdh_data_device_bytes", "ccdh_data_shape", "ccdh_fit_create", "ccdh_fit_clone",
+    "ccdh_data_device_bytes", "ccdh_data_shape", "ccdh_data_precision", "ccdh_fit_create", "ccdh_fit_clone",
     "ccdh_fit_destroy", "ccdh_set_weights", "ccdh_fixed_terms_loglik",
     "ccdh_fixed_terms_grad_hess", "ccdh_zero_xbeta", "ccdh_axpy_xbeta", "ccdh_compute_xbeta",
     "ccdh_remaining_stats", "ccdh_numerator", "ccdh_grad_hess", "ccdh_update_xbeta",
@@ -167,6 +167,7 @@ def load_library(path=LIB_PATH):
         "ccdh_data_destroy": (ci, [vp]),
         "ccdh_data_device_bytes": (ctypes.c_int64, [vp]),
         "ccdh_data_shape": (ci, [vp, vp, vp, vp, vp]),
+        "ccdh_data_precision": (ci, [vp]),
         "ccdh_fit_create": (ci, [vp, vp]),
         "ccdh_fit_clone": (ci, [vp, vp]),
         "ccdh_fit_destroy": (ci, [vp]),

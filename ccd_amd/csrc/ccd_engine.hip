@@ -1689,6 +1689,8 @@ int ccdh_data_destroy(ccdh_data_t data) {
 
 int64_t ccdh_data_device_bytes(ccdh_data_t data) { return data ? data->devBytes : 0; }
 
+int ccdh_data_precision(ccdh_data_t data) { return data ? data->precision : -1; }
+
 int ccdh_data_shape(ccdh_data_t data, int* model, int* K, int* J, int* has_offset) {
   if (!data) return setError(CCDH_E_INVALID, "null handle");
   if (model) *model = data->model;

@@ -382,17 +382,20 @@ int dataCreateLong(const ccdh_long_desc* in, ccdh_data** out) {
       }
       hColPtr.resize(nCols + 1);
       hRows.resize(nnz);
-      hVals.resize(nnz);
       hSparse.resize(nCols);
       HIP_CHECK(hipMemcpyAsync(hColPtr.data(), colPtr.p, (size_t)(nCols + 1) * 8,
                                hipMemcpyDeviceToHost, s));
-      if (nnz > 0) {
-        HIP_CHECK(hipMemcpyAsync(hRows.data(), rows.p, (size_t)nnz * 4, hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipMemcpyAsync(hVals.data(), vals.p, (size_t)nnz * 8, hipMemcpyDeviceToHost, s));
-      }
       HIP_CHECK(hipMemcpyAsync(hSparse.data(), sparse.p, (size_t)nCols * sizeof(int),
                                hipMemcpyDeviceToHost, s));
+      if (nnz > 0)
+        HIP_CHECK(hipMemcpyAsync(hRows.data(), rows.p, (size_t)nnz * 4, hipMemcpyDeviceToHost, s));
       HIP_CHECK(hipStreamSynchronize(s));
+      // values only when a column keeps them (an all-INDICATOR design needs none)
+      if (nnz > 0 && std::find(hSparse.begin(), hSparse.end(), 1) != hSparse.end()) {
+        hVals.resize(nnz);
+        HIP_CHECK(hipMemcpyAsync(hVals.data(), vals.p, (size_t)nnz * 8, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+      }
     }
   }
 
@@ -401,9 +404,10 @@ int dataCreateLong(const ccdh_long_desc* in, ccdh_data** out) {
   const int lead = (poisson ? 1 : 0) + (intercept ? 1 : 0);
   const int J = lead + nCols;
   const int64_t nnzCov = (int64_t)hRows.size();
+  const bool withVals = poisson || !hVals.empty();  // a DENSE offset or a SPARSE column
   std::vector<int64_t> colPtr(J + 1, 0);
   std::vector<int32_t> rows((size_t)lead * n + nnzCov);
-  std::vector<double> vals(rows.size());
+  std::vector<double> vals(withVals ? rows.size() : 0);
   std::vector<int8_t> fmt(J);
   int64_t at = 0;
   for (int c = 0; c < lead; ++c) {
@@ -411,14 +415,17 @@ int dataCreateLong(const ccdh_long_desc* in, ccdh_data** out) {
     fmt[c] = offset ? CCDH_DENSE : CCDH_INTERCEPT;
     for (int k = 0; k < n; ++k) {
       rows[at + k] = k;
-      vals[at + k] = offset ? std::log(hTime[k]) : 1.0;
+      if (withVals) vals[at + k] = offset ? std::log(hTime[k]) : 1.0;
     }
     at += n;
     colPtr[c + 1] = at;
   }
   if (nnzCov > 0) {
     std::memcpy(rows.data() + at, hRows.data(), nnzCov * sizeof(int32_t));
-    std::memcpy(vals.data() + at, hVals.data(), nnzCov * sizeof(double));
+    if (!hVals.empty())
+      std::memcpy(vals.data() + at, hVals.data(), nnzCov * sizeof(double));
+    else if (withVals)
+      std::fill(vals.begin() + at, vals.end(), 1.0);
   }
   for (int c = 0; c < nCols; ++c) {
     fmt[lead + c] = hSparse[c] ? CCDH_SPARSE : CCDH_INDICATOR;
@@ -431,7 +438,7 @@ int dataCreateLong(const ccdh_long_desc* in, ccdh_data** out) {
   desc.J = J;
   desc.col_ptr = colPtr.data();
   desc.rows = rows.data();
-  desc.vals = vals.data();
+  desc.vals = withVals ? vals.data() : nullptr;
   desc.fmt = fmt.data();
   desc.y = hY.data();
   desc.offs = cox ? hTime.data() : nullptr;

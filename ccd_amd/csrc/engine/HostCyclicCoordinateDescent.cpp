@@ -281,6 +281,10 @@ extern "C" int ccdh_fit_mm(ccdh_data_t data, const ccdh_fit_spec* spec, double* 
     if (model != CCDH_LOGISTIC && model != CCDH_CONDITIONAL_LOGISTIC)
       return setError(CCDH_E_INVALID,
                       "MM algorithm: not model-specific (logistic and conditional logistic only)");
+    // FP32 row state puts ~1e-7 relative noise into the log posterior, which trips
+    // the strict non-increasing check long before the fit converges
+    if (ccdh_data_precision(data) != CCDH_FP64)
+      return setError(CCDH_E_INVALID, "MM algorithm: FP64 data only");
   }
   return fitHost(data, spec, beta_out, result, true);
 }

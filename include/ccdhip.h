@@ -134,6 +134,8 @@ int ccdh_data_destroy(ccdh_data_t data);
 int64_t ccdh_data_device_bytes(ccdh_data_t data);
 /* model, rows, columns and offset flag of a data handle */
 int ccdh_data_shape(ccdh_data_t data, int* model, int* K, int* J, int* has_offset);
+/* CCDH_FP64 or CCDH_MIXED, as created */
+int ccdh_data_precision(ccdh_data_t data);
 
 /* Long-format input, as convertToCyclopsData takes it (R/NewDataConversion.R:95-235):
  * one outcome row per row id, covariate entries (covariate id, row id, value).  The
@@ -221,7 +223,9 @@ int ccdh_fit_plain(ccdh_data_t data, const ccdh_fit_spec* spec, double* beta_out
  * free coordinate from one set of statistics (no bounding box), then recomputes xBeta
  * from the transpose and the statistics.  A decrease of the log posterior stops the fit
  * with CCDH_E_FIT ("Non-increasing!"); beta_out then holds the coefficients reached and
- * result->iterations the iteration.  Logistic and conditional logistic models only. */
+ * result->iterations the iteration.  Logistic and conditional logistic models, FP64 data
+ * (on the mixed path the log posterior's FP32 noise trips that check long before the fit
+ * converges). */
 int ccdh_fit_mm(ccdh_data_t data, const ccdh_fit_spec* spec, double* beta_out,
                 ccdh_fit_result* result);
 

@@ -141,3 +141,15 @@ def test_mm_grad_hess_needs_statistics():
     finally:
         fh.close()
         d.close()
+
+
+def test_mm_refused_on_mixed_precision_data():
+    # FP32 row state puts ~1e-7 relative noise into the log posterior, which trips
+    # the strict non-increasing check long before the fit converges
+    p = dict(_PROBLEMS)["clr_synth"]
+    d = api.Data(p, precision=api.MIXED)
+    try:
+        with pytest.raises(api.CcdhError, match="FP64 data only"):
+            api.fit_plain(d, default_spec(p, prior=PRIOR_LAPLACE, variance=0.01), algorithm="mm")
+    finally:
+        d.close()
