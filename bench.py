@@ -179,12 +179,29 @@ def mm_leg(args, api, prob, data, spec, beta_ccd):
     except api.NonIncreasing as e:
         bm, iters, flag, status = e.beta, e.iterations, None, "non-increasing"
     gpu_s = time.perf_counter() - t
+    st = api.mm_last_stats()
+    # algorithmic bytes per launch: chunk pass i + v + 8 B (row index, value, row factor
+    # gather) per nonzero; xBeta i + v + 8 B (column, value, coefficient) per transpose
+    # entry + 16 B per row (row start, xBeta)
+    v = 8 if st["values"] else 0
+    chunk_bytes = st["nnz"] * (4 + v + 8)
+    xb_bytes = st["entries"] * (4 + v + 8) + st["rows"] * 16
+    chunk_ms = st["chunk_ms"] / max(1, st["chunk_launches"])
+    xb_ms = st["xbeta_ms"] / max(1, st["xbeta_launches"])
+    achieved = chunk_bytes / (chunk_ms * 1e-3) / 1e9
     scale = max(1.0, float(np.max(np.abs(beta_ccd))))
     out = {"workload": "MM algorithm, one CV fit of the conditional logistic workload "
                        f"(fold 0, Laplace var {spec.prior_var[-1]:.3g})",
            "status": status, "iterations": iters, "return_flag": flag, "seconds": gpu_s,
            "ms_per_iteration": gpu_s / max(1, iters) * 1e3,
-           "max_rel_coef_diff_vs_ccd": float(np.max(np.abs(bm - beta_ccd)) / scale)}
+           "max_rel_coef_diff_vs_ccd": float(np.max(np.abs(bm - beta_ccd)) / scale),
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_mm_chunks",
+                        "avg_launch_ms": chunk_ms, "launches": st["chunk_launches"]},
+           "xbeta_kernel": {"kernel": "k_xbeta_rows", "avg_launch_ms": xb_ms,
+                            "achieved_gbs": xb_bytes / (xb_ms * 1e-3) / 1e9 if xb_ms > 0 else None}}
+    log(f"mm kernels: k_mm_chunks {chunk_ms * 1e3:.1f} us avg ({achieved:.0f} GB/s), "
+        f"k_xbeta_rows {xb_ms * 1e3:.1f} us avg")
     log(f"mm: {status} after {iters} iterations in {gpu_s:.2f}s "
         f"({out['ms_per_iteration']:.2f} ms/iteration), vs CCD {out['max_rel_coef_diff_vs_ccd']:.2e}")
     if args.mm_cpu_iters > 0:

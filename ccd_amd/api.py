@@ -25,7 +25,8 @@ EXPORTS = [
     "ccdh_fixed_terms_grad_hess", "ccdh_zero_xbeta", "ccdh_axpy_xbeta", "ccdh_compute_xbeta",
     "ccdh_remaining_stats", "ccdh_numerator", "ccdh_grad_hess", "ccdh_update_xbeta",
     "ccdh_loglik", "ccdh_gradient_objective", "ccdh_pred_loglik", "ccdh_get_xbeta",
-    "ccdh_mm_grad_hess", "ccdh_fit_plain", "ccdh_fit_mm", "ccdh_batch_fit", "ccdh_batch_last_stats", "ccdh_cv_fold_weights",
+    "ccdh_mm_grad_hess", "ccdh_fit_plain", "ccdh_fit_mm", "ccdh_fit_mm_stats",
+    "ccdh_mm_last_stats", "ccdh_batch_fit", "ccdh_batch_last_stats", "ccdh_cv_fold_weights",
     "ccdh_cv_grid_point", "ccdh_bootstrap_weights", "ccdh_search_step", "ccdh_search_quadratic",
     "ccdh_cox_groups",
     "ccdh_server_create", "ccdh_server_submit",
@@ -65,6 +66,18 @@ class LongDesc(ctypes.Structure):
         ("covariate_id", ctypes.c_void_p), ("entry_row_id", ctypes.c_void_p),
         ("value", ctypes.c_void_p), ("add_intercept", ctypes.c_int),
     ]
+
+
+class MmStats(ctypes.Structure):
+    _fields_ = [
+        ("chunk_launches", ctypes.c_int64), ("chunk_ms", ctypes.c_double),
+        ("xbeta_launches", ctypes.c_int64), ("xbeta_ms", ctypes.c_double),
+        ("nnz", ctypes.c_int64), ("entries", ctypes.c_int64), ("rows", ctypes.c_int64),
+        ("values", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class FitSpecC(ctypes.Structure):
@@ -188,6 +201,8 @@ def load_library(path=LIB_PATH):
         "ccdh_mm_grad_hess": (ci, [vp, vp, ci, vp]),
         "ccdh_fit_plain": (ci, [vp, vp, vp, vp]),
         "ccdh_fit_mm": (ci, [vp, vp, vp, vp]),
+        "ccdh_fit_mm_stats": (ci, [vp, vp]),
+        "ccdh_mm_last_stats": (ci, [vp]),
         "ccdh_batch_fit": (ci, [vp, ci, vp, vp, vp, vp]),
         "ccdh_batch_last_stats": (ci, [vp]),
         "ccdh_cv_fold_weights": (ci, [vp, ci, ci, ci, ctypes.c_long, ci, vp]),
@@ -351,6 +366,13 @@ def fit_plain(data: Data, spec: FitSpec, heldout=None, algorithm="ccd"):
         raise NonIncreasing(lib.ccdh_last_error().decode(), beta, res.iterations)
     _check(rc)
     return beta, res.as_dict()
+
+
+def mm_last_stats():
+    """Event-timed device time of the last MM fit's chunk pass and xBeta kernels."""
+    st = MmStats()
+    _check(load_library().ccdh_mm_last_stats(ctypes.byref(st)))
+    return st.as_dict()
 
 
 def fit_batch(data: Data, specs, heldouts=None, threads=0, sweeps_per_sync=0, use_graph=1,

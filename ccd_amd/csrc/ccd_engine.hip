@@ -1189,6 +1189,8 @@ void fitDestroy(ccdh_fit* f) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (f->hres) (void)hipHostFree(f->hres);
+  for (hipEvent_t e : f->mmEv)
+    if (e) (void)hipEventDestroy(e);
   if (f->stream) (void)hipStreamDestroy(f->stream);
   delete f;
 }
@@ -1595,12 +1597,44 @@ static int ensureTranspose(ccdh_data* d, hipStream_t s) {
 static int ensureMm(ccdh_fit* f) {
   ccdh_data* d = f->d;
   int rc = ensureTranspose(d, f->stream);
+  for (int i = 0; !rc && i < 4; ++i)
+    if (!f->mmEv[i]) {
+      hipError_t e = hipEventCreate(&f->mmEv[i]);
+      if (e != hipSuccess) rc = setError(CCDH_E_DEVICE, "hipEventCreate");
+    }
   if (!rc && !f->mmPart) rc = devAlloc(&f->mmPart, 2 * (size_t)std::max(d->nChunks, 1), nullptr);
   if (!rc && !f->mmGH) rc = devAlloc(&f->mmGH, 2 * (size_t)d->J, nullptr);
   if (!rc && !f->mmBeta) rc = devAlloc(&f->mmBeta, (size_t)d->J, nullptr);
   if (!rc && !f->mmFixed) rc = devAlloc(&f->mmFixed, (size_t)d->J, nullptr);
   if (!rc && !f->mmR) rc = devAlloc(&f->mmR, (size_t)d->K, nullptr);
   return rc;
+}
+
+// adds the last xBeta launch's event time once it has run
+static int readXbEvents(ccdh_fit* f) {
+  if (!f->mmXbPending) return 0;
+  float ms = 0.0f;
+  HIP_TRY(hipEventSynchronize(f->mmEv[3]));
+  HIP_TRY(hipEventElapsedTime(&ms, f->mmEv[2], f->mmEv[3]));
+  f->mmStats.xbeta_launches++;
+  f->mmStats.xbeta_ms += ms;
+  f->mmXbPending = false;
+  return 0;
+}
+
+int mmStats(ccdh_fit* f, ccdh_mm_stats* out) {
+  int rc = readXbEvents(f);
+  if (rc) return rc;
+  ccdh_data* d = f->d;
+  *out = f->mmStats;
+  out->nnz = d->hColPtr[d->J];
+  out->entries = 0;
+  if (d->tRowPtr) HIP_TRY(hipMemcpy(&out->entries, d->tRowPtr + d->K, 8, hipMemcpyDeviceToHost));
+  out->rows = d->K;
+  out->values = 0;
+  for (int j = 0; j < d->J; ++j)
+    if (d->hFmt[j] == CCDH_SPARSE || d->hFmt[j] == CCDH_DENSE) out->values = 1;
+  return 0;
 }
 
 // ModelSpecifics::computeXBeta(beta) (ModelSpecifics.hpp:313-364): xBeta from
@@ -1612,8 +1646,13 @@ int computeXBetaRows(ccdh_fit* f, const double* beta) {
   ccdh_data* d = f->d;
   HIP_TRY(hipMemcpyAsync(f->mmBeta, beta, d->J * sizeof(double), hipMemcpyHostToDevice,
                          f->stream));
+  rc = readXbEvents(f);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(f->mmEv[2], f->stream));
   LAUNCH_R(k_xbeta_rows, gridFor(d->K, 1), d->K, d->tRowPtr, d->tCol, d->tVal, f->mmBeta,
            XB(T_));
+  HIP_TRY(hipEventRecord(f->mmEv[3], f->stream));
+  f->mmXbPending = true;
   f->statsKnown = false;
   return 0;
 }
@@ -1635,9 +1674,11 @@ int mmGradHess(ccdh_fit* f, const uint8_t* fixed, double* gh) {
            f->mmR);
   const int wavesPerBlock = kThreads / kWave;
   if (d->nChunks > 0) {
+    HIP_TRY(hipEventRecord(f->mmEv[0], f->stream));
     k_mm_chunks<<<(d->nChunks + wavesPerBlock - 1) / wavesPerBlock, kThreads, 0, f->stream>>>(
         d->view(), f->mmR, d->mmChunk0, d->mmChunkCol, d->nChunks, f->mmFixed, f->mmPart);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(f->mmEv[1], f->stream));
   }
   const int finishBlocks = std::min((d->J + wavesPerBlock - 1) / wavesPerBlock, 16384);
   k_mm_finish<<<finishBlocks, kThreads, 0, f->stream>>>(d->J, d->mmChunk0, f->mmPart, f->xjy,
@@ -1646,7 +1687,13 @@ int mmGradHess(ccdh_fit* f, const uint8_t* fixed, double* gh) {
   HIP_TRY(hipMemcpyAsync(gh, f->mmGH, 2 * (size_t)d->J * sizeof(double), hipMemcpyDeviceToHost,
                          f->stream));
   HIP_TRY(hipStreamSynchronize(f->stream));
-  return 0;
+  if (d->nChunks > 0) {
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, f->mmEv[0], f->mmEv[1]));
+    f->mmStats.chunk_launches++;
+    f->mmStats.chunk_ms += ms;
+  }
+  return readXbEvents(f);
 }
 
 }  // namespace ccdamd
@@ -1768,6 +1815,12 @@ int ccdh_axpy_xbeta(ccdh_fit_t fit, double beta, int j) {
 int ccdh_compute_xbeta(ccdh_fit_t fit, const double* beta) {
   GUARD(fit);
   return computeXBetaRows(fit, beta);
+}
+
+int ccdh_fit_mm_stats(ccdh_fit_t fit, ccdh_mm_stats* out) {
+  GUARD(fit);
+  if (!out) return setError(CCDH_E_INVALID, "null argument");
+  return mmStats(fit, out);
 }
 
 int ccdh_mm_grad_hess(ccdh_fit_t fit, const uint8_t* fixed, int use_weights, double* gh_out) {

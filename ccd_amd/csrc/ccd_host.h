@@ -82,6 +82,9 @@ struct ccdh_fit {
   double* mmGH = nullptr;
   double* mmBeta = nullptr;
   uint8_t* mmFixed = nullptr;
+  hipEvent_t mmEv[4] = {nullptr, nullptr, nullptr, nullptr};  // chunk pass, xBeta: start/stop
+  bool mmXbPending = false;  // an xBeta launch whose events are not yet read
+  ccdh_mm_stats mmStats{};
   double fixedLL = 0.0;
   bool weightsSet = false, statsKnown = false;
   std::vector<double> hKw;
@@ -118,6 +121,7 @@ int predLogLik(ccdh_fit* f, const double* w, double* out);
 int getXBeta(ccdh_fit* f, double* out);
 int computeXBetaRows(ccdh_fit* f, const double* beta);
 int mmGradHess(ccdh_fit* f, const uint8_t* fixed, double* gh);
+int mmStats(ccdh_fit* f, ccdh_mm_stats* out);
 
 // batched / served engine helpers
 int validateSpec(const ccdh_data* d, const ccdh_fit_spec& sp);

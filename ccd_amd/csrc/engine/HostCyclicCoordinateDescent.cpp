@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -232,6 +233,9 @@ class HostCyclicCoordinateDescent {
 
 using namespace ccdamd;
 
+static std::mutex g_mmStatsMu;
+static ccdh_mm_stats g_mmStats{};
+
 static int fitHost(ccdh_data_t data, const ccdh_fit_spec* spec, double* beta_out,
                    ccdh_fit_result* result, bool mm) {
   if (!data || !spec || !beta_out || !result) return setError(CCDH_E_INVALID, "null argument");
@@ -245,13 +249,22 @@ static int fitHost(ccdh_data_t data, const ccdh_fit_spec* spec, double* beta_out
     shared->handle = data;
     HipModelSpecifics ms(shared, model, K, J);
     HostCyclicCoordinateDescent ccd(ms, J, hasOffset != 0, *spec, K, mm);
+    auto keepStats = [&]() {
+      ccdh_mm_stats st{};
+      if (mm && ccdh_fit_mm_stats(ms.handle(), &st) == CCDH_OK) {
+        std::lock_guard<std::mutex> lock(g_mmStatsMu);
+        g_mmStats = st;
+      }
+    };
     try {
       ccd.update();
     } catch (const NonIncreasing& e) {
       for (int j = 0; j < J; ++j) beta_out[j] = ccd.beta[j];
       result->iterations = ccd.iterationReached;
+      keepStats();
       return setError(CCDH_E_FIT, e.what());
     }
+    keepStats();
     for (int j = 0; j < J; ++j) beta_out[j] = ccd.beta[j];
     result->log_likelihood = ccd.logLikelihood();
     result->log_prior = ccd.logPrior();
@@ -287,4 +300,11 @@ extern "C" int ccdh_fit_mm(ccdh_data_t data, const ccdh_fit_spec* spec, double* 
       return setError(CCDH_E_INVALID, "MM algorithm: FP64 data only");
   }
   return fitHost(data, spec, beta_out, result, true);
+}
+
+extern "C" int ccdh_mm_last_stats(ccdh_mm_stats* out) {
+  if (!out) return setError(CCDH_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> lock(g_mmStatsMu);
+  *out = g_mmStats;
+  return CCDH_OK;
 }

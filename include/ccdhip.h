@@ -229,6 +229,24 @@ int ccdh_fit_plain(ccdh_data_t data, const ccdh_fit_spec* spec, double* beta_out
 int ccdh_fit_mm(ccdh_data_t data, const ccdh_fit_spec* spec, double* beta_out,
                 ccdh_fit_result* result);
 
+/* Device time of the MM step's two main kernels, from HIP events on the fit's
+ * stream: the per-column chunk pass (k_mm_chunks) and xBeta from the transpose
+ * (k_xbeta_rows), with the sizes their bytes follow from.  No counterpart in the
+ * reference (its CPU timers, printTiming). */
+typedef struct {
+  int64_t chunk_launches;
+  double chunk_ms;     /* summed over launches */
+  int64_t xbeta_launches;
+  double xbeta_ms;
+  int64_t nnz;         /* nonzeros of the data set's compressed columns */
+  int64_t entries;     /* entries of its transpose */
+  int64_t rows;
+  int values;          /* 1: SPARSE / DENSE values are read (8 B per nonzero) */
+} ccdh_mm_stats;
+/* For one fit handle (all its MM steps so far) and for the last ccdh_fit_mm call. */
+int ccdh_fit_mm_stats(ccdh_fit_t fit, ccdh_mm_stats* out);
+int ccdh_mm_last_stats(ccdh_mm_stats* out);
+
 /* Batched engine options; zero fields pick the defaults. */
 typedef struct {
   int threads;           /* workgroup size: 256 (<= 256) or 512 */

@@ -153,3 +153,19 @@ def test_mm_refused_on_mixed_precision_data():
             api.fit_plain(d, default_spec(p, prior=PRIOR_LAPLACE, variance=0.01), algorithm="mm")
     finally:
         d.close()
+
+
+def test_mm_kernel_stats():
+    p = dict(_PROBLEMS)["clr_synth"]
+    sp = default_spec(p, prior=PRIOR_LAPLACE, variance=0.01)
+    d = api.Data(p)
+    try:
+        _, ig = api.fit_plain(d, sp, algorithm="mm")
+    finally:
+        d.close()
+    st = api.mm_last_stats()
+    # one chunk pass and one xBeta launch per MM iteration
+    assert st["chunk_launches"] == ig["iterations"] == st["xbeta_launches"]
+    assert st["chunk_ms"] > 0 and st["xbeta_ms"] > 0
+    assert st["nnz"] == p.nnz and st["entries"] == p.nnz and st["rows"] == p.K
+    assert st["values"] == 0
