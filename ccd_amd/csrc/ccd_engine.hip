@@ -1525,6 +1525,7 @@ int getXBeta(ccdh_fit* f, double* out) {
 // Row-major copy of X and the MM column chunks, built once per data set on the
 // device: entries per row, an exclusive scan, a scatter, a per-row sort.
 static int ensureTranspose(ccdh_data* d, hipStream_t s) {
+  std::lock_guard<std::mutex> lock(d->lazyMu);
   if (d->tRowPtr) return 0;
   const int K = d->K, J = d->J;
   bool anyDense = false, anyIndicator = false;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -40,6 +41,7 @@ struct ccdh_data {
   int32_t* mmChunk0 = nullptr;  // [J+1] first chunk of each column
   int32_t* mmChunkCol = nullptr;// [nChunks] column of each chunk
   int nChunks = 0;
+  std::mutex lazyMu;  // clones on other threads may reach the first build together
   bool placed = false;  // device arrays carved from a caller's block (not freed here)
   // host copies of the small per-column / per-row vectors
   std::vector<int64_t> hColPtr, hSegPtr;

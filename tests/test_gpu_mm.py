@@ -169,3 +169,33 @@ def test_mm_kernel_stats():
     assert st["chunk_ms"] > 0 and st["xbeta_ms"] > 0
     assert st["nnz"] == p.nnz and st["entries"] == p.nnz and st["rows"] == p.K
     assert st["values"] == 0
+
+
+def test_transpose_built_once_under_concurrent_clones():
+    # fit handles over one data set on two threads reach the lazy build together
+    import threading
+    p = dict(_PROBLEMS)["clr_synth"]
+    rng = np.random.default_rng(25)
+    betas = [rng.normal(0, 0.3, p.J) for _ in range(2)]
+    d = api.Data(p)
+    out, errs = [None, None], []
+
+    def run(i):
+        try:
+            fh = api.FitHandle(d)
+            for _ in range(3):
+                fh.compute_xbeta(betas[i])
+            out[i] = fh.xbeta()
+            fh.close()
+        except Exception as e:  # reported below
+            errs.append(e)
+
+    ts = [threading.Thread(target=run, args=(i,)) for i in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    d.close()
+    assert not errs
+    for i in range(2):
+        np.testing.assert_allclose(out[i], orc.mm_xbeta(p, betas[i]), rtol=1e-14, atol=1e-14)
