@@ -385,7 +385,8 @@ def main():
         f"({stats['long_columns']} visits), objective {stats['ticks_objective'] * tick:.1f}s; "
         f"sweeps {stats['sweeps']}, gathered nnz {stats['gather_nnz']}, updated nnz {stats['update_nnz']}")
     log("launch timeline (ms, fits in flight, workgroups per fit):", stats["timeline"])
-    tuning = {"autotune": bool(args.autotune), "pass_items": stats["tuned_pass_items"],
+    tuning = {"autotune": bool(args.autotune), "threads": stats["tuned_threads"],
+              "pass_items": stats["tuned_pass_items"],
               "max_gang": stats["tuned_max_gang"], "tune_ms": stats["tune_ms"],
               "candidate_coords_per_ms": [r for r in stats["tune_rates"] if r > 0]}
     log("tuning:", tuning)

@@ -1772,6 +1772,8 @@ int initRanks(int64_t nnz, int K) {
 
 // Sweep kernel variants: (256 lanes, pass width 8 or 4), (512 lanes, width 8).
 static const void* sweepKernel(bool mixed, int threads, int itl) {
+  if (clampThreads(threads) == 512 && itl == 4)
+    return mixed ? (const void*)k_slot_sweep<float, 512, 4> : (const void*)k_slot_sweep<double, 512, 4>;
   if (clampThreads(threads) == 512)
     return mixed ? (const void*)k_slot_sweep<float, 512, 8> : (const void*)k_slot_sweep<double, 512, 8>;
   if (itl == 4)
@@ -2184,7 +2186,10 @@ struct TuneKey {
   }
 };
 std::mutex g_tuneMu;
-std::map<TuneKey, std::pair<int, int>> g_tuned;
+struct TuneChoice {
+  int threads, itl, gang;
+};
+std::map<TuneKey, TuneChoice> g_tuned;
 
 constexpr long long kTrialTicks = 5000000;  // 50 ms of device time per candidate
 
@@ -2212,29 +2217,38 @@ int timeTrial(bool mixed, const void* dslots, FitStatus* dst, const GangEntry* d
   return 0;
 }
 
-// Pass width x gang policy timed on this workload (one quantum per candidate,
-// back to the starting point after each); the choice is cached per workload.
+// Launch shape (workgroup size x pass width) x gang policy timed on this
+// workload (one quantum per candidate, back to the starting point after each);
+// the choice is cached per workload.  With the workgroup size left open the
+// shapes are 256 x 8 and 512 x 4 (two waves per SIMD, the same pass width);
+// a caller's own size keeps it and varies the pass width.
 template <class Trial, class Reset>
-int tuneBatch(const TuneKey& key, int threads, int maxGang, Trial trial, Reset reset, int* itlOut,
-              int* gangOut) {
+int tuneBatch(const TuneKey& key, int threads, bool autoThreads, int maxGang, Trial trial,
+              Reset reset, TuneChoice* out) {
   const auto t0 = std::chrono::steady_clock::now();
   {
     std::lock_guard<std::mutex> g(g_tuneMu);
     auto it = g_tuned.find(key);
     if (it != g_tuned.end()) {
-      *itlOut = it->second.first;
-      *gangOut = it->second.second;
+      *out = it->second;
       return 0;
     }
   }
-  std::vector<std::pair<int, int>> cands = {{8, maxGang}, {8, 1}};
-  if (clampThreads(threads) == 256) cands.push_back({4, maxGang});
-  if (maxGang == 1) cands.erase(cands.begin() + 1);
+  std::vector<TuneChoice> cands;
+  if (autoThreads) {
+    cands = {{256, 8, maxGang}, {512, 4, maxGang}};
+    if (maxGang > 1) cands.push_back({256, 8, 1});
+  } else {
+    const int T = clampThreads(threads);
+    cands = {{T, 8, maxGang}};
+    if (maxGang > 1) cands.push_back({T, 8, 1});
+    if (T == 256) cands.push_back({T, 4, maxGang});
+  }
   double best = -1.0;
-  std::pair<int, int> pick = cands[0];
+  TuneChoice pick = cands[0];
   for (size_t c = 0; c < cands.size(); ++c) {
     double rate = 0.0;
-    int rc = trial(cands[c].first, cands[c].second, &rate);
+    int rc = trial(cands[c], &rate);
     if (rc) return rc;
     g_stats.tune_rates[c] = rate;
     if (rate > best) {
@@ -2244,8 +2258,7 @@ int tuneBatch(const TuneKey& key, int threads, int maxGang, Trial trial, Reset r
     rc = reset();
     if (rc) return rc;
   }
-  *itlOut = pick.first;
-  *gangOut = pick.second;
+  *out = pick;
   {
     std::lock_guard<std::mutex> g(g_tuneMu);
     g_tuned[key] = pick;
@@ -2269,11 +2282,12 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
   const int J = d->J;
   const bool mixed = std::is_same<R, float>::value;
   const int model = d->model;
-  int threads = (opt && opt->threads) ? opt->threads : 256;
+  const bool autoThreads = !(opt && opt->threads);
+  int threads = autoThreads ? 256 : opt->threads;
   threads = std::max(64, std::min(1024, threads / 64 * 64));
   const int maxGang = (opt && opt->max_gang > 0) ? opt->max_gang : kShards;
   const int64_t minNnz = (opt && opt->gang_min_nnz > 0) ? opt->gang_min_nnz : 32768;
-  int itl = (opt && opt->pass_items == 4 && clampThreads(threads) == 256) ? 4 : kITLDefault;
+  int itl = (opt && opt->pass_items == 4) ? 4 : kITLDefault;
   const bool autotune = opt && opt->autotune && model != CCDH_COX;
   // graph replay of a fixed launch when gangs are off
   const bool useGraph = (opt ? opt->use_graph != 0 : true) && maxGang == 1 && !autotune;
@@ -2324,7 +2338,9 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
     anyHeld |= specs[f].heldout != nullptr;
   }
   const SlotLayout L = slotLayout(model, d->K, J, maxN, maxColumnLength(d), anyHeld, sizeof(R), Fmax);
-  const int capacity = std::min(slotGangCapacity(mixed, threads, 8), slotGangCapacity(mixed, threads, 4));
+  // co-resident workgroups of the launch shapes this batch may use
+  int capacity = std::min(slotGangCapacity(mixed, threads, 8), slotGangCapacity(mixed, threads, 4));
+  if (autoThreads && opt && opt->autotune) capacity = std::min(capacity, slotGangCapacity(mixed, 512, 4));
   if (capacity <= 0) return setError(CCDH_E_DEVICE, "sweep kernel occupancy query failed");
   (void)xcdCount();
   char* arena = nullptr;
@@ -2451,16 +2467,19 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
   std::memset(&g_stats, 0, sizeof(g_stats));
   g_stats.tuned_pass_items = itl;
   g_stats.tuned_max_gang = gangCap;
+  g_stats.tuned_threads = clampThreads(threads);
   if (!o.ev0) HIP_TRY(hipEventCreate(&o.ev0));
   if (!o.ev1) HIP_TRY(hipEventCreate(&o.ev1));
   if (autotune) {
-    const TuneKey key{d->model, d->K, d->J, nFit, bundleMax, d->precision, clampThreads(threads),
-                      maxGang, d->nnz};
+    const TuneKey key{d->model, d->K, d->J, nFit, bundleMax, d->precision,
+                      autoThreads ? 0 : clampThreads(threads), maxGang, d->nnz};
+    TuneChoice choice{};
     rc = tuneBatch(
-        key, threads, maxGang,
-        [&](int candItl, int candGang, double* rate) -> int {
-          itl = candItl;
-          gangCap = candGang;
+        key, threads, autoThreads, maxGang,
+        [&](const TuneChoice& cand, double* rate) -> int {
+          threads = cand.threads;
+          itl = cand.itl;
+          gangCap = cand.gang;
           int r = buildMap();
           if (r) return r;
           return timeTrial(mixed, dslots, dst, dmap, mapN, dxch, mapGangs, threads, itl, nFit, J,
@@ -2482,10 +2501,14 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
           HIP_TRY(hipStreamSynchronize(s));
           return 0;
         },
-        &itl, &gangCap);
+        &choice);
     if (rc) return rc;
+    threads = choice.threads;
+    itl = choice.itl;
+    gangCap = choice.gang;
     g_stats.tuned_pass_items = itl;
     g_stats.tuned_max_gang = gangCap;
+    g_stats.tuned_threads = clampThreads(threads);
   }
   rc = buildMap();
   if (rc) return rc;

@@ -249,13 +249,14 @@ int ccdh_mm_last_stats(ccdh_mm_stats* out);
 
 /* Batched engine options; zero fields pick the defaults. */
 typedef struct {
-  int threads;           /* workgroup size: 256 (<= 256) or 512 */
+  int threads;           /* workgroup size: 256 (<= 256) or 512; 0: 256, or with autotune
+                            256 x 8 or 512 x 4 (workgroup size x pass width), as timed */
   int sweeps_per_sync;   /* sweeps per launch: with max_gang 1 (0: 4); with gangs a cap on top of
                             the time quantum (0: none) */
   int use_graph;         /* replay the sweep launch from a hipGraph (only with max_gang 1) */
   int max_gang;          /* workgroups one fit may use (1: one each; 0: 64) */
   int64_t gang_min_nnz;  /* nonzeros per workgroup below which a fit's gang stops growing (0: 32768) */
-  int pass_items;        /* long-column pass width, nonzeros per lane: 8 (0) or 4 (256 lanes) */
+  int pass_items;        /* long-column pass width, nonzeros per lane: 8 (0) or 4 */
   int autotune;          /* time pass width x gangs on this workload first; the choice is cached */
   double quantum_ms;     /* with gangs: a launch ends at the first coordinate boundary after
                             this long, and gang sizes are re-dealt between launches (0: 200) */
@@ -297,6 +298,7 @@ typedef struct {
   double tune_ms;
   double tune_rates[4];
   int32_t bundles;         /* slots the fits were bundled into */
+  int32_t tuned_threads;   /* workgroup size of the sweep launches */
 } ccdh_batch_stats;
 int ccdh_batch_last_stats(ccdh_batch_stats* out);
 

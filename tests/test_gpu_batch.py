@@ -262,6 +262,21 @@ def test_batch_pass_width_changes_only_rounding():
     assert [r["iterations"] for r in i4] == [r["iterations"] for r in i8]
 
 
+@pytest.mark.parametrize("name", ["clr", "poisson"])
+def test_batch_512_threads_4_items_matches_oracle(name):
+    # two waves per SIMD: the same pass width as 256 x 8, other reduction order
+    p, prior, var, maxit = SYNTH[name]()
+    specs, held = cv_specs(p, prior, var, maxit, n_fold=2, grid=(1.0, 0.1))
+    d = api.Data(p)
+    b5, i5 = api.fit_batch(d, specs, heldouts=held, threads=512, pass_items=4, gang_min_nnz=1)
+    b2, i2 = api.fit_batch(d, specs, heldouts=held, gang_min_nnz=1)
+    d.close()
+    for i, sp in enumerate(specs):
+        bc, ic = orc.fit(p, sp, heldout=held[i])
+        check_fit(b5[i], i5[i], bc, ic)
+    np.testing.assert_allclose(b5, b2, rtol=1e-10, atol=1e-13)
+
+
 def test_batch_autotune_matches_oracle_and_caches():
     p, prior, var, maxit = SYNTH["poisson"]()
     specs, held = cv_specs(p, prior, var, maxit, n_fold=2, grid=(1.0, 0.1))
@@ -269,6 +284,7 @@ def test_batch_autotune_matches_oracle_and_caches():
     bg, ig = api.fit_batch(d, specs, heldouts=held, autotune=1, gang_min_nnz=1)
     st = api.batch_last_stats()
     assert st["tuned_pass_items"] in (4, 8) and st["tuned_max_gang"] >= 1
+    assert (st["tuned_threads"], st["tuned_pass_items"]) in ((256, 8), (512, 4))
     assert st["tune_ms"] > 0 and max(st["tune_rates"]) > 0
     for i, sp in enumerate(specs):
         bc, ic = orc.fit(p, sp, heldout=held[i])

@@ -13,7 +13,8 @@ from ccd_amd.data import COX, INTERCEPT, LOGISTIC, PRIOR_LAPLACE, default_spec
 from oracle import oracle as orc
 from oracle.ingest import ingest
 from tests.test_gpu_plain import check_fit
-from tests.test_ingest import ERROR_CASES, assert_same_problem, round_trip_problems, to_long
+from tests.test_ingest import (EDGE_CASES, ERROR_CASES, assert_same_problem, round_trip_problems,
+                               to_long)
 
 pytestmark = pytest.mark.gpu
 
@@ -68,3 +69,11 @@ def test_fit_over_ingested_data():
     bc, ic = orc.fit(p, sp)
     check_fit(bg, ig, bc, ic)
     check_fit(bb[0], ib[0], bc, ic)
+
+
+@pytest.mark.parametrize("name,case", EDGE_CASES, ids=[c[0] for c in EDGE_CASES])
+def test_device_ingest_edge_cases(name, case):
+    d = api.Data.from_long(**case)
+    q = d.problem
+    d.close()
+    assert_same_problem(q, ingest(**case))

@@ -163,3 +163,26 @@ def test_ingest_no_intercept_for_conditional_models():
     q = ingest(CONDITIONAL_LOGISTIC, row_id=[1, 2], y=[0, 1], stratum_id=[3, 3],
                covariate_id=[5], entry_row_id=[2], add_intercept=True)
     assert q.J == 1 and q.fmt[0] == INDICATOR
+
+
+EDGE_CASES = [
+    # no covariate entries: the intercept only
+    ("no_entries", dict(model=LOGISTIC, row_id=[5, 3], y=[1, 0], covariate_id=[],
+                        entry_row_id=[])),
+    # every entry names an unknown row: no covariate column
+    ("unknown_rows", dict(model=LOGISTIC, row_id=[5, 3], y=[1, 0], covariate_id=[7, 8],
+                          entry_row_id=[1, 2], value=[1.0, 2.0])),
+    # Poisson without covariates: log-time offset and intercept
+    ("poisson_offset_only", dict(model=POISSON, row_id=[5, 3], y=[1, 0], time=[2.0, 3.0],
+                                 covariate_id=[], entry_row_id=[])),
+]
+
+
+def test_ingest_edge_cases():
+    q = ingest(**EDGE_CASES[0][1])
+    assert q.J == 1 and list(q.fmt) == [INTERCEPT] and list(q.y) == [0.0, 1.0]
+    q = ingest(**EDGE_CASES[1][1])
+    assert q.J == 1 and list(q.col_ptr) == [0, 2]
+    q = ingest(**EDGE_CASES[2][1])
+    assert list(q.fmt) == [DENSE, INTERCEPT] and q.has_offset
+    np.testing.assert_array_equal(q.vals[:2], np.log([3.0, 2.0]))
