@@ -401,7 +401,8 @@ def main():
         t = json.load(open(tpath))
         if (t.get("strata") == n_strata and t.get("covariates") == P and t.get("fits") == n_fits
                 and precision == api.FP64 and args.sweeps_per_sync == 0 and args.max_gang == 0
-                and args.bundle in (0, 1)):
+                and args.bundle in (0, 1) and t.get("threads") == stats["tuned_threads"]
+                and t.get("pass_items") == stats["tuned_pass_items"]):
             traffic = t["traffic_bytes_per_launch"]
 
     # plain engine and CPU oracle on a spread of the same fits
@@ -468,7 +469,9 @@ def main():
         "max_rel_coef_diff_plain_vs_batched": plain_err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_slot_sweep", "avg_launch_ms": per_launch_ms,
+                     "kernel": f"k_slot_sweep<{'double' if precision == api.FP64 else 'float'}, "
+                               f"{stats['tuned_threads']}, {stats['tuned_pass_items']}>",
+                     "avg_launch_ms": per_launch_ms,
                      "launches": stats["launches"]},
         "cpu_baseline": cpu,
         "tuning": tuning,
