@@ -86,7 +86,8 @@ def serve_leg(args, api, synth):
         return default_spec(p, prior=prior, variance=var)
 
     specs = [spec_of(i, v) for _, i, v in reqs]
-    srv = api.Server(precision=api.FP64, quantum_ms=args.serve_quantum_ms)
+    srv = api.Server(precision=api.FP64, quantum_ms=args.serve_quantum_ms,
+                     threads=args.serve_threads)
     srv.wait(srv.submit(pool[0], spec_of(0, 1.0)))  # load code objects (untimed)
     t0 = time.perf_counter()
     ids = []
@@ -283,6 +284,8 @@ def main():
     ap.add_argument("--serve-pmax", type=int, default=20_000, help="largest request columns")
     ap.add_argument("--serve-cox-kmax", type=int, default=20_000, help="largest Cox request rows")
     ap.add_argument("--serve-quantum-ms", type=float, default=20.0)
+    ap.add_argument("--serve-threads", type=int, default=0,
+                    help="server workgroup size: 256 (0) or 512 (x 4 items per lane)")
     ap.add_argument("--serve-plain-seconds", type=float, default=20.0,
                     help="time budget of the plain sequential sample (0: none)")
     ap.add_argument("--skip-batch", action="store_true", help="serving leg only (probe)")

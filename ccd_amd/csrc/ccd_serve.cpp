@@ -96,8 +96,11 @@ class Server final : public ccdh_server {
     precision = o.precision;
     maxSlots_ = o.max_slots > 0 ? o.max_slots : 1024;
     threads_ = o.threads > 0 ? o.threads : 256;
+    // 512 threads (any size above 256) take 4 items per lane: the same pass
+    // width as 256 x 8, two waves per SIMD
+    itl_ = threads_ > 256 ? 4 : 8;
     maxGang_ = o.max_gang > 0 ? o.max_gang : kShards;
-    capacity_ = slotGangCapacity(kMixed, threads_, 8);
+    capacity_ = slotGangCapacity(kMixed, threads_, itl_);
     if (capacity_ <= 0) return setError(CCDH_E_DEVICE, "sweep kernel occupancy query failed");
     (void)xcdCount();
     const double qms = o.quantum_ms > 0 ? o.quantum_ms : 20.0;
@@ -426,7 +429,7 @@ class Server final : public ccdh_server {
                        schedStream_) != hipSuccess ||
         hipEventRecord(ev0_, schedStream_) != hipSuccess)
       return setError(CCDH_E_DEVICE, "launch set-up");
-    int rc = slotLaunchSweeps(kMixed, dSlots_, dSt_, dMap_, n, dXch_, gangs, threads_, 8, 1 << 30,
+    int rc = slotLaunchSweeps(kMixed, dSlots_, dSt_, dMap_, n, dXch_, gangs, threads_, itl_, 1 << 30,
                               quantumTicks_, schedStream_);
     if (rc) return rc;
     if (hipEventRecord(ev1_, schedStream_) != hipSuccess ||
@@ -589,7 +592,7 @@ class Server final : public ccdh_server {
 
   static constexpr int64_t kGangMinNnz = 32768;
   ccdh_server_options opt_{};
-  int maxSlots_ = 0, threads_ = 0, maxGang_ = 0, capacity_ = 0;
+  int maxSlots_ = 0, threads_ = 0, itl_ = 8, maxGang_ = 0, capacity_ = 0;
   std::vector<GangEntry> map_;
   GangEntry* dMap_ = nullptr;
   Xch* dXch_ = nullptr;

@@ -27,9 +27,11 @@ def _requests():
     return out
 
 
-def test_serve_mixed_requests_match_oracle():
+@pytest.mark.parametrize("threads", [256, 512])
+def test_serve_mixed_requests_match_oracle(threads):
+    # 512 threads run 4 items per lane (two waves per SIMD)
     reqs = _requests()
-    with api.Server(quantum_ms=2.0) as srv:
+    with api.Server(quantum_ms=2.0, threads=threads) as srv:
         ids = [srv.submit(p, sp) for p, sp in reqs]
         got = [srv.wait(i) for i in ids]
         st = srv.stats()
