@@ -202,15 +202,15 @@ int dataCreateLong(const ccdh_long_desc* in, ccdh_data** out) {
   HIP_CHECK(kdb.alloc(n));
   HIP_CHECK(perm.alloc(n));
   HIP_CHECK(perm2.alloc(n));
-  HIP_CHECK(hipMemcpyAsync(dRowId.p, in->row_id, n * 8, hipMemcpyHostToDevice, s));
-  HIP_CHECK(hipMemcpyAsync(dY.p, in->y, n * 8, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(dRowId.p, in->row_id, (size_t)n * 8, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(dY.p, in->y, (size_t)n * 8, hipMemcpyHostToDevice, s));
   if (in->time) {
     HIP_CHECK(dTime.alloc(n));
-    HIP_CHECK(hipMemcpyAsync(dTime.p, in->time, n * 8, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(dTime.p, in->time, (size_t)n * 8, hipMemcpyHostToDevice, s));
   }
   if (strata) {
     HIP_CHECK(dStratum.alloc(n));
-    HIP_CHECK(hipMemcpyAsync(dStratum.p, in->stratum_id, n * 8, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(dStratum.p, in->stratum_id, (size_t)n * 8, hipMemcpyHostToDevice, s));
   }
   k_iota<<<blocksFor(n), kT, 0, s>>>(n, perm.p);
   HIP_CHECK(hipGetLastError());
@@ -250,12 +250,12 @@ int dataCreateLong(const ccdh_long_desc* in, ccdh_data** out) {
   std::vector<int32_t> hStrata;
   k_gather<double><<<blocksFor(n), kT, 0, s>>>(n, dY.p, perm.p, kda.p);
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipMemcpyAsync(hY.data(), kda.p, n * 8, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(hY.data(), kda.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
   if (in->time && (cox || poisson)) {
     hTime.resize(n);
     k_gather<double><<<blocksFor(n), kT, 0, s>>>(n, dTime.p, perm.p, kdb.p);
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(hTime.data(), kdb.p, n * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(hTime.data(), kdb.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
   }
   if (strata) {
     DevBuf<int32_t> flags, pid;
@@ -268,7 +268,7 @@ int dataCreateLong(const ccdh_long_desc* in, ccdh_data** out) {
       return hipcub::DeviceScan::InclusiveSum(t, b, flags.p, pid.p, n, s);
     }));
     hStrata.resize(n);
-    HIP_CHECK(hipMemcpyAsync(hStrata.data(), pid.p, n * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(hStrata.data(), pid.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
   }
 
