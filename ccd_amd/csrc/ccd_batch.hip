@@ -162,10 +162,10 @@ __device__ __forceinline__ bool quantumOver(long long deadline) {
 // records back in one coalesced hop.
 constexpr int kIT = 4;
 constexpr int kITLDefault = 8;  // long-column pass width (nonzeros per lane); 4 is the alternative
-constexpr int kBlkIT = 2;       // nonzeros per lane of a block column (one pass of <= 512)
 
 template <int NMAX>
 struct PassLds {
+  static constexpr int kItems = NMAX;  // workgroup size x pass width
   double v1[NMAX];
   double v2[NMAX];
   unsigned char start[NMAX];
@@ -632,13 +632,15 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
     // update changes another's gradient, so this is the sequential result.
     const int B = (F == 1 && MODEL != M_COX && d.blk != nullptr) ? d.blk[j] : 1;
     if (B > 1) {
-      static_assert(kBlkIT * 256 >= kBlockNnz, "a block column must fit one pass");
+      // nonzeros per lane of a block column: one pass of the workgroup covers it
+      constexpr int kLanes = LDS::kItems / ITL;
+      constexpr int kBlkIT = (kBlockNnz + kLanes - 1) / kLanes;
+      static_assert(kBlkIT * kLanes >= kBlockNnz, "a block column must fit one pass");
       const long long t0 = wall_clock64();
       Lanes<kBlkIT, R> L[kBlockCols];
       double v[2 * kBlockCols], bj[kBlockCols], bnd[kBlockCols], delta[kBlockCols];
-      int64_t nAll[kBlockCols], nMine[kBlockCols];
+      int nAll[kBlockCols], nMine[kBlockCols], runsC[kBlockCols];  // <= kBlockNnz each
       bool live[kBlockCols], stg[kBlockCols];
-      long long runsC[kBlockCols];
       bool stop = false;
 #pragma unroll
       for (int c = 0; c < kBlockCols; ++c) {
@@ -657,12 +659,12 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
           b = cb + off[gg.s0];
           e = cb + off[gg.s1];
         }
-        nAll[c] = ce - cb;
-        nMine[c] = e - b;
+        nAll[c] = (int)(ce - cb);
+        nMine[c] = (int)(e - b);
         const int fmtc = d.fmt[jj];
         bj[c] = s.beta[jj];
         bnd[c] = s.bound[jj];
-        runsC[c] = MODEL == M_CLR ? (long long)(d.segPtr[jj + 1] - d.segPtr[jj]) : nAll[c];
+        runsC[c] = MODEL == M_CLR ? (int)(d.segPtr[jj + 1] - d.segPtr[jj]) : nAll[c];
         stg[c] = MODEL == M_CLR && runsC[c] < nAll[c];
         if (threadIdx.x == 0) S.cValid[0] = 0;
         passLoad<MODEL, kBlkIT, R>(s, fmtc, cb, e, b, useW, true, L[c]);
