@@ -406,7 +406,10 @@ def main():
                 and precision == api.FP64 and args.sweeps_per_sync == 0 and args.max_gang == 0
                 and args.bundle in (0, 1) and t.get("threads") == stats["tuned_threads"]
                 and t.get("pass_items") == stats["tuned_pass_items"]):
-            traffic = t["traffic_bytes_per_launch"]
+            # the batch's measured traffic over this run's launches (their number
+            # varies a little with where the time quanta end)
+            traffic = t.get("traffic_bytes_total", t["traffic_bytes_per_launch"] * t["launches"]) \
+                / max(1, stats["launches"])
 
     # plain engine and CPU oracle on a spread of the same fits
     sample = np.linspace(0, n_fits - 1, max(args.plain_fits, args.cpu_fits, 1)).astype(int)
