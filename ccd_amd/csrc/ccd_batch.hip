@@ -2476,18 +2476,51 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
     const TuneKey key{d->model, d->K, d->J, nFit, bundleMax, d->precision,
                       autoThreads ? 0 : clampThreads(threads), maxGang, d->nnz};
     TuneChoice choice{};
+    // the starting point (every slot's state and every fit's status after
+    // set-up), copied aside before the first trial and copied back after each;
+    // without room for the copy, the coefficients are uploaded again and the
+    // set-up kernel re-run instead
+    const size_t arenaBytes = (size_t)nSlot * L.bytes;
+    char* snapArena = nullptr;
+    FitStatus* snapSt = nullptr;
+    bool snapTried = false;
     rc = tuneBatch(
         key, threads, autoThreads, maxGang,
         [&](const TuneChoice& cand, double* rate) -> int {
           threads = cand.threads;
           itl = cand.itl;
           gangCap = cand.gang;
+          if (!snapTried) {
+            snapTried = true;
+            if (hipMalloc((void**)&snapArena, arenaBytes) == hipSuccess &&
+                hipMalloc((void**)&snapSt, nFit * sizeof(FitStatus)) == hipSuccess) {
+              o.ptrs.push_back(snapArena);
+              o.ptrs.push_back(snapSt);
+              HIP_TRY(hipMemcpyAsync(snapArena, arena, arenaBytes, hipMemcpyDeviceToDevice, s));
+              HIP_TRY(hipMemcpyAsync(snapSt, dst, nFit * sizeof(FitStatus),
+                                     hipMemcpyDeviceToDevice, s));
+            } else {
+              (void)hipGetLastError();
+              if (snapArena) (void)hipFree(snapArena);
+              snapArena = nullptr;
+              snapSt = nullptr;
+            }
+          }
           int r = buildMap();
           if (r) return r;
           return timeTrial(mixed, dslots, dst, dmap, mapN, dxch, mapGangs, threads, itl, nFit, J,
                            fit0, fitsOf, o, s, rate);
         },
         [&]() -> int {
+          if (snapSt) {
+            HIP_TRY(hipMemcpyAsync(arena, snapArena, arenaBytes, hipMemcpyDeviceToDevice, s));
+            HIP_TRY(hipMemcpyAsync(dst, snapSt, nFit * sizeof(FitStatus), hipMemcpyDeviceToDevice,
+                                   s));
+            HIP_TRY(hipMemcpyAsync(dactive, active.data(), nSlot * sizeof(int32_t),
+                                   hipMemcpyHostToDevice, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            return 0;
+          }
           // back to the starting point: coefficients, statuses, then the set-up kernel
           for (int b = 0; b < nSlot; ++b) {
             const int r = uploadStart(b, (1u << fitsOf[b]) - 1u,
