@@ -15,7 +15,6 @@
 #include <chrono>
 #include <cmath>
 #include <memory>
-#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -233,8 +232,7 @@ class HostCyclicCoordinateDescent {
 
 using namespace ccdamd;
 
-static std::mutex g_mmStatsMu;
-static ccdh_mm_stats g_mmStats{};
+static thread_local ccdh_mm_stats g_mmStats{};  // the last MM fit on this host thread
 
 static int fitHost(ccdh_data_t data, const ccdh_fit_spec* spec, double* beta_out,
                    ccdh_fit_result* result, bool mm) {
@@ -251,10 +249,7 @@ static int fitHost(ccdh_data_t data, const ccdh_fit_spec* spec, double* beta_out
     HostCyclicCoordinateDescent ccd(ms, J, hasOffset != 0, *spec, K, mm);
     auto keepStats = [&]() {
       ccdh_mm_stats st{};
-      if (mm && ccdh_fit_mm_stats(ms.handle(), &st) == CCDH_OK) {
-        std::lock_guard<std::mutex> lock(g_mmStatsMu);
-        g_mmStats = st;
-      }
+      if (mm && ccdh_fit_mm_stats(ms.handle(), &st) == CCDH_OK) g_mmStats = st;
     };
     try {
       ccd.update();
@@ -304,7 +299,6 @@ extern "C" int ccdh_fit_mm(ccdh_data_t data, const ccdh_fit_spec* spec, double* 
 
 extern "C" int ccdh_mm_last_stats(ccdh_mm_stats* out) {
   if (!out) return setError(CCDH_E_INVALID, "null argument");
-  std::lock_guard<std::mutex> lock(g_mmStatsMu);
   *out = g_mmStats;
   return CCDH_OK;
 }

@@ -243,7 +243,8 @@ typedef struct {
   int64_t rows;
   int values;          /* 1: SPARSE / DENSE values are read (8 B per nonzero) */
 } ccdh_mm_stats;
-/* For one fit handle (all its MM steps so far) and for the last ccdh_fit_mm call. */
+/* For one fit handle (all its MM steps so far) and for the last ccdh_fit_mm call on
+ * this host thread. */
 int ccdh_fit_mm_stats(ccdh_fit_t fit, ccdh_mm_stats* out);
 int ccdh_mm_last_stats(ccdh_mm_stats* out);
 
