@@ -1439,15 +1439,18 @@ int logLik(ccdh_fit* f, int useCV, double* out) {
       return 0;
     }
     case CCDH_CONDITIONAL_LOGISTIC: {
-      rc = rowsReduce(f, RR_LL_NUM, useCV, nullptr, nullptr, nullptr);
-      if (rc) return rc;
-      const double num = f->hres[0];
+      // both reductions, then one read-back: the numerator in slot 0, the
+      // groups' sum in slot 2 (the second kernel starts after the first ends,
+      // so they can share the partials and the ticket)
+      const int K = d->K;
+      LAUNCH_R(k_rows_reduce, gridFor(K), RR_LL_NUM, K, XB(T_), OEB(T_), d->y, f->kw, useCV,
+               nullptr, nullptr, nullptr, f->partials, f->ticket, f->dres);
       k_groups_log<<<gridFor(f->N), kThreads, 0, f->stream>>>(f->N, f->nw, f->denom, f->partials,
-                                                              f->ticket, f->dres);
+                                                              f->ticket, f->dres + 2);
       HIP_TRY(hipGetLastError());
-      rc = fetch(f, 1);
+      rc = fetch(f, 3);
       if (rc) return rc;
-      *out = num - f->hres[0];
+      *out = f->hres[0] - f->hres[2];
       return 0;
     }
     case CCDH_COX: {
