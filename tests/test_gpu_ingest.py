@@ -77,3 +77,15 @@ def test_device_ingest_edge_cases(name, case):
     q = d.problem
     d.close()
     assert_same_problem(q, ingest(**case))
+
+
+def test_fit_over_ingested_mixed_precision_data():
+    # FP32 xBeta / exp streams over an ingested data set: the mixed tolerance of
+    # the FP64 oracle fit of the original problem
+    name, p, t = [c for c in round_trip_problems() if c[0] == "lr_synth"][0]
+    d = api.Data.from_long(**to_long(p, np.random.default_rng(34), time=t), precision=api.MIXED)
+    sp = default_spec(p, prior=PRIOR_LAPLACE, variance=1.0)
+    bg, ig = api.fit_plain(d, sp)
+    d.close()
+    bc, ic = orc.fit(p, sp)
+    check_fit(bg, ig, bc, ic, rtol=2e-4, atol=2e-5, ll_rtol=1e-6, same_path=False)
