@@ -29,6 +29,8 @@ extern "C" int ccdh_cv_fold_weights(const int32_t* ids, int K, int by_pid, int n
                                     int task, double* weights_out) {
   if (!ids || !weights_out || K <= 0 || n_fold <= 0 || task < 0)
     return ccdamd::setError(CCDH_E_INVALID, "bad cross-validation arguments");
+  if (seed == -99)  // the selector's clock seed would differ from task to task
+    return ccdamd::setError(CCDH_E_INVALID, "seed -99 (clock seed): pass an explicit seed");
   int N = K;
   if (by_pid) {
     N = 0;
@@ -47,10 +49,12 @@ extern "C" int ccdh_cv_fold_weights(const int32_t* ids, int K, int by_pid, int n
   start[n_fold] = N;
   std::vector<int> perm(N);
   for (int i = 0; i < N; ++i) perm[i] = i;
+  // Seed -1 makes the selector deterministic (AbstractSelector.cpp:32-48): the
+  // permutation stays the identity (permute, CrossValidationSelector.cpp:140-144).
   std::mt19937 prng;
   prng.seed(seed);
   for (int i = 0; i <= task; ++i)
-    if (i % n_fold == 0) std::shuffle(perm.begin(), perm.end(), prng);
+    if (i % n_fold == 0 && seed != -1) std::shuffle(perm.begin(), perm.end(), prng);
   const int fold = task % n_fold;
   std::fill(weights_out, weights_out + K, 1.0);
   if (by_pid) {
@@ -74,6 +78,8 @@ extern "C" int ccdh_bootstrap_weights(const int32_t* ids, int K, long seed, int 
                                       const double* exclude, int n_exclude, double* weights_out) {
   if (!ids || !weights_out || K <= 0 || replicate < 0 || (exclude && n_exclude < 0))
     return ccdamd::setError(CCDH_E_INVALID, "bad bootstrap arguments");
+  if (seed == -99)
+    return ccdamd::setError(CCDH_E_INVALID, "seed -99 (clock seed): pass an explicit seed");
   int N = 0;
   for (int k = 0; k < K; ++k) {
     if (ids[k] < 0) return ccdamd::setError(CCDH_E_INVALID, "negative id");
@@ -90,8 +96,9 @@ extern "C" int ccdh_bootstrap_weights(const int32_t* ids, int K, long seed, int 
   if (included.empty()) return ccdamd::setError(CCDH_E_INVALID, "no pid to draw");
   const int nIn = (int)included.size();
   std::vector<int> count(std::max(N, n_exclude), 0);
+  // Seed -1 leaves the engine default-constructed (AbstractSelector.cpp:32-48).
   std::mt19937 prng;
-  prng.seed(seed);
+  if (seed != -1) prng.seed(seed);
   for (int sample = 0; sample <= replicate + 1; ++sample) {
     std::uniform_int_distribution<int> uniform(0, nIn - 1);
     std::fill(count.begin(), count.end(), 0);

@@ -258,3 +258,51 @@ def request_stream(n_requests=200, rate=50.0, K_range=(10_000, 200_000),
     idx = rng.integers(0, len(pool), n_requests)
     var = np.exp(rng.uniform(np.log(0.01), np.log(1.0), n_requests))
     return pool, [(float(t[i]), int(idx[i]), float(var[i])) for i in range(n_requests)]
+
+
+def separable(model=LOGISTIC, nrows=10_000, ncovars=5, effect_sd=1.0, zero_prop=0.9,
+              seed=666):
+    """A design with one separable covariate, shaped like the reference's
+    separability tests (tests/testthat/test-largeBernoulli.R:25-101:
+    simulateCyclopsData(nstrata = 1, ncovars = 5), R/Simulation.R:37-97): rows draw
+    Poisson(ncovars/100) distinct covariates, effects are log-normal for a tenth of
+    the covariates; logistic outcomes ~ Bernoulli(U(0.1, 0.3) * rr), survival
+    outcomes with exponential event times (rate U(0.01, 0.03) * rr, 1 + rounded) and
+    uniform censoring (1 + round U(0, 499)).  Covariate ncovars + 1 repeats
+    covariate 3 on the rows with an event, so its likelihood keeps rising as its
+    coefficient grows (the maximum does not exist).  Logistic fits get an intercept;
+    Cox rows are sorted by (-time, y, row)."""
+    rng = np.random.default_rng(seed)
+    sd = effect_sd * (rng.random(ncovars) < 1 - zero_prop)
+    rr = np.exp(rng.normal(0.0, 1.0, ncovars) * sd)
+    per_row = np.minimum(rng.poisson(ncovars / 100.0, nrows), ncovars)
+    rr_row = np.ones(nrows)
+    pairs = []
+    for i in np.flatnonzero(per_row):
+        cols = rng.choice(ncovars, per_row[i], replace=False)
+        rr_row[i] = np.prod(rr[cols])
+        pairs.extend((i, c) for c in cols)
+    r = np.array([a for a, _ in pairs], dtype=np.int64)
+    c = np.array([b for _, b in pairs], dtype=np.int64)
+    if model == LOGISTIC:
+        prob = rng.uniform(0.1, 0.3) * rr_row
+        y = (rng.random(nrows) < prob).astype(np.float64)
+        time = None
+    else:
+        rate = rng.uniform(0.01, 0.03) * rr_row
+        t_out = 1 + np.round(rng.exponential(1.0 / rate))
+        t_cens = 1 + np.round(rng.uniform(0, 499, nrows))
+        y = (t_cens > t_out).astype(np.float64)
+        time = np.where(t_cens < t_out, t_cens, t_out)
+    sep = (c == 2) & (y[r] == 1)
+    r = np.concatenate([r, r[sep]])
+    c = np.concatenate([c, np.full(int(sep.sum()), ncovars)])
+    P = ncovars + 1
+    if model == LOGISTIC:
+        col_ptr, rows = _csc_from_pairs(r, c, nrows, P)
+        return _assemble(LOGISTIC, nrows, P, col_ptr, rows, y, intercept=True)
+    order = np.lexsort((np.arange(nrows), y, -time))
+    inv = np.empty(nrows, dtype=np.int64)
+    inv[order] = np.arange(nrows)
+    col_ptr, rows = _csc_from_pairs(inv[r], c, nrows, P)
+    return _assemble(COX, nrows, P, col_ptr, rows, y[order], offs=time[order])

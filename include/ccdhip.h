@@ -306,13 +306,17 @@ int ccdh_batch_last_stats(ccdh_batch_stats* out);
 /* ---- cross-validation batch definition ---- */
 /* CrossValidationSelector::getWeights after the permutations of `task`
  * (drivers/CrossValidationSelector.cpp:21-116,138-143): ids are the exchangeable
- * unit of each row (stratum for by_pid, ignored otherwise). */
+ * unit of each row (stratum for by_pid, ignored otherwise).  Seed -1 is the
+ * selector's deterministic mode (no shuffle; drivers/AbstractSelector.cpp:32-48);
+ * the clock seed -99 is refused (CCDH_E_INVALID), since each task is a separate call. */
 int ccdh_cv_fold_weights(const int32_t* ids, int K, int by_pid, int n_fold, long seed, int task,
                          double* weights_out);
 /* GridSearchCrossValidationDriver::computeGridPoint (GridSearchCrossValidationDriver.cpp:38-48) */
 double ccdh_cv_grid_point(double lower, double upper, int size, int step);
 
-/* ---- bootstrap batch definition ---- */
+/* ---- bootstrap batch definition ----
+ * Seed -1 leaves the selector's std::mt19937 default-constructed
+ * (drivers/AbstractSelector.cpp:32-48); -99 is refused as above. */
 /* BootstrapSelector::permute + getWeights as BootstrapDriver::drive calls them
  * (drivers/BootstrapSelector.cpp:17-95, drivers/BootstrapDriver.cpp:47-71): the
  * weights of replicate `replicate` (0-based), each row weighing the number of

@@ -900,12 +900,15 @@ int orc_cv_weights(const int32_t* ids, int K, int by_pid, int nfold, long seed, 
     if (i < extra) index++;
   }
   start.push_back(N);
+  // AbstractSelector.cpp:32-48: seed -1 makes the selector deterministic (permute
+  // does not shuffle, CrossValidationSelector.cpp:140-144); reseed (:60-66) seeds
+  // the engine whatever the seed.
   std::mt19937 prng;
   prng.seed(seed);
   std::vector<int> perm(N);
   for (int i = 0; i < N; ++i) perm[i] = i;
   for (int i = 0; i <= task; ++i)
-    if (i % nfold == 0) std::shuffle(perm.begin(), perm.end(), prng);
+    if (i % nfold == 0 && seed != -1) std::shuffle(perm.begin(), perm.end(), prng);
   const int fold = task % nfold;
   for (int k = 0; k < K; ++k) weights_out[k] = 1.0;
   if (by_pid) {
@@ -933,8 +936,9 @@ int orc_bootstrap_weights(const int32_t* ids, int K, long seed, int replicate,
   } else {
     for (int i = 0; i < N; ++i) indicesIncluded.push_back(i);
   }
+  // AbstractSelector.cpp:32-48: seed -1 leaves the engine default-constructed.
   std::mt19937 prng;
-  prng.seed(seed);
+  if (seed != -1) prng.seed(seed);
   std::multiset<int> selected;
   auto permute = [&]() {
     selected.clear();

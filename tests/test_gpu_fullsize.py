@@ -196,3 +196,45 @@ def test_full_clr_long_ingest(clr_full):
     q = d.problem
     d.close()
     assert_same_problem(q, p)
+
+
+def headline_cv_specs(p, folds=10, grid=16, seed=123):
+    """The benchmark's batch: grid step g (Laplace variance log-spaced 1e-4..1)
+    times fold f is fit g * folds + f, fold weights by stratum."""
+    specs, held = [], []
+    for g in range(grid):
+        var = orc.grid_point(1e-4, 1.0, grid, g)
+        for f in range(folds):
+            w = orc.cv_weights(p.strata, folds, seed, f, by_pid=True)
+            specs.append(default_spec(p, prior=PRIOR_LAPLACE, variance=var, weights=w))
+            held.append(1.0 - w)
+    return specs, held
+
+
+def test_full_cv160_batch_matches_oracle_across_the_grid():
+    """The benchmark's own 160-fit batch (its data, seed and default options:
+    autotuned launch shape, gangs re-dealt on a time quantum) against the oracle on
+    the strongest-prior fit (0: variance 1e-4, fold 0), a mid-grid fit (87:
+    variance grid step 8, fold 7) and the weakest-prior fit (159: variance 1, fold
+    9), the batch's critical path, which runs on the largest gangs; held-out
+    scores too.  The oracle's fits come from tests/golden/cv160_oracle.npz
+    (tests/golden/make_cv160.py: the weakest-prior fit takes minutes on one core),
+    checked to be of this very data set."""
+    import os
+    from tests.golden.make_cv160 import fingerprint
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "cv160_oracle.npz"))
+    p, _ = synth.clr_strata()
+    assert np.array_equal(fingerprint(p), ref["fingerprint"])
+    specs, held = headline_cv_specs(p)
+    d = api.Data(p)
+    print("cv160 batch", flush=True)
+    bb, ib = api.fit_batch(d, specs, heldouts=held, autotune=1)
+    st = api.batch_last_stats()
+    d.close()
+    assert max(t[2] for t in st["timeline"]) >= 8  # the weak tier ran on gangs
+    assert ref["iterations"][-1] > 30
+    for k, i in enumerate(ref["fits"]):
+        ic = {n: ref[n][k] for n in ("log_likelihood", "iterations", "return_flag")}
+        check_fit(bb[i], ib[i], ref["beta"][k], ic)
+        np.testing.assert_allclose(ib[i]["pred_log_likelihood"], ref["pred_log_likelihood"][k],
+                                   rtol=1e-8)
