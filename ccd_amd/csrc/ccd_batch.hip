@@ -221,7 +221,7 @@ __device__ __forceinline__ void passLoad(const Slot<R>& s, int fmt, int64_t b, i
     L.w[it] = (MODEL != M_CLR && useW && valid) ? s.kw[rf] : 1.0;
     const GrpSt gs = (MODEL == M_CLR && L.own[it]) ? s.grp[(size_t)L.key[it] * F + f]
                                                    : GrpSt{1.0, 0.0};
-    L.den[it] = gs.den;
+    L.den[it] = MODEL == M_CLR ? gs.den : (valid ? (double)s.rden[rf] : 1.0);
     L.wn[it] = gs.nw;
   }
 }
@@ -310,7 +310,7 @@ __device__ __forceinline__ void passGather(int fmt, LDS& S, int nLocal, bool sta
       if (!L.own[it]) continue;
       const R n1 = L.o[it] * L.x[it];
       if (MODEL == M_LOGISTIC) {
-        const R dd = (R)1 + L.o[it];
+        const R dd = (R)L.den[it];
         const R t = n1 / dd;
         const R hh = ind ? t * ((R)1 - t) : ((n1 * L.x[it]) / dd - t * t);
         v[0] += useW ? L.w[it] * (double)t : (double)t;
@@ -340,6 +340,8 @@ __device__ __forceinline__ void passUpdate(const Slot<R>& s, LDS& S, int nLocal,
     const R nx = L.xo[it] + rd * L.x[it];
     const R no = rexp(nx);
     s.row[(size_t)r * F + f] = RowSt<R>{nx, no};
+    // the row's denominator moves by the change (independent rows)
+    if (MODEL != M_CLR) s.rden[(size_t)r * F + f] = (R)L.den[it] + (no - L.o[it]);
     val[it][0] = (double)no - (double)L.o[it];
   }
   if (MODEL == M_CLR) {
@@ -361,10 +363,10 @@ __device__ __forceinline__ void storeRecords(const Records<R>& rec, int64_t off,
     rec.row[q] = L.row[it];
     rec.o[q] = L.o[it];
     rec.xo[q] = L.xo[it];
+    rec.den[q] = L.den[it];
     if (MODEL == M_CLR) {
       rec.key[q] = L.key[it];
       rec.flags[q] = (L.own[it] ? 1 : 0) | (L.start[it] ? 2 : 0);
-      rec.den[q] = L.den[it];
     }
   }
 }
@@ -391,6 +393,7 @@ __device__ __forceinline__ void loadRecords(const DevX& d, int fmt, int64_t e, i
     } else {
       L.own[it] = valid;
       L.start[it] = valid;
+      L.den[it] = valid ? rec.den[q] : 1.0;
     }
   }
 }
@@ -736,10 +739,10 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
       rec.row += s.recStride;
       rec.o += s.recStride;
       rec.xo += s.recStride;
+      rec.den += s.recStride;
       if (MODEL == M_CLR) {
         rec.key += s.recStride;
         rec.flags += s.recStride;
-        rec.den += s.recStride;
       }
     }
     const int64_t cb = d.colPtr[j], ce = d.colPtr[j + 1];
@@ -1170,8 +1173,8 @@ __device__ __forceinline__ double fitLogLik(const Slot<R>& s, const Part& pt, Ga
     const double w = useW ? s.kw[u] : 1.0;
     double c = (MODEL == M_COX) ? (yi != 1 ? 0.0 : x) : yi * x;
     v[0] += useW ? c * w : c;
-    if (MODEL == M_LOGISTIC) v[1] += w * log(1.0 + (double)rs.e);
-    if (MODEL == M_POISSON) v[1] += w * (double)rs.e;
+    if (MODEL == M_LOGISTIC) v[1] += w * log((double)s.rden[u]);
+    if (MODEL == M_POISSON) v[1] += w * (double)s.rden[u];
   }
   if (MODEL == M_CLR) {
     for (int64_t u = ((int64_t)pt.grpLo << lF) + threadIdx.x; u < ((int64_t)pt.grpHi << lF);
@@ -1360,8 +1363,13 @@ __device__ __forceinline__ void initPart(const Slot<R>& s, bool full, int rank, 
       s.grp[u].den = acc;
     }
   } else {
-    for (int64_t u = ((int64_t)rowLo << lF) + threadIdx.x; u < ((int64_t)rowHi << lF); u += blockDim.x)
-      if (mine) rowSt[u].e = rexp(rowSt[u].x);
+    for (int64_t u = ((int64_t)rowLo << lF) + threadIdx.x; u < ((int64_t)rowHi << lF); u += blockDim.x) {
+      if (!mine) continue;
+      const R o = rexp(rowSt[u].x);
+      rowSt[u].e = o;
+      // the model's null value plus exp(xBeta) (ModelSpecifics.hpp:1588-1612)
+      if (MODEL != M_COX) s.rden[u] = (MODEL == M_LOGISTIC ? (R)1 : (R)0) + o;
+    }
     if (MODEL == M_COX) {
       __syncthreads();
       for (int p = gLo + threadIdx.x; p < gHi; p += blockDim.x) {
@@ -1498,7 +1506,7 @@ __device__ __forceinline__ void finishSlot(const Slot<R>& s, FitStatus* st) {
       const RowSt<R> rs = s.row[u];
       const double x = (double)rs.x;
       const double y = s.d.y[k];
-      if (MODEL == M_LOGISTIC) v[0] += y * w2[u] * (x - log(1.0 + (double)rs.e));
+      if (MODEL == M_LOGISTIC) v[0] += y * w2[u] * (x - log((double)s.rden[u]));
       if (MODEL == M_POISSON) v[0] += (y * x - exp(x)) * w2[u];
       if (MODEL == M_CLR) v[0] += y * w2[u] * (x - log(s.grp[(size_t)s.d.strata[k] * F + f].den));
     }
@@ -1987,6 +1995,8 @@ SlotLayout slotLayout(int model, int K, int J, int nCap, int64_t maxLen, bool he
   const size_t nR = cox ? 1 : (size_t)std::max<int64_t>(maxLen, 1);
   const size_t Fz = (size_t)std::max(1, F);
   L.row = take((size_t)K * 2 * rs * Fz);
+  const bool indep = model == CCDH_LOGISTIC || model == CCDH_POISSON;
+  L.rden = indep ? take((size_t)K * rs * Fz) : kNone;
   L.kw = take((size_t)K * 8 * Fz);
   L.heldout = heldout && !cox ? take((size_t)K * 8 * Fz) : kNone;
   L.grp = grouped ? take(nG * sizeof(GrpSt) * Fz) : kNone;
@@ -2012,7 +2022,7 @@ SlotLayout slotLayout(int model, int K, int J, int nCap, int64_t maxLen, bool he
   const bool clr = model == CCDH_CONDITIONAL_LOGISTIC;
   L.recKey = clr ? take(2 * nRF * 4) : kNone;
   L.recFlags = clr ? take(2 * nRF) : kNone;
-  L.recDen = clr ? take(2 * nRF * 8) : kNone;
+  L.recDen = (clr || indep) ? take(2 * nRF * 8) : kNone;
   L.bytes = off;
   return L;
 }
@@ -2050,6 +2060,7 @@ int slotPrepare(const ccdh_data* d, const ccdh_fit_spec* sps, int F, char* block
   x.lF = F == 8 ? 3 : (F == 4 ? 2 : (F == 2 ? 1 : 0));
   x.fit0 = 0;
   x.row = at<RowSt<R>>(block, L.row);
+  x.rden = at<R>(block, L.rden);
   x.kw = at<double>(block, L.kw);
   x.heldout = sp.heldout ? at<double>(block, L.heldout) : nullptr;
   x.grp = at<GrpSt>(block, L.grp);

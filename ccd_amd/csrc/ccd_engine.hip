@@ -87,12 +87,17 @@ __global__ void k_axpy(DevX d, R* xb, int j, int fmt, double beta) {
   }
 }
 
-// offsExpXBeta for every row (independent models: denominators are 1+oeb / oeb)
+// offsExpXBeta for every row; independent models start each row's denominator at
+// the model's null value plus it (1 + oeb for logistic, oeb for Poisson;
+// computeRemainingStatisticsImpl, ModelSpecifics.hpp:1588-1612); den may be null
 template <typename R>
-__global__ void k_exp_rows(int K, const R* xb, R* oeb) {
+__global__ void k_exp_rows(int K, const R* xb, R* oeb, R* den, R nullValue) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < K;
-       k += (int64_t)gridDim.x * blockDim.x)
-    oeb[k] = rexp(xb[k]);
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const R o = rexp(xb[k]);
+    oeb[k] = o;
+    if (den) den[k] = nullValue + o;
+  }
 }
 
 // grouped: one thread per stratum, rows summed in row order (as the reference)
@@ -160,8 +165,9 @@ __global__ void k_xjy(DevX d, const double* kw, int useW, double* xjy) {
 
 // gradient / Hessian of one coordinate, independent rows (LR, Poisson)
 template <int MODEL, typename R>
-__global__ void k_gh_indep(DevX d, const R* oeb, const double* kw, int j, int fmt, int useW,
-                           const double* xjy, double* partials, unsigned* ticket, double* out) {
+__global__ void k_gh_indep(DevX d, const R* oeb, const R* rden, const double* kw, int j, int fmt,
+                           int useW, const double* xjy, double* partials, unsigned* ticket,
+                           double* out) {
   const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
   const bool ind = isIndicator(fmt);
   double v[2] = {0.0, 0.0};
@@ -173,7 +179,7 @@ __global__ void k_gh_indep(DevX d, const R* oeb, const double* kw, int j, int fm
     const R n1 = o * x;
     const double w = useW ? kw[k] : 1.0;
     if (MODEL == M_LOGISTIC) {
-      const R den = (R)1 + o;
+      const R den = rden[k];
       const R t = n1 / den;
       const R hh = ind ? t * ((R)1 - t) : ((n1 * x) / den - t * t);
       v[0] += useW ? w * (double)t : (double)t;
@@ -362,7 +368,7 @@ __global__ void k_cox_tile_use(int N, int mode, int fmtIndicator, const double* 
 
 // updateXBeta, independent rows
 template <typename R>
-__global__ void k_update_indep(DevX d, R* xb, R* oeb, int j, int fmt, double delta) {
+__global__ void k_update_indep(DevX d, R* xb, R* oeb, R* rden, int j, int fmt, double delta) {
   const int64_t b = d.colPtr[j], e = d.colPtr[j + 1];
   const R rd = (R)delta;
   for (int64_t p = b + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < e;
@@ -370,7 +376,9 @@ __global__ void k_update_indep(DevX d, R* xb, R* oeb, int j, int fmt, double del
     const int k = colRow(d, fmt, p, b);
     const R nx = xb[k] + rd * colValue<R>(d, fmt, p);
     xb[k] = nx;
-    oeb[k] = rexp(nx);
+    const R o = rexp(nx);
+    rden[k] += o - oeb[k];  // the row's denominator moves by the change
+    oeb[k] = o;
   }
 }
 
@@ -452,7 +460,7 @@ enum RowMode : int {
 
 // Row-level sums over K rows: out[0], out[1] (see RowMode)
 template <typename R>
-__global__ void k_rows_reduce(int mode, int K, const R* xb, const R* oeb, const double* y,
+__global__ void k_rows_reduce(int mode, int K, const R* xb, const R* rden, const double* y,
                               const double* kw, int useW, const double* w2,
                               const int32_t* group, const double* gden, double* partials,
                               unsigned* ticket, double* out) {
@@ -469,13 +477,13 @@ __global__ void k_rows_reduce(int mode, int K, const R* xb, const R* oeb, const 
       case RR_LL_LR: {
         const int yi = (int)yy;
         v[0] += useW ? (yi * x) * w : yi * x;
-        v[1] += w * log(1.0 + (double)oeb[k]);
+        v[1] += w * log((double)rden[k]);
         break;
       }
       case RR_LL_POISSON: {
         const int yi = (int)yy;
         v[0] += useW ? (yi * x) * w : yi * x;
-        v[1] += w * (double)oeb[k];
+        v[1] += w * (double)rden[k];
         break;
       }
       case RR_LL_NUM: {
@@ -490,7 +498,7 @@ __global__ void k_rows_reduce(int mode, int K, const R* xb, const R* oeb, const 
         break;
       }
       case RR_PRED_LR:
-        v[0] += yy * w2[k] * (x - log(1.0 + (double)oeb[k]));
+        v[0] += yy * w2[k] * (x - log((double)rden[k]));
         break;
       case RR_PRED_POISSON:
         v[0] += (yy * x - exp(x)) * w2[k];
@@ -1142,6 +1150,10 @@ int fitCreate(ccdh_data* d, ccdh_fit** out) {
   HIP_TRY(hipMalloc(&f->oeb, (size_t)d->K * rs));
   HIP_TRY(hipMemsetAsync(f->xb, 0, (size_t)d->K * rs, f->stream));
   HIP_TRY(hipMemsetAsync(f->oeb, 0, (size_t)d->K * rs, f->stream));
+  if (d->model == CCDH_LOGISTIC || d->model == CCDH_POISSON) {
+    HIP_TRY(hipMalloc(&f->rden, (size_t)d->K * rs));
+    HIP_TRY(hipMemsetAsync(f->rden, 0, (size_t)d->K * rs, f->stream));
+  }
   rc = rc ? rc : devAlloc(&f->kw, d->K, nullptr);
   rc = rc ? rc : devAlloc(&f->w2, d->K, nullptr);
   rc = rc ? rc : devAlloc(&f->xjy, d->J, nullptr);
@@ -1182,7 +1194,7 @@ int fitCreate(ccdh_data* d, ccdh_fit** out) {
 void fitDestroy(ccdh_fit* f) {
   if (!f) return;
   if (f->stream) (void)hipStreamSynchronize(f->stream);
-  void* ptrs[] = {f->xb,  f->oeb,       f->kw,      f->w2,      f->xjy,     f->partials,
+  void* ptrs[] = {f->xb,  f->oeb,  f->rden,  f->kw,      f->w2,      f->xjy,     f->partials,
                   f->ticket, f->dres,   f->denom,   f->nw,      f->numer,   f->numer2,
                   f->accTmp, f->pid,    f->pidFirst, f->resetFlag, f->tileAgg, f->tileCarry,
                   f->mmR,    f->mmPart, f->mmGH,     f->mmBeta,    f->mmFixed};
@@ -1239,6 +1251,7 @@ static int fetch(ccdh_fit* f, int n) {
 
 #define XB(T) ((T*)f->xb)
 #define OEB(T) ((T*)f->oeb)
+#define RDEN(T) ((T*)f->rden)
 
 int setWeights(ccdh_fit* f, const double* w, int useCV) {
   ccdh_data* d = f->d;
@@ -1306,7 +1319,8 @@ int remainingStats(ccdh_fit* f, int useW) {
   if (d->model == CCDH_CONDITIONAL_LOGISTIC) {
     LAUNCH_R(k_remaining_clr, gridFor(d->nStrata, 1), d->view(), XB(T_), OEB(T_), f->denom);
   } else {
-    LAUNCH_R(k_exp_rows, gridFor(K), K, XB(T_), OEB(T_));
+    LAUNCH_R(k_exp_rows, gridFor(K), K, XB(T_), OEB(T_), RDEN(T_),
+             (T_)(d->model == CCDH_LOGISTIC ? 1 : 0));
     if (d->model == CCDH_COX) {
       LAUNCH_R(k_remaining_cox, gridFor(f->N, 1), f->N, f->pid, f->pidFirst, OEB(T_), f->kw, useW,
                f->denom);
@@ -1357,18 +1371,18 @@ int gradHessLaunch(ccdh_fit* f, int j, int useW) {
     case CCDH_LOGISTIC:
       if (f->mixed)
         k_gh_indep<M_LOGISTIC, float><<<gridFor(n), kThreads, 0, f->stream>>>(
-            d->view(), OEB(float), f->kw, j, fmt, useW, f->xjy, f->partials, f->ticket, f->dres);
+            d->view(), OEB(float), RDEN(float), f->kw, j, fmt, useW, f->xjy, f->partials, f->ticket, f->dres);
       else
         k_gh_indep<M_LOGISTIC, double><<<gridFor(n), kThreads, 0, f->stream>>>(
-            d->view(), OEB(double), f->kw, j, fmt, useW, f->xjy, f->partials, f->ticket, f->dres);
+            d->view(), OEB(double), RDEN(double), f->kw, j, fmt, useW, f->xjy, f->partials, f->ticket, f->dres);
       break;
     case CCDH_POISSON:
       if (f->mixed)
         k_gh_indep<M_POISSON, float><<<gridFor(n), kThreads, 0, f->stream>>>(
-            d->view(), OEB(float), f->kw, j, fmt, useW, f->xjy, f->partials, f->ticket, f->dres);
+            d->view(), OEB(float), RDEN(float), f->kw, j, fmt, useW, f->xjy, f->partials, f->ticket, f->dres);
       else
         k_gh_indep<M_POISSON, double><<<gridFor(n), kThreads, 0, f->stream>>>(
-            d->view(), OEB(double), f->kw, j, fmt, useW, f->xjy, f->partials, f->ticket, f->dres);
+            d->view(), OEB(double), RDEN(double), f->kw, j, fmt, useW, f->xjy, f->partials, f->ticket, f->dres);
       break;
     case CCDH_CONDITIONAL_LOGISTIC: {
       const int64_t ns = d->hSegCount(j);
@@ -1404,7 +1418,7 @@ int updateXBeta(ccdh_fit* f, double delta, int j, int useW) {
   switch (d->model) {
     case CCDH_LOGISTIC:
     case CCDH_POISSON:
-      LAUNCH_R(k_update_indep, gridFor(n), d->view(), XB(T_), OEB(T_), j, fmt, delta);
+      LAUNCH_R(k_update_indep, gridFor(n), d->view(), XB(T_), OEB(T_), RDEN(T_), j, fmt, delta);
       break;
     case CCDH_CONDITIONAL_LOGISTIC:
       LAUNCH_R(k_update_clr, gridFor(d->hSegCount(j), 2), d->view(), XB(T_), OEB(T_), f->denom, j,
@@ -1421,7 +1435,7 @@ int updateXBeta(ccdh_fit* f, double delta, int j, int useW) {
 static int rowsReduce(ccdh_fit* f, int mode, int useW, const double* w2, const int32_t* group,
                       const double* gden) {
   const int K = f->d->K;
-  LAUNCH_R(k_rows_reduce, gridFor(K), mode, K, XB(T_), OEB(T_), f->d->y, f->kw, useW, w2, group,
+  LAUNCH_R(k_rows_reduce, gridFor(K), mode, K, XB(T_), RDEN(T_), f->d->y, f->kw, useW, w2, group,
            gden, f->partials, f->ticket, f->dres);
   return fetch(f, 2);
 }
@@ -1443,7 +1457,7 @@ int logLik(ccdh_fit* f, int useCV, double* out) {
       // groups' sum in slot 2 (the second kernel starts after the first ends,
       // so they can share the partials and the ticket)
       const int K = d->K;
-      LAUNCH_R(k_rows_reduce, gridFor(K), RR_LL_NUM, K, XB(T_), OEB(T_), d->y, f->kw, useCV,
+      LAUNCH_R(k_rows_reduce, gridFor(K), RR_LL_NUM, K, XB(T_), RDEN(T_), d->y, f->kw, useCV,
                nullptr, nullptr, nullptr, f->partials, f->ticket, f->dres);
       k_groups_log<<<gridFor(f->N), kThreads, 0, f->stream>>>(f->N, f->nw, f->denom, f->partials,
                                                               f->ticket, f->dres + 2);

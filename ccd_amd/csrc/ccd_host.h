@@ -61,6 +61,10 @@ struct ccdh_fit {
   int nCap = 0;  // allocated group capacity (N+1 <= nCap)
   void* xb = nullptr;
   void* oeb = nullptr;
+  // logistic / Poisson: each row's denominator (1 + exp(xBeta) / exp(xBeta)) as a
+  // running sum of its changes, as the reference keeps denomPid per row
+  // (ModelSpecifics.hpp:1567-1574, 1605-1609); nullptr for the grouped models
+  void* rden = nullptr;
   double* kw = nullptr;
   double* w2 = nullptr;  // scratch weights (held-out scoring)
   double* xjy = nullptr;

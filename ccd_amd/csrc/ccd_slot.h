@@ -89,6 +89,10 @@ struct Slot {
   int F, lF;  // fits, log2 F
   int fit0;   // status index of the first fit
   RowSt<R>* row;
+  // logistic / Poisson: each (row, fit)'s denominator as a running sum of its
+  // changes (the reference's per-row denomPid, ModelSpecifics.hpp:1567-1574);
+  // interleaved like row; nullptr for the grouped models
+  R* rden;
   double* kw;
   double* heldout;  // nullptr when the fits have no held-out weights
   GrpSt* grp;       // groups (grouped models only)
@@ -130,7 +134,7 @@ struct GangEntry {
 
 // Byte offsets of every array of one slot inside its block (256-byte aligned).
 struct SlotLayout {
-  size_t row, kw, heldout, grp, numer, numer2, xjy, beta, bound, pvar, ptype, fixed,
+  size_t row, rden, kw, heldout, grp, numer, numer2, xjy, beta, bound, pvar, ptype, fixed,
       pid, pidFirst, reset, pidShard, recRow, recKey, recFlags, recO, recXo, recDen;
   int64_t recStride;
   size_t bytes;

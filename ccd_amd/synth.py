@@ -306,3 +306,42 @@ def separable(model=LOGISTIC, nrows=10_000, ncovars=5, effect_sd=1.0, zero_prop=
     inv[order] = np.arange(nrows)
     col_ptr, rows = _csc_from_pairs(inv[r], c, nrows, P)
     return _assemble(COX, nrows, P, col_ptr, rows, y[order], offs=time[order])
+
+
+def lr_separated(seed=666, n_sep=60):
+    """Logistic design of separable(LOGISTIC, seed) whose last covariate is replaced
+    by an indicator on `n_sep` rows with an event and no other covariate: the
+    likelihood rises without bound in that coefficient alone, so every fit from
+    zero marches it up until exp(x beta) / (1 + exp(x beta)) rounds to 1, where
+    the gradient and Hessian are both 0 and the step is 0 / 0."""
+    from .data import build_problem
+    p = separable(LOGISTIC, seed=seed)
+    covered = np.zeros(p.K, dtype=bool)
+    for j in range(1, p.J):
+        covered[p.column(j)[0]] = True
+    rng = np.random.default_rng(seed)
+    rows = np.sort(rng.choice(np.flatnonzero((p.y == 1) & ~covered), n_sep, replace=False))
+    cols = [(INDICATOR, p.column(j)[0], None) for j in range(1, p.J - 1)]
+    cols.append((INDICATOR, rows, None))
+    return build_problem(LOGISTIC, cols, p.y, intercept=True)
+
+
+def poisson_overflow(seed=5, K=5000, slope=0.01):
+    """Poisson design whose first covariate is SPARSE with values 500..1000 on 1% of
+    the rows and counts ~ Poisson(exp(slope * x + 0.3 per indicator)) there, three
+    indicator covariates, no intercept.  From zero the first Newton step is clipped
+    to the initial bound 2, x beta reaches 1000..2000 and exp overflows; with the
+    bound divided by 10 the fit descends from x beta ~ 200 and converges."""
+    from .data import SPARSE, build_problem
+    rng = np.random.default_rng(seed)
+    big = np.flatnonzero(rng.random(K) < 0.01)
+    xv = rng.uniform(500.0, 1000.0, len(big))
+    cols = [(SPARSE, big, xv)]
+    eta = np.zeros(K)
+    for _ in range(3):
+        r = np.flatnonzero(rng.random(K) < 0.05)
+        cols.append((INDICATOR, r, None))
+        eta[r] += 0.3
+    eta[big] += slope * xv
+    y = rng.poisson(np.exp(eta)).astype(np.float64)
+    return build_problem(POISSON, cols, y)

@@ -10,7 +10,13 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def names():
-    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+    """Problem fixtures (every npz holding a data set; cv160_oracle.npz holds fits only)."""
+    out = []
+    for p in glob.glob(os.path.join(GOLDEN, "*.npz")):
+        with np.load(p, allow_pickle=False) as z:
+            if "model" in z.files:
+                out.append(os.path.basename(p)[:-4])
+    return sorted(out)
 
 
 def load(name):

