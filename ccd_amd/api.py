@@ -28,7 +28,7 @@ EXPORTS = [
     "ccdh_mm_grad_hess", "ccdh_fit_plain", "ccdh_fit_mm", "ccdh_fit_mm_stats",
     "ccdh_mm_last_stats", "ccdh_batch_fit", "ccdh_batch_last_stats", "ccdh_cv_fold_weights",
     "ccdh_cv_grid_point", "ccdh_bootstrap_weights", "ccdh_search_step", "ccdh_search_quadratic",
-    "ccdh_cox_groups",
+    "ccdh_cox_groups", "ccdh_cox_groups_noalloc",
     "ccdh_server_create", "ccdh_server_submit",
     "ccdh_server_submit_resident", "ccdh_server_poll", "ccdh_server_wait",
     "ccdh_server_get_stats", "ccdh_server_destroy",
@@ -213,6 +213,7 @@ def load_library(path=LIB_PATH):
         "ccdh_search_step": (ci, [vp, vp, vp, ci, cd, cd, cd, cd, vp, vp, vp]),
         "ccdh_search_quadratic": (ci, [vp, vp, vp, ci, vp]),
         "ccdh_cox_groups": (ci, [vp, vp, vp, vp, vp, vp, vp]),
+        "ccdh_cox_groups_noalloc": (ci, [vp, vp, vp, vp, vp, vp, vp]),
         "ccdh_server_create": (ci, [vp, vp]),
         "ccdh_server_submit": (ci, [vp, vp, vp, vp]),
         "ccdh_server_submit_resident": (ci, [vp, vp, vp, vp]),
@@ -452,9 +453,11 @@ def profile_batch(data: Data, spec: FitSpec, index, values, mode_beta, **opts):
     return np.array([r["log_likelihood"] for r in res]), betas, res
 
 
-def cox_groups(data: Data, weights=None):
+def cox_groups(data: Data, weights=None, noalloc=False):
     """Risk-set groups of a Cox data set under `weights`, built on the device:
-    (pid [K], reset [N+1], first row [N+1], first group of each row shard [65], N)."""
+    (pid [K], reset [N+1], first row [N+1], first group of each row shard [65], N).
+    noalloc: through ccdh_cox_groups_noalloc (no device allocation, at most
+    1,048,576 rows)."""
     K = data.problem.K
     w = None if weights is None else np.ascontiguousarray(weights, np.float64)
     pid = np.zeros(K, np.int32)
@@ -462,8 +465,9 @@ def cox_groups(data: Data, weights=None):
     first = np.zeros(K + 1, np.int32)
     shard = np.zeros(65, np.int32)
     n = ctypes.c_int()
-    _check(load_library().ccdh_cox_groups(data.handle, _p(w), _p(pid), _p(reset), _p(first),
-                                          _p(shard), ctypes.byref(n)))
+    lib = load_library()
+    fn = lib.ccdh_cox_groups_noalloc if noalloc else lib.ccdh_cox_groups
+    _check(fn(data.handle, _p(w), _p(pid), _p(reset), _p(first), _p(shard), ctypes.byref(n)))
     N = n.value
     return pid, reset[:N + 1], first[:N + 1], shard, N
 

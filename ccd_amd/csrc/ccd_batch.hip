@@ -2823,6 +2823,124 @@ int coxGroupsOnDevice(ccdh_data* d, const double* weights, int32_t* pidOut, uint
   return 0;
 }
 
+// ------------------------------------------- Cox grouping without allocation
+// ccdh_cox_groups_noalloc stages everything ccdh_cox_groups allocates per call
+// (the slot descriptor, the fit status, the weights and the grouping arrays the
+// kernel writes) in one file-scope device array instead.  Each GPU holds its own
+// copy for the life of the process; its address on a device comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+constexpr int kNoAllocMaxRows = 1 << 20;
+
+struct NoAllocLayout {
+  size_t slot, st, active, kw, pid, pidFirst, reset, numer, numer2, pidShard, bytes;
+};
+
+constexpr size_t naAlign(size_t v) { return (v + 255) & ~(size_t)255; }
+
+constexpr NoAllocLayout noAllocLayout(size_t K) {
+  NoAllocLayout L{};
+  size_t off = 0;
+  const size_t slotBytes =
+      sizeof(Slot<double>) > sizeof(Slot<float>) ? sizeof(Slot<double>) : sizeof(Slot<float>);
+  L.slot = off; off = naAlign(off + slotBytes);
+  L.st = off; off = naAlign(off + sizeof(FitStatus));
+  L.active = off; off = naAlign(off + sizeof(int32_t));
+  L.kw = off; off = naAlign(off + K * sizeof(double));
+  L.pid = off; off = naAlign(off + K * sizeof(int32_t));
+  L.pidFirst = off; off = naAlign(off + (K + 1) * sizeof(int32_t));
+  L.reset = off; off = naAlign(off + (K + 1));
+  L.numer = off; off = naAlign(off + (K + 1) * sizeof(double));
+  L.numer2 = off; off = naAlign(off + (K + 1) * sizeof(double));
+  L.pidShard = off; off = naAlign(off + (kShards + 1) * sizeof(int32_t));
+  L.bytes = off;
+  return L;
+}
+
+constexpr size_t kNoAllocBytes = noAllocLayout(kNoAllocMaxRows).bytes;  // about 33 MiB
+
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+__global__ void k_na_fill(double* a, int n, double v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = v;
+}
+
+std::mutex g_naMu;
+std::map<int, char*> g_naScratch;  // device -> address of its copy of `scratch`
+
+// The address of this device's copy of `scratch` (the current device's), looked
+// up on its first use; the caller holds g_naMu.
+int noAllocScratch(char** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  auto it = g_naScratch.find(dev);
+  if (it == g_naScratch.end()) {
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    it = g_naScratch.emplace(dev, (char*)ptr).first;
+  }
+  *out = it->second;
+  return 0;
+}
+
+template <typename R>
+int coxGroupsNoAlloc(ccdh_data* d, const double* weights, int32_t* pidOut, uint8_t* resetOut,
+                     int32_t* pidFirstOut, int32_t* pidShardOut, int* nOut) {
+  const int K = d->K;
+  std::lock_guard<std::mutex> lock(g_naMu);  // calls in this process take turns
+  char* base = nullptr;
+  int rc = noAllocScratch(&base);
+  if (rc) return rc;
+  const NoAllocLayout L = noAllocLayout(kNoAllocMaxRows);
+  Slot<R> x;
+  std::memset(&x, 0, sizeof(x));
+  x.d = d->view();
+  x.model = d->model;
+  x.K = K;
+  x.J = d->J;
+  x.N = K;
+  x.F = 1;
+  x.lF = 0;
+  x.fit0 = 0;
+  x.kw = (double*)(base + L.kw);
+  x.pid = (int32_t*)(base + L.pid);
+  x.pidFirst = (int32_t*)(base + L.pidFirst);
+  x.reset = (uint8_t*)(base + L.reset);
+  x.numer = (double*)(base + L.numer);
+  x.numer2 = (double*)(base + L.numer2);
+  x.pidShard = (int32_t*)(base + L.pidShard);
+  FitStatus st;
+  std::memset(&st, 0, sizeof(st));
+  const int32_t zero = 0;
+  // the null stream: ordered after every earlier call of this thread on the device
+  hipStream_t s = nullptr;
+  HIP_TRY(hipMemcpyAsync(base + L.slot, &x, sizeof(x), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(base + L.st, &st, sizeof(st), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(base + L.active, &zero, sizeof(zero), hipMemcpyHostToDevice, s));
+  if (weights) {
+    HIP_TRY(hipMemcpyAsync(x.kw, weights, (size_t)K * sizeof(double), hipMemcpyHostToDevice, s));
+  } else {
+    k_na_fill<<<std::min((K + 255) / 256, 1024), 256, 0, s>>>(x.kw, K, 1.0);
+    HIP_TRY(hipGetLastError());
+  }
+  rc = slotLaunchCoxGroups(std::is_same<R, float>::value, base + L.slot, (FitStatus*)(base + L.st),
+                           (const int32_t*)(base + L.active), 1, 256, s);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(&st, base + L.st, sizeof(st), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  const int N = st.N;
+  HIP_TRY(hipMemcpyAsync(pidOut, x.pid, (size_t)K * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  if (resetOut) HIP_TRY(hipMemcpyAsync(resetOut, x.reset, N + 1, hipMemcpyDeviceToHost, s));
+  if (pidFirstOut)
+    HIP_TRY(hipMemcpyAsync(pidFirstOut, x.pidFirst, (size_t)(N + 1) * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, s));
+  if (pidShardOut)
+    HIP_TRY(hipMemcpyAsync(pidShardOut, x.pidShard, (kShards + 1) * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  *nOut = N;
+  return 0;
+}
+
 }  // namespace
 
 extern "C" int ccdh_cox_groups(ccdh_data_t data, const double* weights, int32_t* pid_out,
@@ -2836,6 +2954,26 @@ extern "C" int ccdh_cox_groups(ccdh_data_t data, const double* weights, int32_t*
                                           shard_pid_out, n_out)
                : coxGroupsOnDevice<double>(data, weights, pid_out, reset_out, pid_first_out,
                                            shard_pid_out, n_out);
+  } catch (const std::exception& e) {
+    return setError(CCDH_E_DEVICE, e.what());
+  }
+}
+
+extern "C" int ccdh_cox_groups_noalloc(ccdh_data_t data, const double* weights,
+                                       int32_t* pid_out, uint8_t* reset_out,
+                                       int32_t* pid_first_out, int32_t* shard_pid_out,
+                                       int* n_out) {
+  if (!data || !pid_out || !n_out) return setError(CCDH_E_INVALID, "null argument");
+  if (data->model != CCDH_COX) return setError(CCDH_E_INVALID, "not a Cox data set");
+  if (data->K > kNoAllocMaxRows)
+    return setError(CCDH_E_INVALID, "ccdh_cox_groups_noalloc: " + std::to_string(data->K) +
+                                        " rows, at most " + std::to_string(kNoAllocMaxRows));
+  try {
+    return data->precision == CCDH_MIXED
+               ? coxGroupsNoAlloc<float>(data, weights, pid_out, reset_out, pid_first_out,
+                                         shard_pid_out, n_out)
+               : coxGroupsNoAlloc<double>(data, weights, pid_out, reset_out, pid_first_out,
+                                          shard_pid_out, n_out);
   } catch (const std::exception& e) {
     return setError(CCDH_E_DEVICE, e.what());
   }

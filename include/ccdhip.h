@@ -335,6 +335,18 @@ int ccdh_bootstrap_weights(const int32_t* ids, int K, long seed, int replicate,
 int ccdh_cox_groups(ccdh_data_t data, const double* weights, int32_t* pid_out, uint8_t* reset_out,
                     int32_t* pid_first_out, int32_t* shard_pid_out, int* n_out);
 
+/* The same groups as ccdh_cox_groups, computed without allocating device memory.
+ * No counterpart in the reference: for callers that cannot allocate (code that
+ * must not call hipMalloc, such as a serving loop with a fixed memory budget).
+ * Everything ccdh_cox_groups allocates per call is staged in one file-scope
+ * device array of about 33 MiB that each GPU holds until the process ends, so
+ * data sets of at most 1,048,576 rows are accepted (CCDH_E_INVALID beyond).
+ * Calls in one process take turns; calls that overlap on one GPU from several
+ * processes race on the array. */
+int ccdh_cox_groups_noalloc(ccdh_data_t data, const double* weights, int32_t* pid_out,
+                            uint8_t* reset_out, int32_t* pid_first_out, int32_t* shard_pid_out,
+                            int* n_out);
+
 /* ---- auto-search cross-validation ---- */
 /* UniModalSearch::tried for each (xs[i], ys[i], sds[i]) in order, then ::step
  * (utils/HParSearch.h:30-65, utils/HParSearch.cpp:22-120) as
