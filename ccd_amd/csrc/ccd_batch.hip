@@ -946,82 +946,184 @@ __device__ __forceinline__ void pidBlock(int pLo, int pHi, int* a, int* z) {
   *z = pLo + min(n, ((int)threadIdx.x + 1) * per);
 }
 
+// ---- risk-set scans over a rank's pids, one contiguous stretch per wave
+// Wave w of the workgroup owns the pids [a, z) of the rank's range (whole steps
+// of kWave * kScanItems pids); it walks them step by step, lane l taking the
+// kScanItems consecutive pids q0 + l * kScanItems + [0, kScanItems), so one
+// step's loads of a wave cover one contiguous stretch of every pid array.  A
+// lane folds its pids in order, a wave scan over the lanes' folds orders the
+// step, and the step's total carries into the next step.
+constexpr int kScanItems = 4;
+
+__device__ __forceinline__ void waveRange(int pLo, int pHi, int* a, int* z) {
+  const int nw = ((int)blockDim.x + kWave - 1) / kWave;
+  const int wid = (int)threadIdx.x / kWave;
+  const int n = pHi - pLo;
+  const int step = kWave * kScanItems;
+  const int per = ((n + nw - 1) / nw + step - 1) / step * step;
+  *a = pLo + min(n, wid * per);
+  *z = pLo + min(n, (wid + 1) * per);
+}
+
+template <int NV>
+__device__ __forceinline__ Seg<NV> waveScanIncl(Seg<NV> x) {
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const Seg<NV> o = segShflUp(x, off);
+    if (lane >= off) x = segCombine(o, x);
+  }
+  return x;
+}
+
+template <int NV>
+__device__ __forceinline__ Seg<NV> segFromLane(const Seg<NV>& a, int src) {
+  Seg<NV> r;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) r.v[i] = __shfl(a.v[i], src, kWave);
+  r.f = __shfl(a.f, src, kWave);
+  return r;
+}
+
+// Each wave's total -> the combination of the earlier waves' totals (this wave's
+// carry) and of all of them (`total`), in wave order.
+template <int NV>
+__device__ __forceinline__ Seg<NV> waveCarries(const Seg<NV>& waveTotal, Seg<NV>& total) {
+  __shared__ Seg<NV> wsh[kMaxBlockThreads / kWave];
+  const int wid = threadIdx.x / kWave;
+  const int nw = (blockDim.x + kWave - 1) / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) wsh[wid] = waveTotal;
+  __syncthreads();
+  Seg<NV> carry = segZero<NV>();
+  for (int w = 0; w < wid; ++w) carry = segCombine(carry, wsh[w]);
+  Seg<NV> tot = carry;
+  for (int w = wid; w < nw; ++w) tot = segCombine(tot, wsh[w]);
+  total = tot;
+  __syncthreads();
+  return carry;
+}
+
+// Segmented sums without segments: a fit whose rows form one stratum has no
+// reset after its first pid, and its scans add plainly.
+template <int NV, bool SEGS>
+__device__ __forceinline__ Seg<NV> scanCombine(const Seg<NV>& a, const Seg<NV>& b) {
+  if constexpr (SEGS) {
+    return segCombine(a, b);
+  } else {
+    Seg<NV> r;
+    r.f = 0;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) r.v[i] = a.v[i] + b.v[i];
+    return r;
+  }
+}
+
+template <int NV, bool SEGS>
+__device__ __forceinline__ Seg<NV> waveScanInclS(Seg<NV> x) {
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const Seg<NV> o = segShflUp(x, off);
+    if (lane >= off) x = scanCombine<NV, SEGS>(o, x);
+  }
+  return x;
+}
+
 // Gradient and Hessian terms of one coordinate over this rank's pids [pLo, pHi)
 // (computeGradientAndHessianImpl's cumulative branch, ModelSpecifics.hpp:904-1020,
 // with BreslowTiedCoxProportionalHazards::incrementGradientAndHessian,
 // ModelSpecifics.h:1185-1202).  The accumulated denominator and numerators are a
 // segmented prefix sum over pids that restarts at each stratum
-// (computeAccumlatedDenominator, ModelSpecifics.hpp:1670-1694).  Each thread
-// owns a contiguous block of pids: pass 1 folds its block into one segmented
-// total, a block scan of the totals gives each thread its carry (and a gang
-// scan across ranks the rank's carry), and pass 2 walks the block again with the
-// running sums, adding nW*t and the Hessian term to v and clearing the
-// numerators for the next coordinate.  Two block barriers per coordinate.
-template <typename R>
-__device__ __forceinline__ bool coxScan(const Slot<R>& s, int pLo, int pHi, bool ind, Gang& gg,
-                                        double (&v)[2]) {
+// (computeAccumlatedDenominator, ModelSpecifics.hpp:1670-1694).  Pass 1 folds
+// each wave's stretch into its segmented total; the waves' totals give each
+// wave its carry and the rank's total, and a gang scan across ranks the rank's
+// carry; pass 2 walks the stretch again with the running sums, adding nW*t and
+// the Hessian term to v for the pids with events (nW = 0 adds exactly 0) and
+// clearing the numerators for the next coordinate.  NV = 2 (indicator column:
+// denominator, numerator) or 3 (with the second numerator); SEGS = false when the
+// fit has one stratum (no reset after its first pid).
+template <typename R, int NV, bool SEGS>
+__device__ __forceinline__ bool coxScanT(const Slot<R>& s, int pLo, int pHi, Gang& gg,
+                                         double (&v)[2]) {
   const GrpSt* grp = s.grp;
   double* numer = s.numer;
   double* numer2 = s.numer2;
   const uint8_t* reset = s.reset;
+  const int lane = threadIdx.x & (kWave - 1);
+  constexpr int kStep = kWave * kScanItems;
+  auto load = [&](int i, Seg<NV>& x) {
+    x.f = SEGS ? reset[i] : 0;
+    x.v[0] = grp[i].den;
+    x.v[1] = numer[i];
+    if constexpr (NV > 2) x.v[2] = numer2[i];
+  };
   int a, z;
-  pidBlock(pLo, pHi, &a, &z);
-  Seg<3> agg = segZero<3>();
-  for (int i0 = a; i0 < z; i0 += kItems) {
-    Seg<3> it[kItems];
+  waveRange(pLo, pHi, &a, &z);
+  Seg<NV> wagg = segZero<NV>();
+  for (int q0 = a; q0 < z; q0 += kStep) {
+    const int i0 = q0 + lane * kScanItems;
+    Seg<NV> l = segZero<NV>();
 #pragma unroll
-    for (int u = 0; u < kItems; ++u) {
-      const int i = i0 + u;
-      it[u] = segZero<3>();
-      if (i < z) {
-        it[u].f = reset[i];
-        it[u].v[0] = grp[i].den;
-        it[u].v[1] = numer[i];
-        it[u].v[2] = ind ? 0.0 : numer2[i];
+    for (int u = 0; u < kScanItems; ++u) {
+      if (i0 + u < z) {
+        Seg<NV> x;
+        load(i0 + u, x);
+        l = scanCombine<NV, SEGS>(l, x);
       }
     }
-#pragma unroll
-    for (int u = 0; u < kItems; ++u) agg = segCombine(agg, it[u]);
+    wagg = scanCombine<NV, SEGS>(wagg, segFromLane(waveScanInclS<NV, SEGS>(l), kWave - 1));
   }
-  Seg<3> total;
-  Seg<3> run = blockSegScanExcl<3>(agg, total);
+  Seg<NV> total;
+  Seg<NV> run = waveCarries<NV>(wagg, total);
   if (gg.G > 1) {
-    Seg<3> carry;
-    if (!gangCarry<3>(gg, total, carry)) return false;
-    run = segCombine(carry, run);
+    Seg<NV> carry;
+    if (!gangCarry<NV>(gg, total, carry)) return false;
+    run = scanCombine<NV, SEGS>(carry, run);
   }
-  for (int i0 = a; i0 < z; i0 += kItems) {
-    Seg<3> it[kItems];
-    double wn[kItems];
+  for (int q0 = a; q0 < z; q0 += kStep) {
+    const int i0 = q0 + lane * kScanItems;
+    Seg<NV> it[kScanItems];
+    Seg<NV> l = segZero<NV>();
 #pragma unroll
-    for (int u = 0; u < kItems; ++u) {
-      const int i = i0 + u;
-      it[u] = segZero<3>();
-      wn[u] = 0.0;
-      if (i < z) {
-        const GrpSt gs = grp[i];
-        it[u].f = reset[i];
-        it[u].v[0] = gs.den;
-        it[u].v[1] = numer[i];
-        it[u].v[2] = ind ? 0.0 : numer2[i];
-        wn[u] = gs.nw;
-      }
+    for (int u = 0; u < kScanItems; ++u) {
+      it[u] = segZero<NV>();
+      if (i0 + u < z) load(i0 + u, it[u]);
+      l = scanCombine<NV, SEGS>(l, it[u]);
     }
+    const Seg<NV> inc = waveScanInclS<NV, SEGS>(l);
+    Seg<NV> r = segShflUp(inc, 1);
+    if (lane == 0) r = segZero<NV>();
+    r = scanCombine<NV, SEGS>(run, r);
 #pragma unroll
-    for (int u = 0; u < kItems; ++u) {
+    for (int u = 0; u < kScanItems; ++u) {
       const int i = i0 + u;
       if (i >= z) break;
-      run = segCombine(run, it[u]);
-      const double accD = run.v[0];
-      const double t = run.v[1] / accD;
-      const double g = wn[u] * t;
-      v[0] += g;
-      v[1] += ind ? g * (1.0 - t) : wn[u] * (run.v[2] / accD - t * t);
+      r = scanCombine<NV, SEGS>(r, it[u]);
+      const double wn = grp[i].nw;
+      if (wn != 0.0) {
+        const double accD = r.v[0];
+        const double t = r.v[1] / accD;
+        const double g = wn * t;
+        v[0] += g;
+        if constexpr (NV > 2) v[1] += wn * (r.v[2] / accD - t * t);
+        else v[1] += g * (1.0 - t);
+      }
       if (it[u].v[1] != 0.0) numer[i] = 0.0;
-      if (!ind && it[u].v[2] != 0.0) numer2[i] = 0.0;
+      if constexpr (NV > 2) {
+        if (it[u].v[2] != 0.0) numer2[i] = 0.0;
+      }
     }
+    run = scanCombine<NV, SEGS>(run, segFromLane(inc, kWave - 1));
   }
   return true;
+}
+
+template <typename R>
+__device__ __forceinline__ bool coxScan(const Slot<R>& s, int pLo, int pHi, bool ind, Gang& gg,
+                                        double (&v)[2]) {
+  const bool segs = s.d.nStrata > 1;
+  if (ind) return segs ? coxScanT<R, 2, true>(s, pLo, pHi, gg, v) : coxScanT<R, 2, false>(s, pLo, pHi, gg, v);
+  return segs ? coxScanT<R, 3, true>(s, pLo, pHi, gg, v) : coxScanT<R, 3, false>(s, pLo, pHi, gg, v);
 }
 
 // Sum of nW_i * log(accumulated denominator_i) over this rank's pids (the Cox
@@ -1032,30 +1134,63 @@ template <typename R>
 __device__ __forceinline__ double coxLogDen(const Slot<R>& s, int pLo, int pHi, Gang* gg) {
   const GrpSt* grp = s.grp;
   const uint8_t* reset = s.reset;
+  const int lane = threadIdx.x & (kWave - 1);
+  constexpr int kStep = kWave * kScanItems;
   int a, z;
-  pidBlock(pLo, pHi, &a, &z);
-  Seg<1> agg = segZero<1>();
-  for (int i = a; i < z; ++i) {
-    Seg<1> x;
-    x.f = reset[i];
-    x.v[0] = grp[i].den;
-    agg = segCombine(agg, x);
+  waveRange(pLo, pHi, &a, &z);
+  Seg<1> wagg = segZero<1>();
+  for (int q0 = a; q0 < z; q0 += kStep) {
+    const int i0 = q0 + lane * kScanItems;
+    Seg<1> l = segZero<1>();
+#pragma unroll
+    for (int u = 0; u < kScanItems; ++u) {
+      const int i = i0 + u;
+      if (i < z) {
+        Seg<1> x;
+        x.f = reset[i];
+        x.v[0] = grp[i].den;
+        l = segCombine(l, x);
+      }
+    }
+    wagg = segCombine(wagg, segFromLane(waveScanIncl(l), kWave - 1));
   }
   Seg<1> total;
-  Seg<1> run = blockSegScanExcl<1>(agg, total);
+  Seg<1> run = waveCarries<1>(wagg, total);
   if (gg != nullptr && gg->G > 1) {
     Seg<1> carry;
     if (!gangCarry<1>(*gg, total, carry)) return 0.0;
     run = segCombine(carry, run);
   }
   double acc = 0.0;
-  for (int i = a; i < z; ++i) {
-    const GrpSt gs = grp[i];
-    Seg<1> x;
-    x.f = reset[i];
-    x.v[0] = gs.den;
-    run = segCombine(run, x);
-    if (gs.nw != 0.0) acc += gs.nw * log(run.v[0]);
+  for (int q0 = a; q0 < z; q0 += kStep) {
+    const int i0 = q0 + lane * kScanItems;
+    Seg<1> it[kScanItems];
+    double wn[kScanItems];
+    Seg<1> l = segZero<1>();
+#pragma unroll
+    for (int u = 0; u < kScanItems; ++u) {
+      const int i = i0 + u;
+      it[u] = segZero<1>();
+      wn[u] = 0.0;
+      if (i < z) {
+        const GrpSt gs = grp[i];
+        it[u].f = reset[i];
+        it[u].v[0] = gs.den;
+        wn[u] = gs.nw;
+      }
+      l = segCombine(l, it[u]);
+    }
+    const Seg<1> inc = waveScanIncl(l);
+    Seg<1> r = segShflUp(inc, 1);
+    if (lane == 0) r = segZero<1>();
+    r = segCombine(run, r);
+#pragma unroll
+    for (int u = 0; u < kScanItems; ++u) {
+      if (i0 + u >= z) break;
+      r = segCombine(r, it[u]);
+      if (wn[u] != 0.0) acc += wn[u] * log(r.v[0]);
+    }
+    run = segCombine(run, segFromLane(inc, kWave - 1));
   }
   return acc;
 }
@@ -1075,6 +1210,7 @@ __device__ __forceinline__ int cycleCox(const Slot<R>& s, FitStatus& cs, int j0,
   const int pLo = multi ? s.pidShard[gg.s0] : 0;
   const int pHi = multi ? s.pidShard[gg.s1] : s.N;
   long long cG = 0, cU = 0, cP = 0;
+  long long tN = 0, tS = 0, tU = 0;  // device ticks: numerators, scan, exchange + update
   int jEnd = s.J;
   for (int j = j0; j < s.J; ++j) {
     if (!multi && j > j0 && (j & 3) == 0 && quantumOver(deadline)) {
@@ -1098,10 +1234,14 @@ __device__ __forceinline__ int cycleCox(const Slot<R>& s, FitStatus& cs, int j0,
     const int pt = s.ptype[j];
     double v[2] = {0.0, 0.0};
     bool stop = false;
+    const long long t0 = wall_clock64();
+    long long t1 = t0, t2 = t0;
     if (njAll > 0) {
       coxNumerators<R>(s, fmt, cb, b, e, useW);
       __syncthreads();
+      t1 = wall_clock64();
       if (!coxScan<R>(s, pLo, pHi, isIndicator(fmt), gg, v)) return -1;
+      t2 = wall_clock64();
       cP += pHi - pLo;
     }
     blockSum<2>(v);
@@ -1128,6 +1268,9 @@ __device__ __forceinline__ int cycleCox(const Slot<R>& s, FitStatus& cs, int j0,
       cU += e - b;
     }
     __syncthreads();
+    tN += t1 - t0;
+    tS += t2 - t1;
+    tU += wall_clock64() - t2;
     if (stop) {
       jEnd = j + 1;
       break;
@@ -1137,6 +1280,11 @@ __device__ __forceinline__ int cycleCox(const Slot<R>& s, FitStatus& cs, int j0,
     atomicAdd((unsigned long long*)&cs.gatherNnz, (unsigned long long)cG);
     atomicAdd((unsigned long long*)&cs.updateNnz, (unsigned long long)cU);
     atomicAdd((unsigned long long*)&cs.scanPids, (unsigned long long)cP);
+    if (lead) {
+      atomicAdd((unsigned long long*)&cs.tSmall, (unsigned long long)tN);
+      atomicAdd((unsigned long long*)&cs.tLarge, (unsigned long long)tS);
+      atomicAdd((unsigned long long*)&cs.nLarge, (unsigned long long)tU);
+    }
   }
   return jEnd;
 }
@@ -1551,7 +1699,10 @@ __global__ void k_slot_objective(const Slot<R>* slots, FitStatus* st, const int3
   }
 }
 
-template <typename R, int BLOCK, int ITL>
+// ONLY: -1 dispatches on each slot's model (launches that mix models: the
+// server); a model id builds that model's path alone (a batch over one data
+// set), which needs fewer registers than all four together.
+template <typename R, int BLOCK, int ITL, int ONLY>
 __global__ void __launch_bounds__(BLOCK)
     k_slot_sweep(const Slot<R>* slots, FitStatus* st, const GangEntry* map, Xch* xch,
                  int sweeps, long long quantum) {
@@ -1598,11 +1749,15 @@ __global__ void __launch_bounds__(BLOCK)
     gg.rowLo = 0;
     gg.rowHi = s.K;
   }
-  switch (s.model) {
-    case M_LOGISTIC: sweepSlot<M_LOGISTIC, R, ITL>(s, fs, S, sweeps, deadline, gg); break;
-    case M_POISSON: sweepSlot<M_POISSON, R, ITL>(s, fs, S, sweeps, deadline, gg); break;
-    case M_CLR: sweepSlot<M_CLR, R, ITL>(s, fs, S, sweeps, deadline, gg); break;
-    case M_COX: sweepSlot<M_COX, R, ITL>(s, fs, S, sweeps, deadline, gg); break;
+  if constexpr (ONLY >= 0) {
+    sweepSlot<ONLY, R, ITL>(s, fs, S, sweeps, deadline, gg);
+  } else {
+    switch (s.model) {
+      case M_LOGISTIC: sweepSlot<M_LOGISTIC, R, ITL>(s, fs, S, sweeps, deadline, gg); break;
+      case M_POISSON: sweepSlot<M_POISSON, R, ITL>(s, fs, S, sweeps, deadline, gg); break;
+      case M_CLR: sweepSlot<M_CLR, R, ITL>(s, fs, S, sweeps, deadline, gg); break;
+      case M_COX: sweepSlot<M_COX, R, ITL>(s, fs, S, sweeps, deadline, gg); break;
+    }
   }
 }
 
@@ -1643,9 +1798,24 @@ __global__ void k_cox_groups(Slot<R>* slots, FitStatus* st, const int32_t* activ
   }
   __syncthreads();
   const int carry = sLast[threadIdx.x];
+  // the row shard holding row k (shards start at tie groups)
+  auto shardOf = [&](int k) {
+    int lo = 0, hi = d.nShards;  // shardRow[lo] <= k < shardRow[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) / 2;
+      if (d.shardRow[mid] <= k) lo = mid;
+      else hi = mid;
+    }
+    return lo;
+  };
+  const bool pieces = s.coxPieces != 0;
   auto starts = [&](int prev, int k) {
-    return prev < 0 || d.strata[k] != d.strata[prev] ||
-           !(d.y[prev] == 1.0 && d.time[prev] == d.time[k] && d.y[k] == 1.0);
+    const bool newPid = prev < 0 || d.strata[k] != d.strata[prev] ||
+                        !(d.y[prev] == 1.0 && d.time[prev] == d.time[k] && d.y[k] == 1.0);
+    if (!pieces || !newPid) return newPid;
+    // a piece ends with an event group, at a stratum's end or at a shard's end
+    return prev < 0 || d.strata[k] != d.strata[prev] || d.y[prev] == 1.0 ||
+           shardOf(prev) != shardOf(k);
   };
   int cnt = 0;
   int prev = carry;
@@ -1781,26 +1951,50 @@ int initRanks(int64_t nnz, int K) {
 }
 
 // Sweep kernel variants: (256 lanes, pass width 8 or 4), (512 lanes, width 8).
-static const void* sweepKernel(bool mixed, int threads, int itl) {
-  if (clampThreads(threads) == 512 && itl == 4)
-    return mixed ? (const void*)k_slot_sweep<float, 512, 4> : (const void*)k_slot_sweep<double, 512, 4>;
-  if (clampThreads(threads) == 512)
-    return mixed ? (const void*)k_slot_sweep<float, 512, 8> : (const void*)k_slot_sweep<double, 512, 8>;
-  if (itl == 4)
-    return mixed ? (const void*)k_slot_sweep<float, 256, 4> : (const void*)k_slot_sweep<double, 256, 4>;
-  return mixed ? (const void*)k_slot_sweep<float, 256, 8> : (const void*)k_slot_sweep<double, 256, 8>;
+template <typename R, int ONLY>
+static const void* sweepKernelOf(int threads, int itl) {
+  const bool t512 = clampThreads(threads) == 512;
+  if constexpr (ONLY == M_COX) {
+    return t512 ? (const void*)k_slot_sweep<R, 512, 1, M_COX> : (const void*)k_slot_sweep<R, 256, 1, M_COX>;
+  } else if constexpr (ONLY >= 0) {
+    if (t512 && itl == 4) return (const void*)k_slot_sweep<R, 512, 4, ONLY>;
+    if (!t512 && itl == 8) return (const void*)k_slot_sweep<R, 256, 8, ONLY>;
+    return nullptr;
+  } else {
+    if (t512) return itl == 4 ? (const void*)k_slot_sweep<R, 512, 4, -1> : (const void*)k_slot_sweep<R, 512, 8, -1>;
+    return itl == 4 ? (const void*)k_slot_sweep<R, 256, 4, -1> : (const void*)k_slot_sweep<R, 256, 8, -1>;
+  }
+}
+
+// The sweep kernel for a launch whose slots are all of model `only` (-1: any
+// mix of models); a shape without a one-model build takes the general kernel.
+template <typename R>
+static const void* sweepKernelR(int threads, int itl, int only) {
+  const void* fn = nullptr;
+  switch (only) {
+    case M_LOGISTIC: fn = sweepKernelOf<R, M_LOGISTIC>(threads, itl); break;
+    case M_POISSON: fn = sweepKernelOf<R, M_POISSON>(threads, itl); break;
+    case M_CLR: fn = sweepKernelOf<R, M_CLR>(threads, itl); break;
+    case M_COX: fn = sweepKernelOf<R, M_COX>(threads, itl); break;
+    default: break;
+  }
+  return fn ? fn : sweepKernelOf<R, -1>(threads, itl);
+}
+
+static const void* sweepKernel(bool mixed, int threads, int itl, int only = -1) {
+  return mixed ? sweepKernelR<float>(threads, itl, only) : sweepKernelR<double>(threads, itl, only);
 }
 
 int slotLaunchSweeps(bool mixed, const void* slots, FitStatus* st, const GangEntry* map, int n,
                      Xch* xch, bool gangs, int threads, int itl, int sweeps,
-                     long long quantumTicks, hipStream_t s) {
+                     long long quantumTicks, hipStream_t s, int only) {
   if (n <= 0) return 0;
   if (gangs) {
     k_gang_reset<<<(n + 255) / 256, 256, 0, s>>>(map, n, xch);
     HIP_TRY(hipGetLastError());
   }
   long long q = quantumTicks;
-  const void* fn = sweepKernel(mixed, threads, itl);
+  const void* fn = sweepKernel(mixed, threads, itl, only);
   const int block = clampThreads(threads);
   void* args[] = {(void*)&slots, (void*)&st, (void*)&map, (void*)&xch, (void*)&sweeps, (void*)&q};
   if (gangs) {
@@ -2076,6 +2270,7 @@ int slotPrepare(const ccdh_data* d, const ccdh_fit_spec* sps, int F, char* block
   x.pidFirst = at<int32_t>(block, L.pidFirst);
   x.reset = at<uint8_t>(block, L.reset);
   x.pidShard = at<int32_t>(block, L.pidShard);
+  x.coxPieces = 1;
   x.rec.row = at<int32_t>(block, L.recRow);
   x.rec.key = at<int32_t>(block, L.recKey);
   x.rec.flags = at<uint8_t>(block, L.recFlags);
@@ -2210,10 +2405,10 @@ constexpr long long kTrialTicks = 5000000;  // 50 ms of device time per candidat
 int timeTrial(bool mixed, const void* dslots, FitStatus* dst, const GangEntry* dmap, int mapN,
               Xch* dxch, bool gangs, int threads, int itl, int nFit, int J,
               const std::vector<int>& fit0, const std::vector<int>& fitsOf, Owned& o,
-              hipStream_t s, double* rate) {
+              hipStream_t s, double* rate, int model) {
   HIP_TRY(hipEventRecord(o.ev0, s));
   int rc = slotLaunchSweeps(mixed, dslots, dst, dmap, mapN, dxch, gangs, threads, itl, 1 << 30,
-                            kTrialTicks, s);
+                            kTrialTicks, s, model);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(o.ev1, s));
   HIP_TRY(hipMemcpyAsync(o.hst, dst, nFit * sizeof(FitStatus), hipMemcpyDeviceToHost, s));
@@ -2452,7 +2647,7 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
   // sweep launches, optionally replayed from a captured graph
   auto enqueueSweeps = [&]() -> int {
     int r = slotLaunchSweeps(mixed, dslots, dst, dmap, mapN, dxch, mapGangs, threads, itl,
-                             sweepsPerSync, quantumTicks, s);
+                             sweepsPerSync, quantumTicks, s, model);
     if (r) return r;
     HIP_TRY(hipMemcpyAsync(o.hst, dst, nFit * sizeof(FitStatus), hipMemcpyDeviceToHost, s));
     return 0;
@@ -2520,7 +2715,7 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
           int r = buildMap();
           if (r) return r;
           return timeTrial(mixed, dslots, dst, dmap, mapN, dxch, mapGangs, threads, itl, nFit, J,
-                           fit0, fitsOf, o, s, rate);
+                           fit0, fitsOf, o, s, rate, model);
         },
         [&]() -> int {
           if (snapSt) {
@@ -2801,6 +2996,7 @@ int coxGroupsOnDevice(ccdh_data* d, const double* weights, int32_t* pidOut, uint
   FitStatus st;
   rc = slotPrepare<R>(d, &sp, 1, block, L, &x, &st, o.stream);
   if (rc) return rc;
+  x.coxPieces = 0;  // the tie groups themselves (pids)
   const int32_t zero = 0;
   HIP_TRY(hipMemcpyAsync(dslot, &x, sizeof(x), hipMemcpyHostToDevice, o.stream));
   HIP_TRY(hipMemcpyAsync(dst, &st, sizeof(st), hipMemcpyHostToDevice, o.stream));

@@ -108,6 +108,12 @@ struct Slot {
   int32_t* pidFirst;
   uint8_t* reset;
   int32_t* pidShard;  // Cox: first pid of each row shard (kShards + 1 entries, last = N)
+  // Cox: 1 when the groups are risk-set pieces rather than tie groups (pids):
+  // consecutive pids up to and including an event pid, split at strata and row
+  // shards.  Only an event pid has nW != 0, so the gradient, Hessian and log
+  // likelihood terms fall on pieces ending with one, whose accumulated sums
+  // are the event pid's; the scans then walk pieces instead of pids.
+  int coxPieces;
   Records<R> rec;
   int64_t recStride;  // records per buffer; the second buffer follows the first
   FitCtl ctl;
@@ -187,7 +193,7 @@ int slotLaunchCoxGroups(bool mixed, void* slots, FitStatus* st, const int32_t* a
 // workgroups use 8).
 int slotLaunchSweeps(bool mixed, const void* slots, FitStatus* st, const GangEntry* map, int n,
                      Xch* xch, bool gangs, int threads, int itl, int sweeps,
-                     long long quantumTicks, hipStream_t s);
+                     long long quantumTicks, hipStream_t s, int only = -1);
 // Workgroups a cooperative sweep launch can hold at once.
 int slotGangCapacity(bool mixed, int threads, int itl);
 // XCDs of the device when workgroups are dealt to them round-robin by index

@@ -285,9 +285,10 @@ typedef struct {
   int64_t scan_pids;       /* pids swept by the Cox risk-set scans */
   int64_t objective_rows;  /* rows read by the per-sweep convergence objective */
   int64_t ticks_single;    /* device wall-clock ticks (100 MHz), summed over fits: columns */
-  int64_t ticks_long;      /*   done in one pass / in several passes / convergence objective */
+  int64_t ticks_long;      /*   done in one pass / in several passes / convergence objective
+                              (Cox, lead rank: numerators / risk-set scan / objective) */
   int64_t ticks_objective;
-  int64_t long_columns;    /* multi-pass column visits */
+  int64_t long_columns;    /* multi-pass column visits (Cox: ticks of the exchange and update) */
   double kernel_ms;        /* summed HIP-event time of the sweep launches */
   double wall_ms;          /* host wall time of the whole batch */
   /* the first 128 launches: HIP-event time, fits in flight, workgroups per fit */
