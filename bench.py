@@ -360,6 +360,101 @@ def cpu_leg(args, prob, specs, iters):
             "cpu": cpu_info(), "seconds_per_sweep": [round(x, 3) for x in per_sweep]}
 
 
+def single_fit_leg(args, api, synth, which):
+    """One large fit of a benchmark configuration on the batched engine (gangs
+    of workgroups, device-resident sweeps), timed over a fixed number of sweeps
+    from zero: "poisson" = Poisson 1e6 x 1e5 with offset and intercept, Normal
+    variance 0.1 (FP64 and mixed); "cox" = Breslow Cox 2e6 x 5e4, one stratum,
+    tied days, Laplace variance 0.05.  The oracle's rate comes from a bounded
+    sample of the same sweep on one core."""
+    import dataclasses as dc
+    from ccd_amd.data import PRIOR_LAPLACE, PRIOR_NORMAL, default_spec
+    from oracle import oracle as orc
+    t = time.perf_counter()
+    if which == "poisson":
+        prob, _ = synth.poisson_zipf(seed=synth.DEFAULT_SEED + 2)
+        spec = default_spec(prob, prior=PRIOR_NORMAL, variance=0.1)
+        sweeps, precisions = args.poisson_sweeps, (api.FP64, api.MIXED)
+    else:
+        prob, _ = synth.cox_zipf(seed=synth.DEFAULT_SEED + 3)
+        spec = default_spec(prob, prior=PRIOR_LAPLACE, variance=0.05)
+        sweeps, precisions = args.cox_sweeps, (api.FP64, api.MIXED)
+    log(f"{which}: K={prob.K} J={prob.J} nnz={prob.nnz} generated in {time.perf_counter() - t:.1f}s")
+    out = {"workload": ("Poisson 1e6 x 1e5 Zipf(1.1), offset + intercept, Normal var 0.1"
+                        if which == "poisson" else
+                        "Breslow Cox 2e6 x 5e4 Zipf(1.1), 1 stratum, integer-day ties, Laplace var 0.05"),
+           "rows": prob.K, "covariates": prob.J, "nnz": prob.nnz, "sweeps_timed": sweeps}
+    sp = dc.replace(spec, max_iterations=sweeps)
+    for prec in precisions:
+        name = "fp64" if prec == api.FP64 else "mixed"
+        d = api.Data(prob, precision=prec)
+        api.fit_batch(d, [dc.replace(spec, max_iterations=1)], threads=512)  # code objects, groups
+        t = time.perf_counter()
+        b, info = api.fit_batch(d, [sp], threads=512)
+        el = time.perf_counter() - t
+        st = api.batch_last_stats()
+        d.close()
+        n = max(1, info[0]["iterations"])
+        s_r = 8 if prec == api.FP64 else 4
+        if which == "poisson":
+            # gather: row index + row state + value-free indicator; update: row index,
+            # row state read/write and the running denominator read/write
+            alg = (st["gather_nnz"] * (4 + 2 * s_r) + st["update_nnz"] * (4 + 6 * s_r)
+                   + st["objective_rows"] * 2 * s_r)
+        else:
+            # numerators: row index, group (piece) index, row state; scan: per piece
+            # denominator, nW, numerator and reset flag; update: row index, piece
+            # index, row state read/write and the piece denominator
+            alg = (st["gather_nnz"] * (8 + 2 * s_r) + st["scan_pids"] * 25
+                   + st["update_nnz"] * (8 + 4 * s_r + 8) + st["objective_rows"] * 2 * s_r)
+        per_launch = st["kernel_ms"] / max(1, st["launches"])
+        achieved = alg / max(1e-9, st["kernel_ms"] * 1e-3) / 1e9
+        out[name] = {"seconds_per_sweep": el / n, "sweeps": n, "kernel_ms": st["kernel_ms"],
+                     "gang": max(g for _, _, g in st["timeline"]) if st["timeline"] else None,
+                     "nonzero_coefs": int(np.count_nonzero(b[0])),
+                     "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                                  "kernel": f"k_slot_sweep (one fit, {which})",
+                                  "avg_launch_ms": per_launch, "launches": st["launches"]}}
+        if which == "cox":
+            tick = 1e-8
+            out[name]["phase_seconds"] = {"numerators": st["ticks_single"] * tick,
+                                          "risk_set_scan": st["ticks_long"] * tick,
+                                          "exchange_and_update": st["long_columns"] * tick}
+        log(f"{which} {name}: {el / n:.3f} s/sweep over {n} sweeps, {achieved:.0f} GB/s algorithmic")
+        if prec == api.FP64:
+            b64 = b[0]
+        else:
+            out[name]["max_abs_coef_diff_vs_fp64"] = float(np.max(np.abs(b[0] - b64)))
+    # CPU, one core: Poisson, the oracle's whole first sweep; Cox (whose every
+    # coordinate costs a scan over all risk-set groups), the first sweep over a
+    # sample of the coordinates (the rest fixed at zero), extrapolated per coordinate
+    if args.cpu_fits > 0 and which == "poisson":
+        t = time.perf_counter()
+        orc.fit(prob, dc.replace(spec, max_iterations=1))
+        cpu_s = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": cpu_s, "unit": "s/sweep", "cores": 1, "kind": "port",
+                               "sample": "the first sweep, FP64 oracle (set-up included)"}
+        log(f"{which} cpu oracle: first sweep {cpu_s:.1f}s")
+    elif args.cpu_fits > 0:
+        free = np.unique(np.linspace(0, prob.J - 1, args.single_cpu_coords).astype(int))
+        cs = dc.replace(spec, max_iterations=1)
+        cs.fixed = np.ones(prob.J, np.uint8)
+        cs.fixed[free] = 0
+        if prob.has_offset:
+            cs.fixed[0] = 1
+        t = time.perf_counter()
+        orc.fit(prob, cs)
+        cpu_s = time.perf_counter() - t
+        per_sweep = cpu_s / len(free) * (prob.J - (1 if prob.has_offset else 0))
+        out["cpu_baseline"] = {"value": per_sweep, "unit": "s/sweep", "cores": 1, "kind": "port",
+                               "sample": f"the first sweep over {len(free)} coordinates spread over "
+                                         f"the {prob.J} (others fixed at 0), {cpu_s:.1f}s, "
+                                         "extrapolated per coordinate (set-up included)"}
+        log(f"{which} cpu oracle: {len(free)} coordinates in {cpu_s:.1f}s -> {per_sweep:.0f} s/sweep")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -396,6 +491,12 @@ def main():
     ap.add_argument("--max-gang", type=int, default=0, help="workgroups per fit at most (0: 64)")
     ap.add_argument("--bundle", type=int, default=0, help="fits per slot in lock step (0: default)")
     ap.add_argument("--mm", type=int, default=1, help="MM leg on one CV fit (0: skip)")
+    ap.add_argument("--cox-sweeps", type=int, default=3,
+                    help="Cox 2e6 x 5e4 leg: sweeps timed (0: skip)")
+    ap.add_argument("--poisson-sweeps", type=int, default=3,
+                    help="Poisson 1e6 x 1e5 leg: sweeps timed (0: skip)")
+    ap.add_argument("--single-cpu-coords", type=int, default=200,
+                    help="coordinates of the oracle's sample sweep in the Cox / Poisson legs")
     ap.add_argument("--ingest", type=int, default=1,
                     help="long-format ingestion leg on the CV data set (0: skip)")
     ap.add_argument("--ingest-cpu-fraction", type=float, default=0.25,
@@ -550,6 +651,9 @@ def main():
     ingest_out = None
     if args.ingest and not args.limit:
         ingest_out = ingest_leg(args, api, synth, prob)
+    data.close()
+    cox_out = single_fit_leg(args, api, synth, "cox") if args.cox_sweeps > 0 else None
+    poisson_out = single_fit_leg(args, api, synth, "poisson") if args.poisson_sweeps > 0 else None
 
     out = {
         "metric": METRIC,
@@ -593,8 +697,9 @@ def main():
         "serving": serving,
         "mm": mm,
         "ingest": ingest_out,
+        "cox_2e6x5e4": cox_out,
+        "poisson_1e6x1e5": poisson_out,
     }
-    data.close()
     print(json.dumps(out), flush=True)
 
 

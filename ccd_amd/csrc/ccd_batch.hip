@@ -594,6 +594,47 @@ __device__ __forceinline__ bool gangExchangeN(Gang& g, double* xs, int ns, doubl
   return ok;
 }
 
+// Column metadata of a window of kMetaW coordinates, staged in LDS by one
+// coalesced load per window: the sweep's per-column reads of the fixed flag, the
+// block length, the format, the column's extent (this rank's part of it in a
+// gang) and its stratum-run count then come from LDS instead of a chain of
+// dependent global loads ahead of each column's gather.
+constexpr int kMetaW = 256;
+struct ColMeta {
+  int64_t cb[kMetaW + 1];  // column starts (window-relative index)
+  int32_t b[kMetaW], e[kMetaW];  // this rank's part, relative to the column start
+  int32_t runs[kMetaW];          // stratum runs (grouped models)
+  uint8_t fixed[kMetaW], blk[kMetaW], fmt[kMetaW];
+  int w0;
+};
+
+template <int MODEL, typename R>
+__device__ __forceinline__ void loadColMeta(ColMeta& M, const Slot<R>& s, int w0, const Gang& gg) {
+  const DevX& d = s.d;
+  const int J = s.J;
+  __syncthreads();  // the previous window's readers are done
+  for (int t = threadIdx.x; t <= kMetaW; t += blockDim.x) {
+    const int j = w0 + t;
+    if (j <= J) M.cb[t] = d.colPtr[j];
+    if (t < kMetaW && j < J) {
+      M.fixed[t] = s.fixed[j];
+      M.blk[t] = d.blk != nullptr ? d.blk[j] : 1;
+      M.fmt[t] = d.fmt[j];
+      M.runs[t] = MODEL == M_CLR ? (int32_t)(d.segPtr[j + 1] - d.segPtr[j]) : 0;
+      if (gg.G > 1) {
+        const int32_t* off = d.colShard + (size_t)j * (d.nShards + 1);
+        M.b[t] = off[gg.s0];
+        M.e[t] = off[gg.s1];
+      } else {
+        M.b[t] = 0;
+        M.e[t] = -1;  // the whole column
+      }
+    }
+  }
+  if (threadIdx.x == 0) M.w0 = w0;
+  __syncthreads();
+}
+
 // One coordinate cycle of a slot's fits (independent rows or grouped) over this
 // rank's rows, from coordinate j0; `act` has bit f set for each fit still
 // running (the others' lanes load and store nothing).  Every lane takes its
@@ -623,17 +664,21 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
   // while another still reads coordinate k's.  A gang alternates two buffers
   // (no rank reaches coordinate k+2 before every rank has finished k).
   unsigned recSel = 0;
+  __shared__ ColMeta M;
+  loadColMeta<MODEL, R>(M, s, j0, gg);
   for (int j = j0; j < J; ++j) {
     if (!multi && j > j0 && (j & 3) == 0 && quantumOver(deadline)) {
       jEnd = j;
       break;
     }
-    if (s.fixed[j]) continue;
+    if (j + kBlockCols > M.w0 + kMetaW) loadColMeta<MODEL, R>(M, s, j, gg);  // uniform
+    const int mj = j - M.w0;
+    if (M.fixed[mj]) continue;
     // A block of short columns with pairwise disjoint rows (strata): each column
     // is gathered on its own, then one reduction and one exchange serve the
     // whole block, then each column takes its step and its update.  No column's
     // update changes another's gradient, so this is the sequential result.
-    const int B = (F == 1 && MODEL != M_COX && d.blk != nullptr) ? d.blk[j] : 1;
+    const int B = (F == 1 && MODEL != M_COX && d.blk != nullptr) ? M.blk[mj] : 1;
     if (B > 1) {
       // nonzeros per lane of a block column: one pass of the workgroup covers it
       constexpr int kLanes = LDS::kItems / ITL;
@@ -653,21 +698,20 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
         nAll[c] = nMine[c] = 0;
         runsC[c] = 0;
         stg[c] = false;
-        live[c] = c < B && !s.fixed[jj];  // uniform
+        live[c] = c < B && !M.fixed[mj + c];  // uniform
         if (!live[c]) continue;
-        const int64_t cb = d.colPtr[jj], ce = d.colPtr[jj + 1];
+        const int64_t cb = M.cb[mj + c], ce = M.cb[mj + c + 1];
         int64_t b = cb, e = ce;
         if (multi) {
-          const int32_t* off = d.colShard + (size_t)jj * (d.nShards + 1);
-          b = cb + off[gg.s0];
-          e = cb + off[gg.s1];
+          b = cb + M.b[mj + c];
+          e = cb + M.e[mj + c];
         }
         nAll[c] = (int)(ce - cb);
         nMine[c] = (int)(e - b);
-        const int fmtc = d.fmt[jj];
+        const int fmtc = M.fmt[mj + c];
         bj[c] = s.beta[jj];
         bnd[c] = s.bound[jj];
-        runsC[c] = MODEL == M_CLR ? (int)(d.segPtr[jj + 1] - d.segPtr[jj]) : nAll[c];
+        runsC[c] = MODEL == M_CLR ? M.runs[mj + c] : nAll[c];
         stg[c] = MODEL == M_CLR && runsC[c] < nAll[c];
         if (threadIdx.x == 0) S.cValid[0] = 0;
         passLoad<MODEL, kBlkIT, R>(s, fmtc, cb, e, b, useW, true, L[c]);
@@ -745,23 +789,22 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
         rec.flags += s.recStride;
       }
     }
-    const int64_t cb = d.colPtr[j], ce = d.colPtr[j + 1];
+    const int64_t cb = M.cb[mj], ce = M.cb[mj + 1];
     int64_t b = cb, e = ce;
     if (multi) {
-      const int32_t* off = d.colShard + (size_t)j * (d.nShards + 1);
-      b = cb + off[gg.s0];
-      e = cb + off[gg.s1];
+      b = cb + M.b[mj];
+      e = cb + M.e[mj];
     }
     const int64_t nj = e - b;      // this rank's nonzeros
     const int64_t njAll = ce - cb;  // the column's
-    const int fmt = d.fmt[j];
+    const int fmt = M.fmt[mj];
     // this lane's fit; in a gang only the lead rank's reads count (it publishes them)
     const size_t jf = (size_t)j * F + f;
     double bj = s.beta[jf];
     double bnd = s.bound[jf];
     const double pv = s.pvar[jf], xy = s.xjy[jf];
     const int pt = s.ptype[jf];
-    const long long runs = MODEL == M_CLR ? (long long)(d.segPtr[j + 1] - d.segPtr[j]) : njAll;
+    const long long runs = MODEL == M_CLR ? (long long)M.runs[mj] : njAll;
     const bool staged = MODEL == M_CLR && runs < njAll;
     if ((int)threadIdx.x < F) S.cValid[threadIdx.x] = 0;
     const long long t0 = wall_clock64();
