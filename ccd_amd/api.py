@@ -493,27 +493,23 @@ def search_quadratic(tried):
 
 
 def auto_search_cv(data: Data, spec: FitSpec, ids, n_fold=10, seed=123, start_variance=1.0,
-                   max_steps=10, by_pid=True, **opts):
+                   max_steps=10, by_pid=True, threads=1, reset_coefficients=True, **opts):
     """Auto-search cross-validation over one prior variance
     (AutoSearchCrossValidationDriver::doCrossValidationLoop,
     drivers/AutoSearchCrossValidationDriver.cpp:63-165, nDim = 1): each step fits
-    the n_fold folds at the current variance as one batch (cold starts), scores
-    each by its held-out predictive log likelihood (a fold that does not converge
-    scores NaN and is left out), and the unimodal search (UniModalSearch(10, 0.01,
-    log 1.5)) proposes the next variance, until it converges or max_steps.
+    the n_fold folds at the current variance, scores each by its held-out
+    predictive log likelihood (a fold that does not converge scores NaN and is
+    left out), and the unimodal search (UniModalSearch(10, 0.01, log 1.5))
+    proposes the next variance, until it converges or max_steps.  Starts as in
+    grid_search_cv: reset_coefficients=True, every fold from zero (each step one
+    batch); False, the reference's warm chains over `threads` threads.
     Returns (variance, expected score, trace [(variance, mean, stddev)])."""
-    weights = [cv_fold_weights(ids, n_fold, seed, task, by_pid=by_pid) for task in range(n_fold)]
+    chain = _CvChains(data, spec, ids, n_fold, seed, by_pid, threads, reset_coefficients, None)
     var = float(start_variance)
     tried = []
     expected = float("nan")
     for _ in range(max_steps):
-        specs = []
-        for w in weights:
-            pv = np.where(np.asarray(spec.prior_type) != 0, var, np.asarray(spec.prior_var))
-            specs.append(dataclasses.replace(spec, prior_var=pv, weights=w))
-        _, res = fit_batch(data, specs, heldouts=[1.0 - w for w in weights], **opts)
-        pred = np.array([r["pred_log_likelihood"] if r["return_flag"] == 0 else np.nan
-                         for r in res])
+        pred, _ = chain.step(var, **opts)
         ok = pred[pred == pred]
         mean = float(ok.mean())
         sd = float(np.sqrt(np.mean(ok * ok) - mean * mean))
@@ -522,6 +518,88 @@ def auto_search_cv(data: Data, spec: FitSpec, ids, n_fold=10, seed=123, start_va
         if not cont:
             break
     return var, expected, tried
+
+
+def cv_chains(n_fold, threads):
+    """TaskScheduler's split of a grid point's n_fold tasks over `threads` threads
+    (Thread.h:64-80): thread t runs tasks [t*c, (t+1)*c) in order, c = ceil(n_fold /
+    threads) (fewer threads when the tasks run out)."""
+    T = max(1, min(threads, n_fold))
+    c = n_fold // T + (n_fold % T != 0)
+    return [list(range(t * c, min(n_fold, (t + 1) * c))) for t in range(T) if t * c < n_fold]
+
+
+class _CvChains:
+    """The fold tasks of cross-validation steps (AbstractCrossValidationDriver::
+    doCrossValidationStep, AbstractCrossValidationDriver.cpp:131-248) and the
+    coefficients each thread's CCD carries from task to task."""
+
+    def __init__(self, data, spec, ids, n_fold, seed, by_pid, threads, reset, start):
+        self.data, self.spec, self.n_fold, self.reset = data, spec, n_fold, reset
+        self.weights = [cv_fold_weights(ids, n_fold, seed, f, by_pid=by_pid) for f in range(n_fold)]
+        self.zero = np.zeros(data.problem.J)
+        if data.problem.has_offset:
+            self.zero[0] = spec.beta0[0]
+        self.chains = [list(range(n_fold))] if reset else cv_chains(n_fold, threads)
+        self.beta = [np.array(spec.beta0 if start is None else start, np.float64)
+                     for _ in self.chains]
+
+    def step(self, var, **opts):
+        """Fit every fold at prior variance `var`; (scores [n_fold], results)."""
+        pt = np.asarray(self.spec.prior_type)
+        pv = np.where(pt != 0, var, np.asarray(self.spec.prior_var))
+        n = self.n_fold
+        scores, results = np.full(n, np.nan), [None] * n
+        if self.reset:
+            rounds = [[(0, f) for f in range(n)]]
+        else:
+            depth = max(len(c) for c in self.chains)
+            rounds = [[(t, c[k]) for t, c in enumerate(self.chains) if k < len(c)]
+                      for k in range(depth)]
+        for rnd in rounds:
+            specs = [dataclasses.replace(self.spec, prior_var=pv, weights=self.weights[f],
+                                         beta0=(self.zero if self.reset else self.beta[t]).copy())
+                     for t, f in rnd]
+            b, res = fit_batch(self.data, specs, heldouts=[1.0 - self.weights[f] for _, f in rnd],
+                               **opts)
+            for i, (t, f) in enumerate(rnd):
+                results[f] = res[i]
+                ok = res[i]["return_flag"] == 0
+                if ok:
+                    scores[f] = res[i]["pred_log_likelihood"]
+                if not self.reset:
+                    self.beta[t] = b[i].copy() if ok else self.zero.copy()
+        return scores, results
+
+
+def grid_search_cv(data: Data, spec: FitSpec, ids, variances, n_fold=10, seed=123, by_pid=True,
+                   threads=1, reset_coefficients=False, start=None, **opts):
+    """Grid-search cross-validation (GridSearchCrossValidationDriver::doCrossValidationLoop,
+    GridSearchCrossValidationDriver.cpp:96-140, with AbstractCrossValidationDriver::
+    doCrossValidationStep, AbstractCrossValidationDriver.cpp:131-248) over the prior
+    variances `variances`, one repetition of n_fold folds (fold weights as
+    cv_fold_weights; held-out scores on the complement).
+
+    reset_coefficients=False is the reference's default (createControl(resetCoefficients
+    = FALSE), R/ModelFit.R:506): the tasks run in the reference's thread chains
+    (cv_chains): every task starts from the estimate its chain's previous task
+    reached -- across grid points too -- and a task that does not end with SUCCESS
+    leaves its chain at zero (resetBeta, AbstractCrossValidationDriver.cpp:219-225),
+    its score NaN.  The tasks at the same position of every chain run as one batch.
+    reset_coefficients=True starts every task from zero and runs each grid point's
+    tasks as one batch.  start: the chains' first coefficients (default spec.beta0).
+
+    Returns (point estimates [grid] (mean of the scores that are not NaN), scores
+    [grid x n_fold], results [grid x n_fold], final coefficients of each chain)."""
+    chain = _CvChains(data, spec, ids, n_fold, seed, by_pid, threads, reset_coefficients, start)
+    scores = np.full((len(variances), n_fold), np.nan)
+    results = []
+    for g, var in enumerate(variances):
+        scores[g], res = chain.step(var, **opts)
+        results.append(res)
+    with np.errstate(invalid="ignore"):
+        point = np.array([np.nanmean(r) if np.any(r == r) else np.nan for r in scores])
+    return point, scores, results, chain.beta
 
 
 def cv_grid_point(lower, upper, size, step):

@@ -236,3 +236,41 @@ def cv_weights(ids, nfold, seed, task, by_pid=True):
     out = np.zeros(len(ids), dtype=np.float64)
     lib().orc_cv_weights(_ptr(ids), len(ids), int(by_pid), nfold, seed, task, _ptr(out))
     return out
+
+
+def grid_search_cv_chain(p, spec, ids, variances, n_fold=10, seed=123, by_pid=True, threads=1):
+    """The reference's warm-started grid search (resetCoefficients = FALSE), fit by
+    fit on the CPU: AbstractCrossValidationDriver::doCrossValidationStep
+    (AbstractCrossValidationDriver.cpp:131-248) runs a grid point's tasks on the
+    TaskScheduler's threads (Thread.h:64-80: thread t takes tasks [t*c, (t+1)*c),
+    c = ceil(n_fold / threads)), each thread's CCD starting every task from the
+    coefficients its previous task left (ccdTask->update without resetBeta), and
+    setting them to zero after a task that did not succeed (:219-225); the grid
+    loop (GridSearchCrossValidationDriver.cpp:96-140) keeps the threads' CCDs from
+    grid point to grid point.  Returns (scores [grid x n_fold], results, chains'
+    final coefficients)."""
+    import dataclasses
+    T = max(1, min(threads, n_fold))
+    c = n_fold // T + (n_fold % T != 0)
+    chains = [list(range(t * c, min(n_fold, (t + 1) * c))) for t in range(T) if t * c < n_fold]
+    zero = np.zeros(p.J)
+    if p.has_offset:
+        zero[0] = spec.beta0[0]
+    beta = [np.array(spec.beta0, np.float64) for _ in chains]
+    pt = np.asarray(spec.prior_type)
+    scores = np.full((len(variances), n_fold), np.nan)
+    results = [[None] * n_fold for _ in variances]
+    for g, var in enumerate(variances):
+        pv = np.where(pt != 0, var, np.asarray(spec.prior_var))
+        for t, chain in enumerate(chains):
+            for f in chain:
+                w = cv_weights(ids, n_fold, seed, f, by_pid=by_pid)
+                s = dataclasses.replace(spec, prior_var=pv, weights=w, beta0=beta[t].copy())
+                b, info = fit(p, s, heldout=1.0 - w)
+                results[g][f] = info
+                if info["return_flag"] == 0:
+                    scores[g, f] = info["pred_log_likelihood"]
+                    beta[t] = b
+                else:
+                    beta[t] = zero.copy()
+    return scores, results, beta
