@@ -67,18 +67,26 @@ def algorithmic_bytes(stats, nnz_free, precision):
     return state + index
 
 
-def serve_leg(args, api, synth):
+def serve_leg(args, api, synth, serve_designs=None):
     """Continuous-batching server on a Poisson stream of mixed fit requests."""
     from ccd_amd.data import COX, LOGISTIC, POISSON, PRIOR_LAPLACE, PRIOR_NORMAL, default_spec
     t = time.perf_counter()
-    pool, reqs = synth.request_stream(
-        n_requests=args.serve_requests, rate=args.serve_rate,
-        K_range=(10_000, args.serve_kmax), P_range=(1_000, args.serve_pmax),
-        per_row_range=(20, 60), cox_K_max=args.serve_cox_kmax, seed=20261208,
-        progress=lambda p: log(f"serving pool: design K={p.K} J={p.J} nnz={p.nnz} "
-                               f"after {time.perf_counter() - t:.1f}s"))
-    log(f"serving pool: {len(pool)} designs, nnz {min(p.nnz for p in pool)}.."
-        f"{max(p.nnz for p in pool)}, generated in {time.perf_counter() - t:.1f}s")
+    if args.serve_per_request:
+        # a design of its own per request, generated before the process touched the GPU
+        stream = serve_designs
+        pool = [p for _, p, _ in stream]
+        reqs = [(ta, i, v) for i, (ta, _, v) in enumerate(stream)]
+        del stream
+    else:
+        pool, reqs = synth.request_stream(
+            n_requests=args.serve_requests, rate=args.serve_rate,
+            K_range=(10_000, args.serve_kmax), P_range=(1_000, args.serve_pmax),
+            per_row_range=(20, 60), cox_K_max=args.serve_cox_kmax, seed=20261208,
+            progress=lambda p: log(f"serving pool: design K={p.K} J={p.J} nnz={p.nnz} "
+                                   f"after {time.perf_counter() - t:.1f}s"))
+    log(f"serving: {len(pool)} designs, nnz {min(p.nnz for p in pool)}.."
+        f"{max(p.nnz for p in pool)}, rows {min(p.K for p in pool)}..{max(p.K for p in pool)}, "
+        f"generated in {time.perf_counter() - t:.1f}s")
 
     def spec_of(i, var):
         p = pool[i]
@@ -88,7 +96,8 @@ def serve_leg(args, api, synth):
     specs = [spec_of(i, v) for _, i, v in reqs]
     srv = api.Server(precision=api.FP64, quantum_ms=args.serve_quantum_ms,
                      threads=args.serve_threads)
-    srv.wait(srv.submit(pool[0], spec_of(0, 1.0)))  # load code objects (untimed)
+    warm, _ = synth.lr_indicator(K=2000, P=100, p=0.05, n_true=5, seed=3)
+    srv.wait(srv.submit(warm, default_spec(warm, prior=PRIOR_LAPLACE, variance=1.0)))  # untimed
     t0 = time.perf_counter()
     ids = []
     for (ta, i, _), sp in zip(reqs, specs):
@@ -139,6 +148,8 @@ def serve_leg(args, api, synth):
                                              and plain_t):
             break
         _, i, _ = reqs[k]
+        if pool[i].nnz > args.serve_plain_max_nnz:
+            continue
         t1 = time.perf_counter()
         d = api.Data(pool[i])
         bp, ip = api.fit_plain(d, specs[k])
@@ -153,9 +164,13 @@ def serve_leg(args, api, synth):
     return {
         "workload": "served fit requests: logistic / Poisson / Cox, Zipf(1.1) designs",
         "requests": len(ids), "arrival_rate_per_s": args.serve_rate,
-        "rows": [10_000, args.serve_kmax], "cox_rows_max": args.serve_cox_kmax,
+        "rows": [10_000, args.serve_kmax],
+        "cox_rows_max": args.serve_kmax if args.serve_per_request else args.serve_cox_kmax,
         "covariates": [1_000, args.serve_pmax], "nnz_per_row": [20, 60],
-        "designs": len(pool), "quantum_ms": args.serve_quantum_ms, "dtype": "f64",
+        "designs": len(pool), "design_per_request": bool(args.serve_per_request),
+        "rows_served": [min(p.K for p in pool), max(p.K for p in pool)],
+        "nnz_served": [min(p.nnz for p in pool), max(p.nnz for p in pool)],
+        "quantum_ms": args.serve_quantum_ms, "dtype": "f64",
         "fits_per_sec": fps, "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),
         "p50_queue_s": float(np.percentile(queued, 50)),
@@ -164,6 +179,8 @@ def serve_leg(args, api, synth):
         "iterations_median": int(np.median(iters)), "launches": st["launches"],
         "arena_peak_bytes": st["arena_peak"], "failed": st["failed"],
         "plain_sequential_fits_per_sec": plain_fps, "plain_sample": len(plain_t),
+        "plain_sample_rule": f"requests in random order with at most {args.serve_plain_max_nnz} "
+                             f"nonzeros, {args.serve_plain_seconds:.0f} s budget",
         "speedup_vs_plain_gpu": fps / plain_fps if plain_fps else None,
         "max_rel_coef_diff_plain_vs_served": plain_diff,
     }
@@ -474,11 +491,16 @@ def main():
     ap.add_argument("--max-iter", type=int, default=1000, help="iteration cap (probe runs)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a device copy of this size (torch) for counter calibration")
-    ap.add_argument("--serve-requests", type=int, default=200, help="0: skip the serving leg")
+    ap.add_argument("--serve-requests", type=int, default=60, help="0: skip the serving leg")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="arrivals per second")
-    ap.add_argument("--serve-kmax", type=int, default=200_000, help="largest request rows")
-    ap.add_argument("--serve-pmax", type=int, default=20_000, help="largest request columns")
-    ap.add_argument("--serve-cox-kmax", type=int, default=20_000, help="largest Cox request rows")
+    ap.add_argument("--serve-kmax", type=int, default=2_000_000, help="largest request rows")
+    ap.add_argument("--serve-pmax", type=int, default=100_000, help="largest request columns")
+    ap.add_argument("--serve-per-request", type=int, default=1,
+                    help="1: a design per request (generated on the device); 0: a pool of 12")
+    ap.add_argument("--serve-cox-kmax", type=int, default=20_000,
+                    help="largest Cox request rows (pool mode)")
+    ap.add_argument("--serve-plain-max-nnz", type=int, default=5_000_000,
+                    help="largest request (nonzeros) in the plain sequential sample")
     ap.add_argument("--serve-quantum-ms", type=float, default=20.0)
     ap.add_argument("--serve-threads", type=int, default=0,
                     help="server workgroup size: 256 (0) or 512 (x 4 items per lane)")
@@ -513,9 +535,20 @@ def main():
     from ccd_amd import api, synth
     from ccd_amd.data import CONDITIONAL_LOGISTIC, PRIOR_LAPLACE, default_spec
 
+    serve_designs = None
+    if args.serve_requests > 0 and args.serve_per_request:
+        # generated in forked workers before this process touches the GPU
+        t = time.perf_counter()
+        serve_designs = synth.request_designs(
+            n_requests=args.serve_requests, rate=args.serve_rate,
+            K_range=(10_000, args.serve_kmax), P_range=(1_000, args.serve_pmax),
+            per_row_range=(20, 60), seed=20261208, workers=min(16, os.cpu_count() or 1),
+            progress=lambda p: log(f"serving design: model {p.model} K={p.K} J={p.J} "
+                                   f"nnz={p.nnz} after {time.perf_counter() - t:.1f}s"))
     name, n_cu, hbm = api.device_info()
     log("device", name, n_cu, "CUs", f"{hbm:.0f} GB")
-    serving = serve_leg(args, api, synth) if args.serve_requests > 0 else None
+    serving = serve_leg(args, api, synth, serve_designs) if args.serve_requests > 0 else None
+    serve_designs = None
     if args.skip_batch:
         print(json.dumps({"serving": serving}), flush=True)
         return

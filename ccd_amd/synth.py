@@ -345,3 +345,50 @@ def poisson_overflow(seed=5, K=5000, slope=0.01):
     eta[big] += slope * xv
     y = rng.poisson(np.exp(eta)).astype(np.float64)
     return build_problem(POISSON, cols, y)
+
+
+def _request_design(args):
+    m, K, P, per_row, seed = args
+    n_true = min(100, max(1, P // 10))
+    if m == LOGISTIC:
+        return lr_zipf(K=K, P=P, per_row=per_row, n_true=n_true, seed=seed)[0]
+    if m == POISSON:
+        return poisson_zipf(K=K, P=P, per_row=per_row, n_true=n_true, base=-5.0, seed=seed)[0]
+    return cox_zipf(K=K, P=P, per_row=per_row, n_true=n_true, seed=seed)[0]
+
+
+def request_designs(n_requests=100, rate=50.0, K_range=(10_000, 2_000_000),
+                    P_range=(1_000, 100_000), per_row_range=(20, 60),
+                    models=(LOGISTIC, POISSON, COX), seed=DEFAULT_SEED, workers=1,
+                    progress=None):
+    """A served request stream with a design of its own per request: Poisson
+    arrivals at `rate` per second, model uniform over `models`, rows and columns
+    log-uniform in the ranges, nonzeros per row uniform, Zipf(1.1) columns, a prior
+    variance log-uniform in [0.01, 1].  `workers` > 1 generates the designs in a
+    pool of forked processes (call it before the process has touched the GPU).
+    Returns [(arrival_s, problem, variance)]."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    jobs, var = [], []
+    for _ in range(n_requests):
+        m = models[int(rng.integers(len(models)))]
+        K = int(round(np.exp(rng.uniform(np.log(K_range[0]), np.log(K_range[1])))))
+        P = int(round(np.exp(rng.uniform(np.log(P_range[0]), np.log(P_range[1])))))
+        per_row = int(rng.integers(per_row_range[0], per_row_range[1] + 1))
+        var.append(float(np.exp(rng.uniform(np.log(0.01), np.log(1.0)))))
+        jobs.append((m, K, P, per_row, int(rng.integers(1 << 30))))
+    if workers > 1:
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(workers) as pool:
+            designs = []
+            for prob in pool.imap(_request_design, jobs):
+                designs.append(prob)
+                if progress is not None:
+                    progress(prob)
+    else:
+        designs = []
+        for j in jobs:
+            designs.append(_request_design(j))
+            if progress is not None:
+                progress(designs[-1])
+    return [(float(t[i]), designs[i], var[i]) for i in range(n_requests)]

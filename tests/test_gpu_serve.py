@@ -9,7 +9,7 @@ launches many times.  Admission respects the memory budget.
 import numpy as np
 import pytest
 
-from ccd_amd import api
+from ccd_amd import api, synth
 from ccd_amd.data import CONDITIONAL_LOGISTIC, default_spec
 from oracle import oracle as orc
 from tests.test_gpu_plain import SYNTH, check_fit
@@ -141,3 +141,45 @@ def test_serve_gangs_match_oracle(name):
     check_fit(b, r, bc, ic)
     for bm, rm, _ in (more[0], more[2]):
         check_fit(bm, rm, bc, ic)
+
+
+def _full_range_requests():
+    """A design per request at the serving configuration's ranges: logistic 1e4 rows,
+    logistic 2e5 x 5e4, Poisson 1e6 x 1e5 with offset, Cox 1e6 x 2e4, 20-60
+    nonzeros per row, Zipf(1.1) columns; the large fits capped at a few sweeps
+    so the oracle (Poisson) and the repeated serves stay within the test's time."""
+    from ccd_amd.data import PRIOR_LAPLACE, PRIOR_NORMAL
+    lr_small, _ = synth.lr_zipf(K=10_000, P=1_000, per_row=20, n_true=100, seed=71)
+    lr_mid, _ = synth.lr_zipf(K=200_000, P=50_000, per_row=40, n_true=100, seed=72)
+    pois, _ = synth.poisson_zipf(K=1_000_000, P=100_000, per_row=30, n_true=100, base=-5.0,
+                                 seed=73)
+    cox, _ = synth.cox_zipf(K=1_000_000, P=20_000, per_row=60, n_true=100, seed=74)
+    return [
+        (lr_small, default_spec(lr_small, prior=PRIOR_LAPLACE, variance=0.5), True),
+        (lr_mid, default_spec(lr_mid, prior=PRIOR_LAPLACE, variance=0.1, max_iterations=3), False),
+        (pois, default_spec(pois, prior=PRIOR_NORMAL, variance=0.1, max_iterations=2), True),
+        (cox, default_spec(cox, prior=PRIOR_LAPLACE, variance=0.05, max_iterations=2), False),
+    ]
+
+
+@pytest.mark.timeout(600)
+def test_serve_full_range_requests_alone_and_in_the_mix():
+    """The serving configuration's row and column ranges, a design per request:
+    every result is bit-identical served alone and in the mix (gangs capped at 4
+    workgroups both ways), and the requests the oracle finishes in time (the
+    small logistic fit to convergence, the Poisson fit's first two sweeps) match
+    it to 1e-8."""
+    reqs = _full_range_requests()
+    print("full-range serve: mix", flush=True)
+    with api.Server(quantum_ms=5.0, max_gang=4) as srv:
+        ids = [srv.submit(p, sp) for p, sp, _ in reqs]
+        mix = [srv.wait(i) for i in ids]
+    for k, (p, sp, check) in enumerate(reqs):
+        print("full-range serve: alone", k, p.K, p.J, flush=True)
+        with api.Server(quantum_ms=50.0, max_gang=4) as srv:
+            b, r, _ = srv.wait(srv.submit(p, sp))
+        assert np.array_equal(b, mix[k][0]), k
+        assert r["iterations"] == mix[k][1]["iterations"]
+        if check:
+            bc, ic = orc.fit(p, sp)
+            check_fit(mix[k][0], mix[k][1], bc, ic)
