@@ -405,9 +405,10 @@ def single_fit_leg(args, api, synth, which):
     for prec in precisions:
         name = "fp64" if prec == api.FP64 else "mixed"
         d = api.Data(prob, precision=prec)
-        api.fit_batch(d, [dc.replace(spec, max_iterations=1)], threads=512)  # code objects, groups
+        api.fit_batch(d, [dc.replace(spec, max_iterations=1)], threads=512,
+                      pass_items=4)  # code objects, groups
         t = time.perf_counter()
-        b, info = api.fit_batch(d, [sp], threads=512)
+        b, info = api.fit_batch(d, [sp], threads=512, pass_items=4)
         el = time.perf_counter() - t
         st = api.batch_last_stats()
         d.close()
@@ -722,7 +723,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": f"k_slot_sweep<{'double' if precision == api.FP64 else 'float'}, "
-                               f"{stats['tuned_threads']}, {stats['tuned_pass_items']}>",
+                               f"{stats['tuned_threads']}, {stats['tuned_pass_items']}, 4>",
                      "avg_launch_ms": per_launch_ms,
                      "launches": stats["launches"]},
         "cpu_baseline": cpu,

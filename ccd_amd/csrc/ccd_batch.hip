@@ -594,6 +594,18 @@ __device__ __forceinline__ bool gangExchangeN(Gang& g, double* xs, int ns, doubl
   return ok;
 }
 
+// A solo-run boundary of a gang (see cycleFit): every rank's row and group
+// writes before it are visible to every rank's reads after it -- an agent-scope
+// release before arriving and an acquire after leaving -- and the lead's two
+// values xl[0..1] (its quantum decision and where it stopped) reach every rank.
+__device__ __forceinline__ bool gangSoloSync(Gang& g, double* xl) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  double none[1] = {0.0};
+  const bool ok = gangExchangeN(g, none, 0, xl, 2);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return ok;
+}
+
 // Column metadata of a window of kMetaW coordinates, staged in LDS by one
 // coalesced load per window: the sweep's per-column reads of the fixed flag, the
 // block length, the format, the column's extent (this rank's part of it in a
@@ -666,14 +678,46 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
   unsigned recSel = 0;
   __shared__ ColMeta M;
   loadColMeta<MODEL, R>(M, s, j0, gg);
+  // Solo runs (gangs): consecutive short columns -- one pass of this workgroup at
+  // most -- are gathered, stepped and updated by the lead rank alone over the
+  // whole column while the other ranks skip ahead, so they cost no exchange.
+  // Entering and leaving a run the ranks meet (gangSoloSync), which orders every
+  // rank's row writes before the meeting before every read after it.  A lead
+  // whose quantum runs out inside a run stops there and tells the others when
+  // they meet it at the run's end.
+  const int64_t soloMax = multi ? CH : 0;
+  bool inSolo = false, soloStop = false;
+  auto soloSync = [&](int jStop) -> bool {
+    if (threadIdx.x == 0 && lead) {
+      xl[0] = soloStop ? 1.0 : 0.0;
+      xl[1] = (double)jStop;
+    }
+    return gangSoloSync(gg, xl);
+  };
   for (int j = j0; j < J; ++j) {
-    if (!multi && j > j0 && (j & 3) == 0 && quantumOver(deadline)) {
+    if ((!multi || (inSolo && lead)) && j > j0 && (j & 3) == 0 && quantumOver(deadline)) {
+      soloStop = multi;
       jEnd = j;
       break;
     }
     if (j + kBlockCols > M.w0 + kMetaW) loadColMeta<MODEL, R>(M, s, j, gg);  // uniform
     const int mj = j - M.w0;
     if (M.fixed[mj]) continue;
+    if (multi) {
+      const bool soloCol = M.cb[mj + 1] - M.cb[mj] <= soloMax;
+      if (soloCol != inSolo) {
+        if (!soloSync(j)) return -1;
+        const bool stopped = inSolo && xl[0] != 0.0;
+        inSolo = soloCol;
+        if (stopped) {  // the lead's quantum ran out inside the run just left
+          inSolo = false;
+          jEnd = (int)xl[1];
+          break;
+        }
+      }
+      if (inSolo && !lead) continue;
+    }
+    const bool part = multi && !inSolo;  // this rank's part of the column, one exchange
     // A block of short columns with pairwise disjoint rows (strata): each column
     // is gathered on its own, then one reduction and one exchange serve the
     // whole block, then each column takes its step and its update.  No column's
@@ -702,7 +746,7 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
         if (!live[c]) continue;
         const int64_t cb = M.cb[mj + c], ce = M.cb[mj + c + 1];
         int64_t b = cb, e = ce;
-        if (multi) {
+        if (part) {
           b = cb + M.b[mj + c];
           e = cb + M.e[mj + c];
         }
@@ -721,7 +765,7 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
         v[2 * c + 1] = vv[1];
       }
       blockSumFits<2 * kBlockCols>(v, 1);
-      if (multi) {
+      if (part) {
         if (threadIdx.x == 0) {
           for (int c = 0; c < B; ++c) {
             xs[2 * c] = v[2 * c];
@@ -779,7 +823,7 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
       continue;
     }
     Records<R> rec = s.rec;
-    if (multi && (recSel++ & 1u)) {
+    if (part && (recSel++ & 1u)) {
       rec.row += s.recStride;
       rec.o += s.recStride;
       rec.xo += s.recStride;
@@ -791,7 +835,7 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
     }
     const int64_t cb = M.cb[mj], ce = M.cb[mj + 1];
     int64_t b = cb, e = ce;
-    if (multi) {
+    if (part) {
       b = cb + M.b[mj];
       e = cb + M.e[mj];
     }
@@ -813,7 +857,7 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
     // the exchange also carries the lead rank's end-of-quantum decision
     auto reduce = [&]() -> bool {
       blockSumFits<2>(v, F);
-      if (!multi) return true;
+      if (!part) return true;
       if ((int)threadIdx.x < F) {  // lane f holds fit f's totals and coefficient state
         xs[2 * threadIdx.x] = v[0];
         xs[2 * threadIdx.x + 1] = v[1];
@@ -897,6 +941,10 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
       jEnd = j + 1;
       break;
     }
+  }
+  if (multi && inSolo) {  // every rank leaves the last run together
+    if (!soloSync(jEnd)) return -1;
+    if (xl[0] != 0.0) jEnd = (int)xl[1];
   }
   if (threadIdx.x == 0) {
     FitStatus& cs = st[0];
