@@ -26,7 +26,8 @@ stopping rule on the GPU, and the oracle's first MM iterations for the CPU rate.
 
 Ingestion leg ("ingest"): the CV data set written out in long format (outcome
 rows and covariate entries shuffled, random ids) and ingested on the device
-(ccdh_data_create_long), checked identical to the source; the oracle ingests the
+(ccdh_data_create_long), checked identical to the source, its device-built
+per-column tables identical to the host build; the oracle ingests the
 first quarter of the strata for the CPU rate.
 
   python bench.py [--steps 1] [--warmup 0] [--scale 1.0] [--plain-fits 2] [--cpu-fits 1]
@@ -245,19 +246,23 @@ def ingest_leg(args, api, synth, prob):
     d = api.Data.from_long(**long)
     gpu_s = time.perf_counter() - t
     q = d.problem
+    dev_tables = d.tables()
     d.close()
     identical = bool(np.array_equal(q.col_ptr, prob.col_ptr) and np.array_equal(q.rows, prob.rows)
                      and np.array_equal(q.y, prob.y) and np.array_equal(q.strata, prob.strata))
     t = time.perf_counter()
     d = api.Data(prob)
     csc_s = time.perf_counter() - t
+    host_tables = d.tables()
     d.close()
+    same_tables = all(np.array_equal(dev_tables[k], host_tables[k]) for k in host_tables)
     out = {"workload": f"long-format ingestion of the CV data set ({m} covariate entries, "
                        f"{prob.K} rows, rows and entries shuffled)",
            "seconds": gpu_s, "entries_per_sec": m / gpu_s, "from_csc_seconds": csc_s,
-           "identical_to_source": identical}
+           "identical_to_source": identical,
+           "tables_identical_to_host_build": bool(same_tables)}
     log(f"ingest: {m} entries in {gpu_s:.2f}s ({m / gpu_s / 1e6:.1f} M entries/s; from a host "
-        f"CSC {csc_s:.2f}s), identical {identical}")
+        f"CSC {csc_s:.2f}s), identical {identical}, tables {same_tables}")
     if args.ingest_cpu_fraction > 0:
         from oracle.ingest import ingest
         n_s = max(1, int((prob.strata.max() + 1) * args.ingest_cpu_fraction))
@@ -304,7 +309,10 @@ def oracle_errors(prob, betas, infos):
     per_fit = {}
     for k, f in enumerate(ref["fits"]):
         f = int(f)
+        diff = np.abs(np.ravel(betas[f]) - np.ravel(ref["beta"][k]))
         per_fit[f] = {"max_rel_coef_err": rel_coef_err(betas[f], ref["beta"][k]),
+                      "tolerance_ratio": float(np.max(diff / (1e-10 + 1e-8 * np.abs(
+                          np.ravel(ref["beta"][k]))))),
                       "rel_loglik_err": abs(infos[f]["log_likelihood"] - ref["log_likelihood"][k])
                       / abs(ref["log_likelihood"][k]),
                       "iterations": infos[f]["iterations"],
@@ -312,6 +320,9 @@ def oracle_errors(prob, betas, infos):
                       "nonzero_coefs": int(np.count_nonzero(ref["beta"][k]))}
         log(f"oracle check fit {f}: {per_fit[f]}")
     return {"fits": per_fit, "max_rel": max(v["max_rel_coef_err"] for v in per_fit.values()),
+            "max_tolerance_ratio": max(v["tolerance_ratio"] for v in per_fit.values()),
+            "tolerance": "|b - b_oracle| <= 1e-10 + 1e-8 |b_oracle| per coefficient "
+                         "(ratio <= 1 passes), same iterations",
             "source": "tests/golden/cv160_oracle.npz (FP64 CPU oracle, same data set)"}
 
 

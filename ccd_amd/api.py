@@ -9,7 +9,7 @@ import os
 
 import numpy as np
 
-from .data import CcdProblem, FitSpec
+from .data import CONDITIONAL_LOGISTIC, COX, CcdProblem, FitSpec
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libccdhip.so")
@@ -19,7 +19,7 @@ FP64, MIXED = 0, 1
 # every symbol include/ccdhip.h declares
 EXPORTS = [
     "ccdh_last_error", "ccdh_device_info", "ccdh_data_create", "ccdh_data_create_long",
-    "ccdh_data_get_csc", "ccdh_data_destroy",
+    "ccdh_data_get_csc", "ccdh_data_get_tables", "ccdh_data_destroy",
     "ccdh_data_device_bytes", "ccdh_data_shape", "ccdh_data_precision", "ccdh_fit_create", "ccdh_fit_clone",
     "ccdh_fit_destroy", "ccdh_set_weights", "ccdh_fixed_terms_loglik",
     "ccdh_fixed_terms_grad_hess", "ccdh_zero_xbeta", "ccdh_axpy_xbeta", "ccdh_compute_xbeta",
@@ -179,6 +179,7 @@ def load_library(path=LIB_PATH):
         "ccdh_data_create": (ci, [vp, vp]),
         "ccdh_data_create_long": (ci, [vp, vp]),
         "ccdh_data_get_csc": (ci, [vp, vp, vp, vp, vp, vp, vp, vp]),
+        "ccdh_data_get_tables": (ci, [vp] * 11),
         "ccdh_data_destroy": (ci, [vp]),
         "ccdh_data_device_bytes": (ctypes.c_int64, [vp]),
         "ccdh_data_shape": (ci, [vp, vp, vp, vp, vp]),
@@ -325,6 +326,34 @@ class Data:
     @property
     def device_bytes(self):
         return load_library().ccdh_data_device_bytes(self.handle)
+
+    def tables(self):
+        """The data set's derived per-column tables (ccdh_data_get_tables): CLR
+        stratum runs, nonzero strata and stratum starts; row shards and each
+        column's shard offsets; column blocks (not Cox)."""
+        lib = load_library()
+        m, K, J, off = (ctypes.c_int() for _ in range(4))
+        _check(lib.ccdh_data_shape(self.handle, ctypes.byref(m), ctypes.byref(K), ctypes.byref(J),
+                                   ctypes.byref(off)))
+        n_seg, n_strata = ctypes.c_int64(), ctypes.c_int()
+        _check(lib.ccdh_data_get_tables(self.handle, ctypes.byref(n_seg), ctypes.byref(n_strata),
+                                        *([None] * 8)))
+        col_ptr = np.zeros(J.value + 1, np.int64)
+        _check(lib.ccdh_data_get_csc(self.handle, _p(col_ptr), None, None, None, None, None, None))
+        clr, cox = m.value == CONDITIONAL_LOGISTIC, m.value == COX
+        t = dict(shard_row=np.zeros(65, np.int32), col_shard=np.zeros(J.value * 65, np.int32))
+        if clr:
+            t.update(seg_ptr=np.zeros(J.value + 1, np.int64), seg_off=np.zeros(n_seg.value, np.int32),
+                     seg_key=np.zeros(n_seg.value, np.int32),
+                     nz_key=np.zeros(int(col_ptr[-1]), np.int32),
+                     strat_start=np.zeros(n_strata.value + 1, np.int32))
+        if not cox:
+            t["blk"] = np.zeros(J.value, np.int8)
+        g = lambda k: _p(t[k]) if k in t else None  # noqa: E731
+        _check(lib.ccdh_data_get_tables(self.handle, None, None, g("seg_ptr"), g("seg_off"),
+                                        g("seg_key"), g("nz_key"), g("strat_start"),
+                                        g("shard_row"), g("col_shard"), g("blk")))
+        return t
 
     def close(self):
         if self.handle:

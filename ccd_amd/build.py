@@ -16,6 +16,7 @@ SOURCES = [
     "ccd_engine.hip",
     "ccd_batch.hip",
     "ccd_ingest.hip",
+    "ccd_tables.hip",
     "ccd_cv.cpp",
     "ccd_serve.cpp",
     "engine/HipModelSpecifics.cpp",

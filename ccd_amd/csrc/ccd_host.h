@@ -103,6 +103,8 @@ int setError(int code, const std::string& msg);
 
 int dataCreate(const ccdh_data_desc* desc, ccdh_data** out, Bump* place = nullptr);
 int64_t dataFootprint(const ccdh_data_desc* desc);
+// the derived per-column tables of a data set whose arrays are resident (ccd_tables.hip)
+int buildTablesDevice(ccdh_data* d, hipStream_t s);
 void dataDestroy(ccdh_data* d);
 int fitCreate(ccdh_data* d, ccdh_fit** out);
 void fitDestroy(ccdh_fit* f);

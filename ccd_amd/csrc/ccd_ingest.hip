@@ -5,7 +5,7 @@
 // (R/NewDataConversion.R:95-235, ModelData.cpp:69-146 loadY, 148-243
 // loadMultipleX, and for Poisson finalizeSqlCyclopsData(useOffsetCovariate = -1):
 // RcppModelData.cpp:482-556, ModelData.cpp:609-652).  Here the same steps run as
-// device radix sorts and scans:
+// device radix sorts, scans and compactions (ccd_prim.h):
 //   1. the row order: stable sorts by row id, then (Cox) y, time descending,
 //      then (CLR, Cox) stratum -- least significant key first;
 //   2. stratum numbers by change points in that order (loadY's lastStratumMap);
@@ -15,18 +15,24 @@
 //   5. one sort of the entries by (column, row), a check for repeated entries,
 //      zero values left out, column starts by boundary detection, and a column
 //      is SPARSE when a value other than 0 and 1 occurs (loadMultipleX's upcast).
-// The compressed columns, outcomes and strata then pass through ccdh_data_create,
-// which builds the per-column tables (runs, shards, blocks) as for any input.
+// The data set's arrays are assembled on the device (the LR / Poisson intercept
+// and the Poisson log-time offset column first) and its per-column tables are
+// built there (ccd_tables.hip); only the O(rows + columns) outcome, stratum and
+// column-start vectors come back to the host, which keeps copies of them.  The
+// offset column's log(time) and the Poisson -log(y!) terms are taken on the
+// host, as the reference takes them, so an ingested data set is bit-identical to
+// one built from the same columns by ccdh_data_create.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "ccd_host.h"
+#include "ccd_prim.h"
 
 #define HIP_CHECK(expr)                                                                \
   do {                                                                                 \
@@ -40,16 +46,24 @@ namespace {
 
 constexpr int kT = 256;
 
-int blocksFor(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + kT - 1) / kT, 65535)); }
+int blocksFor(int64_t n) { return prim::gridFor(n); }
 
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() {
     if (p) (void)hipFree(p);
   }
   hipError_t alloc(size_t n) { return hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)); }
 };
+
+template <typename T>
+void swapBuf(DevBuf<T>& a, DevBuf<T>& b) {
+  std::swap(a.p, b.p);
+}
 
 __global__ void k_iota(int n, int32_t* a) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = i;
@@ -61,20 +75,46 @@ __global__ void k_gather(int n, const T* src, const int32_t* idx, T* dst) {
     dst[i] = src[idx[i]];
 }
 
+// sort keys of a pass: the source gathered through the current order
+__global__ void k_key_i64(int64_t n, const int64_t* src, const int32_t* idx, uint64_t* key) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    key[i] = prim::keyOfI64(src[idx ? idx[i] : i]);
+}
+__global__ void k_key_f64(int n, const double* src, const int32_t* idx, bool descending,
+                          uint64_t* key) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    key[i] = prim::keyOfF64(src[idx[i]], descending);
+}
+
 // 1 where the sorted stratum id differs from the previous row's
-__global__ void k_change_flags(int n, const int64_t* s, int32_t* f) {
+__global__ void k_change_flags(int n, const uint64_t* s, uint8_t* f) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     f[i] = (i > 0 && s[i] != s[i - 1]) ? 1 : 0;
 }
 
+// stratum number = change points at or before the row
+__global__ void k_stratum_numbers(int n, const uint8_t* f, const int32_t* pos, int32_t* strata) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    strata[i] = pos[i] + f[i];
+}
+
 template <typename T>
-__global__ void k_adjacent_equal(int n, const T* a, int* found) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x + 1; i < n; i += gridDim.x * blockDim.x)
+__global__ void k_adjacent_equal(int64_t n, const T* a, int* found) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x + 1; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
     if (a[i] == a[i - 1]) atomicOr(found, 1);
 }
 
+// 1 at the first of each run of equal keys
+__global__ void k_unique_flags(int64_t n, const uint64_t* a, uint8_t* f) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    f[i] = (i == 0 || a[i] != a[i - 1]) ? 1 : 0;
+}
+
 // first index in a[0..n) whose value is >= v
-__device__ inline int lowerBound(const int64_t* a, int n, int64_t v) {
+__device__ inline int lowerBound(const uint64_t* a, int n, uint64_t v) {
   int lo = 0, hi = n;
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
@@ -86,25 +126,29 @@ __device__ inline int lowerBound(const int64_t* a, int n, int64_t v) {
   return lo;
 }
 
-// row of each entry (-1: row id not among the outcomes) and its validity flag
-__global__ void k_entry_rows(int m, const int64_t* entryRowId, const int64_t* sortedIds,
-                             const int32_t* rowOf, int n, int32_t* row, uint8_t* valid) {
+// row of each entry (-1: row id not among the outcomes), its validity flag and
+// its covariate id as a sort key
+__global__ void k_entry_rows(int m, const int64_t* entryRowId, const uint64_t* sortedIds,
+                             const int32_t* rowOf, int n, const int64_t* cov, int32_t* row,
+                             uint8_t* valid, uint64_t* covKey) {
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
-    const int64_t id = entryRowId[e];
+    const uint64_t id = prim::keyOfI64(entryRowId[e]);
     const int p = lowerBound(sortedIds, n, id);
     const bool ok = p < n && sortedIds[p] == id;
     row[e] = ok ? rowOf[p] : -1;
     valid[e] = ok ? 1 : 0;
+    covKey[e] = prim::keyOfI64(cov[e]);
   }
 }
 
 // (column << 32 | row) per entry; entries without a row sort last
-__global__ void k_entry_keys(int m, const int64_t* cov, const int32_t* row, const int64_t* uniq,
-                             int nCols, const double* val, uint64_t* key, double* v) {
+__global__ void k_entry_keys(int m, const uint64_t* covKey, const int32_t* row,
+                             const uint64_t* uniq, int nCols, const double* val, uint64_t* key,
+                             double* v) {
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
     const int r = row[e];
     key[e] = r < 0 ? ~0ull
-                   : ((uint64_t)lowerBound(uniq, nCols, cov[e]) << 32) | (uint64_t)(uint32_t)r;
+                   : ((uint64_t)lowerBound(uniq, nCols, covKey[e]) << 32) | (uint64_t)(uint32_t)r;
     v[e] = val ? val[e] : 1.0;
   }
 }
@@ -114,24 +158,60 @@ __global__ void k_nonzero_flags(int m, const double* v, uint8_t* f) {
     f[e] = v[e] != 0.0 ? 1 : 0;
 }
 
-// colPtr[c] = first entry of column c (entries sorted by column); colPtr[nCols] = nnz
-__global__ void k_col_bounds(int nnz, const uint64_t* key, int nCols, int64_t* colPtr) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += gridDim.x * blockDim.x) {
+// colPtr[c] = c * n for the lead columns; colPtr[lead + c] = lead * n + first
+// entry of covariate column c (entries sorted by column); colPtr[J] = nonzeros
+__global__ void k_col_bounds(int64_t nnz, const uint64_t* key, int nCols, int lead, int n,
+                             int64_t* colPtr) {
+  const int64_t base = (int64_t)lead * n;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t c = tid; c < lead; c += stride) colPtr[c] = c * n;
+  if (nnz == 0) {
+    for (int64_t c = tid; c <= nCols; c += stride) colPtr[lead + c] = base;
+    return;
+  }
+  for (int64_t i = tid; i < nnz; i += stride) {
     const int c = (int)(key[i] >> 32);
     const int prev = i == 0 ? -1 : (int)(key[i - 1] >> 32);
-    for (int cc = prev + 1; cc <= c; ++cc) colPtr[cc] = i;
+    for (int cc = prev + 1; cc <= c; ++cc) colPtr[lead + cc] = base + i;
     if (i == nnz - 1)
-      for (int cc = c + 1; cc <= nCols; ++cc) colPtr[cc] = nnz;
+      for (int cc = c + 1; cc <= nCols; ++cc) colPtr[lead + cc] = base + nnz;
   }
 }
 
-// rows and values of the sorted entries; a value other than 1 makes its column SPARSE
-__global__ void k_split(int nnz, const uint64_t* key, const double* v, int32_t* rows, double* vals,
-                        int* sparse) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += gridDim.x * blockDim.x) {
-    rows[i] = (int32_t)(key[i] & 0xffffffffull);
-    vals[i] = v[i];
-    if (v[i] != 1.0) sparse[key[i] >> 32] = 1;
+// rows of the lead columns (their positions) and of the covariate entries; a
+// value other than 1 makes a column SPARSE
+__global__ void k_rows_and_sparse(int64_t nnz, const uint64_t* key, const double* v, int lead,
+                                  int n, int32_t* rows, int* sparse) {
+  const int64_t leadN = (int64_t)lead * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < leadN + nnz;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < leadN) {
+      rows[i] = (int32_t)(i % n);
+      continue;
+    }
+    const int64_t e = i - leadN;
+    rows[i] = (int32_t)(key[e] & 0xffffffffull);
+    if (v[e] != 1.0) sparse[key[e] >> 32] = 1;
+  }
+}
+
+__global__ void k_formats(int J, int lead, bool offset, const int* sparse, int8_t* fmt) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < J; c += gridDim.x * blockDim.x)
+    fmt[c] = c < lead ? (int8_t)(offset && c == 0 ? CCDH_DENSE : CCDH_INTERCEPT)
+                      : (int8_t)(sparse[c - lead] ? CCDH_SPARSE : CCDH_INDICATOR);
+}
+
+// values: [offset column from the host] [intercept 1.0] [entry values]
+__global__ void k_values(int64_t nnz, const double* v, int lead, bool offset, int n, double* vals) {
+  const int64_t leadN = (int64_t)lead * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < leadN + nnz;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < leadN) {
+      if (!(offset && i < n)) vals[i] = 1.0;
+      continue;
+    }
+    vals[i] = v[i - leadN];
   }
 }
 
@@ -143,6 +223,8 @@ int dataCreateLong(const ccdh_long_desc* in, ccdh_data** out) {
   if (model != CCDH_LOGISTIC && model != CCDH_POISSON && model != CCDH_CONDITIONAL_LOGISTIC &&
       model != CCDH_COX)
     return setError(CCDH_E_INVALID, "model");
+  if (in->precision != CCDH_FP64 && in->precision != CCDH_MIXED)
+    return setError(CCDH_E_INVALID, "unsupported precision");
   if (in->n_rows <= 0 || in->n_rows >= (1ll << 31) || in->n_entries < 0 ||
       in->n_entries >= (1ll << 31))
     return setError(CCDH_E_INVALID, "row or entry count out of range");
@@ -163,6 +245,8 @@ int dataCreateLong(const ccdh_long_desc* in, ccdh_data** out) {
     if (seen.size() > 2) return setError(CCDH_E_INVALID, "Cox model only accepts one outcome type");
   }
   const bool strata = clr || (cox && in->stratum_id);
+  const bool intercept = in->add_intercept && (model == CCDH_LOGISTIC || poisson);
+  const int lead = (poisson ? 1 : 0) + (intercept ? 1 : 0);
 
   hipStream_t s = nullptr;
   HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -173,33 +257,38 @@ int dataCreateLong(const ccdh_long_desc* in, ccdh_data** out) {
       (void)hipStreamDestroy(s);
     }
   } guard{s};
-  DevBuf<char> temp;
-  size_t tempBytes = 0;
-  // runs a hipcub call: size query, (re)allocation, then the call
-  auto cub = [&](auto&& call) -> hipError_t {
-    size_t need = 0;
-    hipError_t e = call((void*)nullptr, need);
-    if (e != hipSuccess) return e;
-    if (need > tempBytes) {
-      if (temp.p) (void)hipFree(temp.p);
-      temp.p = nullptr;
-      e = temp.alloc(need);
-      if (e != hipSuccess) return e;
-      tempBytes = need;
+  // the data set under construction, destroyed unless handed out
+  struct DataGuard {
+    ccdh_data* d = new ccdh_data();
+    ~DataGuard() {
+      if (d) dataDestroy(d);
     }
-    return call((void*)temp.p, tempBytes);
+  } dg;
+  ccdh_data* d = dg.d;
+  d->model = model;
+  d->precision = in->precision;
+  d->K = n;
+  d->hasOffset = poisson ? 1 : 0;
+  auto dalloc = [&](auto** p, size_t count) -> hipError_t {
+    using T = std::remove_pointer_t<std::remove_reference_t<decltype(*p)>>;
+    hipError_t e = hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) d->devBytes += (int64_t)(std::max<size_t>(count, 1) * sizeof(T));
+    return e;
   };
+  prim::Scratch sortMem;
+  int dup = 0;
+  DevBuf<int> found;
+  HIP_CHECK(found.alloc(1));
 
   // ---- 1. row order
-  DevBuf<int64_t> dRowId, dStratum, k64a, k64b;
-  DevBuf<double> dY, dTime, kda, kdb;
+  DevBuf<int64_t> dRowId, dStratum;
+  DevBuf<double> dY, dTime;
+  DevBuf<uint64_t> ka, kb;
   DevBuf<int32_t> perm, perm2;
   HIP_CHECK(dRowId.alloc(n));
   HIP_CHECK(dY.alloc(n));
-  HIP_CHECK(k64a.alloc(n));
-  HIP_CHECK(k64b.alloc(n));
-  HIP_CHECK(kda.alloc(n));
-  HIP_CHECK(kdb.alloc(n));
+  HIP_CHECK(ka.alloc(n));
+  HIP_CHECK(kb.alloc(n));
   HIP_CHECK(perm.alloc(n));
   HIP_CHECK(perm2.alloc(n));
   HIP_CHECK(hipMemcpyAsync(dRowId.p, in->row_id, (size_t)n * 8, hipMemcpyHostToDevice, s));
@@ -215,27 +304,28 @@ int dataCreateLong(const ccdh_long_desc* in, ccdh_data** out) {
   k_iota<<<blocksFor(n), kT, 0, s>>>(n, perm.p);
   HIP_CHECK(hipGetLastError());
   // one stable pass: keys gathered through the current order, sorted with it
+  auto pass = [&](bool alt) {
+    if (alt) {
+      swapBuf(perm, perm2);
+      swapBuf(ka, kb);
+    }
+  };
   auto passI64 = [&](const int64_t* src) -> hipError_t {
-    k_gather<int64_t><<<blocksFor(n), kT, 0, s>>>(n, src, perm.p, k64a.p);
+    k_key_i64<<<blocksFor(n), kT, 0, s>>>(n, src, perm.p, ka.p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    e = cub([&](void* t, size_t& b) {
-      return hipcub::DeviceRadixSort::SortPairs(t, b, k64a.p, k64b.p, perm.p, perm2.p, n, 0, 64, s);
-    });
-    std::swap(perm.p, perm2.p);
+    bool alt = false;
+    e = prim::radixSortPairs<int32_t>(ka.p, kb.p, perm.p, perm2.p, n, sortMem, s, &alt);
+    pass(alt);
     return e;
   };
   auto passF64 = [&](const double* src, bool descending) -> hipError_t {
-    k_gather<double><<<blocksFor(n), kT, 0, s>>>(n, src, perm.p, kda.p);
+    k_key_f64<<<blocksFor(n), kT, 0, s>>>(n, src, perm.p, descending, ka.p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    e = cub([&](void* t, size_t& b) {
-      return descending ? hipcub::DeviceRadixSort::SortPairsDescending(t, b, kda.p, kdb.p, perm.p,
-                                                                       perm2.p, n, 0, 64, s)
-                        : hipcub::DeviceRadixSort::SortPairs(t, b, kda.p, kdb.p, perm.p, perm2.p,
-                                                             n, 0, 64, s);
-    });
-    std::swap(perm.p, perm2.p);
+    bool alt = false;
+    e = prim::radixSortPairs<int32_t>(ka.p, kb.p, perm.p, perm2.p, n, sortMem, s, &alt);
+    pass(alt);
     return e;
   };
   HIP_CHECK(passI64(dRowId.p));
@@ -246,205 +336,207 @@ int dataCreateLong(const ccdh_long_desc* in, ccdh_data** out) {
   if (strata) HIP_CHECK(passI64(dStratum.p));
 
   // ---- 2. outcomes in row order, stratum numbers
-  std::vector<double> hY(n), hTime;
-  std::vector<int32_t> hStrata;
-  k_gather<double><<<blocksFor(n), kT, 0, s>>>(n, dY.p, perm.p, kda.p);
+  HIP_CHECK(dalloc(&d->y, n));
+  k_gather<double><<<blocksFor(n), kT, 0, s>>>(n, dY.p, perm.p, d->y);
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipMemcpyAsync(hY.data(), kda.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
-  if (in->time && (cox || poisson)) {
+  d->hY.resize(n);
+  HIP_CHECK(hipMemcpyAsync(d->hY.data(), d->y, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+  std::vector<double> hTime;
+  if (cox || poisson) {
+    DevBuf<double> t;
+    double* tp = nullptr;
+    if (cox) {
+      HIP_CHECK(dalloc(&d->time, n));
+      tp = d->time;
+    } else {
+      HIP_CHECK(t.alloc(n));
+      tp = t.p;
+    }
+    k_gather<double><<<blocksFor(n), kT, 0, s>>>(n, dTime.p, perm.p, tp);
+    HIP_CHECK(hipGetLastError());
     hTime.resize(n);
-    k_gather<double><<<blocksFor(n), kT, 0, s>>>(n, dTime.p, perm.p, kdb.p);
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(hTime.data(), kdb.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
-  }
-  if (strata) {
-    DevBuf<int32_t> flags, pid;
-    HIP_CHECK(flags.alloc(n));
-    HIP_CHECK(pid.alloc(n));
-    k_gather<int64_t><<<blocksFor(n), kT, 0, s>>>(n, dStratum.p, perm.p, k64a.p);
-    k_change_flags<<<blocksFor(n), kT, 0, s>>>(n, k64a.p, flags.p);
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(cub([&](void* t, size_t& b) {
-      return hipcub::DeviceScan::InclusiveSum(t, b, flags.p, pid.p, n, s);
-    }));
-    hStrata.resize(n);
-    HIP_CHECK(hipMemcpyAsync(hStrata.data(), pid.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(hTime.data(), tp, (size_t)n * 8, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
   }
+  HIP_CHECK(dalloc(&d->strata, n));
+  if (strata) {
+    DevBuf<uint8_t> flags;
+    HIP_CHECK(flags.alloc(n));
+    k_key_i64<<<blocksFor(n), kT, 0, s>>>(n, dStratum.p, perm.p, ka.p);
+    k_change_flags<<<blocksFor(n), kT, 0, s>>>(n, ka.p, flags.p);
+    HIP_CHECK(hipGetLastError());
+    prim::Compactor cmp;
+    int nChanges = 0;
+    HIP_CHECK(cmp.plan(flags.p, n, s, &nChanges));
+    k_stratum_numbers<<<blocksFor(n), kT, 0, s>>>(n, flags.p, cmp.pos, d->strata);
+    HIP_CHECK(hipGetLastError());
+    d->nStrata = nChanges + 1;
+    d->hStrata.resize(n);
+    HIP_CHECK(hipMemcpyAsync(d->hStrata.data(), d->strata, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  } else {
+    prim::k_fill<int32_t><<<blocksFor(n), kT, 0, s>>>(d->strata, n, 0);
+    HIP_CHECK(hipGetLastError());
+    d->nStrata = 1;
+    d->hStrata.assign(n, 0);
+  }
+  d->hOffs = cox ? hTime : std::vector<double>(n, 0.0);
 
   // ---- 3. row map: row ids in row order, sorted, with their rows
   DevBuf<int32_t> rowOf;
-  DevBuf<int> found;
   HIP_CHECK(rowOf.alloc(n));
-  HIP_CHECK(found.alloc(1));
-  k_gather<int64_t><<<blocksFor(n), kT, 0, s>>>(n, dRowId.p, perm.p, k64a.p);
-  k_iota<<<blocksFor(n), kT, 0, s>>>(n, perm2.p);
+  k_key_i64<<<blocksFor(n), kT, 0, s>>>(n, dRowId.p, perm.p, ka.p);
+  k_iota<<<blocksFor(n), kT, 0, s>>>(n, rowOf.p);
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(cub([&](void* t, size_t& b) {
-    return hipcub::DeviceRadixSort::SortPairs(t, b, k64a.p, k64b.p, perm2.p, rowOf.p, n, 0, 64, s);
-  }));
+  {
+    bool alt = false;
+    HIP_CHECK(prim::radixSortPairs<int32_t>(ka.p, kb.p, rowOf.p, perm2.p, n, sortMem, s, &alt));
+    if (alt) {
+      swapBuf(ka, kb);
+      swapBuf(rowOf, perm2);
+    }
+  }
+  const uint64_t* sortedIds = ka.p;
   HIP_CHECK(hipMemsetAsync(found.p, 0, sizeof(int), s));
-  k_adjacent_equal<int64_t><<<blocksFor(n), kT, 0, s>>>(n, k64b.p, found.p);
+  k_adjacent_equal<uint64_t><<<blocksFor(n), kT, 0, s>>>(n, sortedIds, found.p);
   HIP_CHECK(hipGetLastError());
-  int dup = 0;
   HIP_CHECK(hipMemcpyAsync(&dup, found.p, sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   if (dup) return setError(CCDH_E_INVALID, "repeated outcome row id");
 
   // ---- 4./5. entries
-  std::vector<int64_t> hColPtr(1, 0);
-  std::vector<int32_t> hRows;
-  std::vector<double> hVals;
-  std::vector<int> hSparse;
   int nCols = 0;
+  int64_t nnz = 0;
+  DevBuf<uint64_t> key, key2;
+  DevBuf<double> v, v2;
   if (m > 0) {
-    DevBuf<int64_t> dCov, dEntryRow, covValid, covSorted, uniq;
-    DevBuf<double> dVal, v, v2;
+    DevBuf<int64_t> dCov, dEntryRow;
+    DevBuf<double> dVal;
     DevBuf<int32_t> row;
     DevBuf<uint8_t> valid;
-    DevBuf<uint64_t> key, key2;
-    DevBuf<int> nSel;
+    DevBuf<uint64_t> covKey, covA, covB;
     HIP_CHECK(dCov.alloc(m));
     HIP_CHECK(dEntryRow.alloc(m));
     HIP_CHECK(row.alloc(m));
     HIP_CHECK(valid.alloc(m));
-    HIP_CHECK(nSel.alloc(1));
+    HIP_CHECK(covKey.alloc(m));
     HIP_CHECK(hipMemcpyAsync(dCov.p, in->covariate_id, (size_t)m * 8, hipMemcpyHostToDevice, s));
     HIP_CHECK(hipMemcpyAsync(dEntryRow.p, in->entry_row_id, (size_t)m * 8, hipMemcpyHostToDevice, s));
     if (in->value) {
       HIP_CHECK(dVal.alloc(m));
       HIP_CHECK(hipMemcpyAsync(dVal.p, in->value, (size_t)m * 8, hipMemcpyHostToDevice, s));
     }
-    k_entry_rows<<<blocksFor(m), kT, 0, s>>>(m, dEntryRow.p, k64b.p, rowOf.p, n, row.p, valid.p);
+    k_entry_rows<<<blocksFor(m), kT, 0, s>>>(m, dEntryRow.p, sortedIds, rowOf.p, n, dCov.p, row.p,
+                                             valid.p, covKey.p);
     HIP_CHECK(hipGetLastError());
     // column ids present among entries with a row: sorted, unique
-    HIP_CHECK(covValid.alloc(m));
-    HIP_CHECK(covSorted.alloc(m));
-    HIP_CHECK(uniq.alloc(m));
-    HIP_CHECK(cub([&](void* t, size_t& b) {
-      return hipcub::DeviceSelect::Flagged(t, b, dCov.p, valid.p, covValid.p, nSel.p, m, s);
-    }));
+    prim::Compactor cmp;
     int nValid = 0;
-    HIP_CHECK(hipMemcpyAsync(&nValid, nSel.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
+    HIP_CHECK(cmp.plan(valid.p, m, s, &nValid));
     if (nValid > 0) {
-      HIP_CHECK(cub([&](void* t, size_t& b) {
-        return hipcub::DeviceRadixSort::SortKeys(t, b, covValid.p, covSorted.p, nValid, 0, 64, s);
-      }));
-      HIP_CHECK(cub([&](void* t, size_t& b) {
-        return hipcub::DeviceSelect::Unique(t, b, covSorted.p, uniq.p, nSel.p, nValid, s);
-      }));
-      HIP_CHECK(hipMemcpyAsync(&nCols, nSel.p, sizeof(int), hipMemcpyDeviceToHost, s));
-      HIP_CHECK(hipStreamSynchronize(s));
+      HIP_CHECK(covA.alloc(nValid));
+      HIP_CHECK(covB.alloc(nValid));
+      HIP_CHECK(cmp.apply(covKey.p, valid.p, m, covA.p, s));
+      bool alt = false;
+      HIP_CHECK(prim::radixSortPairs<int32_t>(covA.p, covB.p, nullptr, nullptr, nValid, sortMem, s,
+                                              &alt));
+      if (alt) swapBuf(covA, covB);
+      DevBuf<uint8_t> first;
+      HIP_CHECK(first.alloc(nValid));
+      k_unique_flags<<<blocksFor(nValid), kT, 0, s>>>(nValid, covA.p, first.p);
+      HIP_CHECK(hipGetLastError());
+      prim::Compactor ucmp;
+      HIP_CHECK(ucmp.plan(first.p, nValid, s, &nCols));
+      HIP_CHECK(ucmp.apply(covA.p, first.p, nValid, covB.p, s));
+      const uint64_t* uniq = covB.p;
       // entries sorted by (column, row); those without a row last
       HIP_CHECK(key.alloc(m));
       HIP_CHECK(key2.alloc(m));
       HIP_CHECK(v.alloc(m));
       HIP_CHECK(v2.alloc(m));
-      k_entry_keys<<<blocksFor(m), kT, 0, s>>>(m, dCov.p, row.p, uniq.p, nCols, dVal.p, key.p, v.p);
+      k_entry_keys<<<blocksFor(m), kT, 0, s>>>(m, covKey.p, row.p, uniq, nCols, dVal.p, key.p, v.p);
       HIP_CHECK(hipGetLastError());
-      HIP_CHECK(cub([&](void* t, size_t& b) {
-        return hipcub::DeviceRadixSort::SortPairs(t, b, key.p, key2.p, v.p, v2.p, m, 0, 64, s);
-      }));
+      HIP_CHECK(prim::radixSortPairs<double>(key.p, key2.p, v.p, v2.p, m, sortMem, s, &alt));
+      if (alt) {
+        swapBuf(key, key2);
+        swapBuf(v, v2);
+      }
       // loadMultipleX: a repeated (row, column) entry is an error, zero values included
       HIP_CHECK(hipMemsetAsync(found.p, 0, sizeof(int), s));
-      k_adjacent_equal<uint64_t><<<blocksFor(nValid), kT, 0, s>>>(nValid, key2.p, found.p);
+      k_adjacent_equal<uint64_t><<<blocksFor(nValid), kT, 0, s>>>(nValid, key.p, found.p);
       HIP_CHECK(hipGetLastError());
       HIP_CHECK(hipMemcpyAsync(&dup, found.p, sizeof(int), hipMemcpyDeviceToHost, s));
       HIP_CHECK(hipStreamSynchronize(s));
       if (dup) return setError(CCDH_E_INVALID, "Repeated row-column entry");
       // zero values are not stored
-      k_nonzero_flags<<<blocksFor(nValid), kT, 0, s>>>(nValid, v2.p, valid.p);
+      k_nonzero_flags<<<blocksFor(nValid), kT, 0, s>>>(nValid, v.p, valid.p);
       HIP_CHECK(hipGetLastError());
-      HIP_CHECK(cub([&](void* t, size_t& b) {
-        return hipcub::DeviceSelect::Flagged(t, b, key2.p, valid.p, key.p, nSel.p, nValid, s);
-      }));
-      HIP_CHECK(cub([&](void* t, size_t& b) {
-        return hipcub::DeviceSelect::Flagged(t, b, v2.p, valid.p, v.p, nSel.p, nValid, s);
-      }));
-      int nnz = 0;
-      HIP_CHECK(hipMemcpyAsync(&nnz, nSel.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      prim::Compactor zcmp;
+      int kept = 0;
+      HIP_CHECK(zcmp.plan(valid.p, nValid, s, &kept));
+      HIP_CHECK(zcmp.apply(key.p, valid.p, nValid, key2.p, s));
+      HIP_CHECK(zcmp.apply(v.p, valid.p, nValid, v2.p, s));
+      swapBuf(key, key2);
+      swapBuf(v, v2);
+      nnz = kept;
       HIP_CHECK(hipStreamSynchronize(s));
-      DevBuf<int64_t> colPtr;
-      DevBuf<int32_t> rows;
-      DevBuf<double> vals;
-      DevBuf<int> sparse;
-      HIP_CHECK(colPtr.alloc(nCols + 1));
-      HIP_CHECK(rows.alloc(nnz));
-      HIP_CHECK(vals.alloc(nnz));
-      HIP_CHECK(sparse.alloc(nCols));
-      HIP_CHECK(hipMemsetAsync(colPtr.p, 0, (size_t)(nCols + 1) * 8, s));
-      HIP_CHECK(hipMemsetAsync(sparse.p, 0, (size_t)nCols * sizeof(int), s));
-      if (nnz > 0) {
-        k_col_bounds<<<blocksFor(nnz), kT, 0, s>>>(nnz, key.p, nCols, colPtr.p);
-        k_split<<<blocksFor(nnz), kT, 0, s>>>(nnz, key.p, v.p, rows.p, vals.p, sparse.p);
-        HIP_CHECK(hipGetLastError());
-      }
-      hColPtr.resize(nCols + 1);
-      hRows.resize(nnz);
-      hSparse.resize(nCols);
-      HIP_CHECK(hipMemcpyAsync(hColPtr.data(), colPtr.p, (size_t)(nCols + 1) * 8,
-                               hipMemcpyDeviceToHost, s));
-      HIP_CHECK(hipMemcpyAsync(hSparse.data(), sparse.p, (size_t)nCols * sizeof(int),
-                               hipMemcpyDeviceToHost, s));
-      if (nnz > 0)
-        HIP_CHECK(hipMemcpyAsync(hRows.data(), rows.p, (size_t)nnz * 4, hipMemcpyDeviceToHost, s));
-      HIP_CHECK(hipStreamSynchronize(s));
-      // values only when a column keeps them (an all-INDICATOR design needs none)
-      if (nnz > 0 && std::find(hSparse.begin(), hSparse.end(), 1) != hSparse.end()) {
-        hVals.resize(nnz);
-        HIP_CHECK(hipMemcpyAsync(hVals.data(), vals.p, (size_t)nnz * 8, hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipStreamSynchronize(s));
-      }
     }
   }
 
   // ---- the data set: [log time offset (Poisson)] [intercept (LR, Poisson)] covariates
-  const bool intercept = in->add_intercept && (model == CCDH_LOGISTIC || poisson);
-  const int lead = (poisson ? 1 : 0) + (intercept ? 1 : 0);
   const int J = lead + nCols;
-  const int64_t nnzCov = (int64_t)hRows.size();
-  const bool withVals = poisson || !hVals.empty();  // a DENSE offset or a SPARSE column
-  std::vector<int64_t> colPtr(J + 1, 0);
-  std::vector<int32_t> rows((size_t)lead * n + nnzCov);
-  std::vector<double> vals(withVals ? rows.size() : 0);
-  std::vector<int8_t> fmt(J);
-  int64_t at = 0;
-  for (int c = 0; c < lead; ++c) {
-    const bool offset = poisson && c == 0;
-    fmt[c] = offset ? CCDH_DENSE : CCDH_INTERCEPT;
-    for (int k = 0; k < n; ++k) {
-      rows[at + k] = k;
-      if (withVals) vals[at + k] = offset ? std::log(hTime[k]) : 1.0;
+  if (J <= 0) return setError(CCDH_E_INVALID, "empty problem");
+  const int64_t total = (int64_t)lead * n + nnz;
+  if (total >= (int64_t)INT32_MAX) return setError(CCDH_E_INVALID, "nnz must be < 2^31");
+  d->J = J;
+  d->nnz = total;
+  DevBuf<int> sparse;
+  HIP_CHECK(sparse.alloc(std::max(nCols, 1)));
+  HIP_CHECK(hipMemsetAsync(sparse.p, 0, (size_t)std::max(nCols, 1) * sizeof(int), s));
+  HIP_CHECK(dalloc(&d->colPtr, (size_t)J + 1));
+  HIP_CHECK(dalloc(&d->rows, (size_t)total));
+  HIP_CHECK(dalloc(&d->fmt, (size_t)J));
+  k_col_bounds<<<blocksFor(std::max<int64_t>(nnz, J + 1)), kT, 0, s>>>(nnz, key.p, nCols, lead, n,
+                                                                       d->colPtr);
+  k_rows_and_sparse<<<blocksFor(total), kT, 0, s>>>(nnz, key.p, v.p, lead, n, d->rows, sparse.p);
+  k_formats<<<blocksFor(J), kT, 0, s>>>(J, lead, poisson, sparse.p, d->fmt);
+  HIP_CHECK(hipGetLastError());
+  d->hColPtr.resize((size_t)J + 1);
+  d->hFmt.resize(J);
+  HIP_CHECK(hipMemcpyAsync(d->hColPtr.data(), d->colPtr, (size_t)(J + 1) * 8,
+                           hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(d->hFmt.data(), d->fmt, (size_t)J, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  // values when a column keeps them: the DENSE offset or a SPARSE column
+  const bool withVals =
+      std::find(d->hFmt.begin(), d->hFmt.end(), (int8_t)CCDH_SPARSE) != d->hFmt.end() || poisson;
+  if (withVals) {
+    HIP_CHECK(dalloc(&d->vals, (size_t)total));
+    std::vector<double> logTime;
+    if (poisson) {
+      logTime.resize(n);
+      for (int k = 0; k < n; ++k) logTime[k] = std::log(hTime[k]);
+      HIP_CHECK(hipMemcpyAsync(d->vals, logTime.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
     }
-    at += n;
-    colPtr[c + 1] = at;
+    k_values<<<blocksFor(total), kT, 0, s>>>(nnz, v.p, lead, poisson, n, d->vals);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s));
   }
-  if (nnzCov > 0) {
-    std::memcpy(rows.data() + at, hRows.data(), nnzCov * sizeof(int32_t));
-    if (!hVals.empty())
-      std::memcpy(vals.data() + at, hVals.data(), nnzCov * sizeof(double));
-    else if (withVals)
-      std::fill(vals.begin() + at, vals.end(), 1.0);
+  // Poisson fixed term per row: -log(y!) as the loop of ModelSpecifics.h:1596-1601
+  if (poisson) {
+    d->hLogFact.resize(n);
+    for (int k = 0; k < n; ++k) {
+      double t = 0.0;
+      for (int i = 2; i <= (int)d->hY[k]; ++i) t += -std::log((double)i);
+      d->hLogFact[k] = t;
+    }
   }
-  for (int c = 0; c < nCols; ++c) {
-    fmt[lead + c] = hSparse[c] ? CCDH_SPARSE : CCDH_INDICATOR;
-    colPtr[lead + c + 1] = at + hColPtr[c + 1];
-  }
-  ccdh_data_desc desc{};
-  desc.model = model;
-  desc.precision = in->precision;
-  desc.K = n;
-  desc.J = J;
-  desc.col_ptr = colPtr.data();
-  desc.rows = rows.data();
-  desc.vals = withVals ? vals.data() : nullptr;
-  desc.fmt = fmt.data();
-  desc.y = hY.data();
-  desc.offs = cox ? hTime.data() : nullptr;
-  desc.strata = strata ? hStrata.data() : nullptr;
-  desc.has_offset = poisson ? 1 : 0;
-  return dataCreate(&desc, out);
+  int rc = buildTablesDevice(d, s);
+  if (rc) return rc;
+  HIP_CHECK(hipStreamSynchronize(s));
+  *out = d;
+  dg.d = nullptr;
+  return 0;
 }
 
 }  // namespace ccdamd

@@ -170,6 +170,20 @@ int ccdh_data_create_long(const ccdh_long_desc* desc, ccdh_data_t* out);
  * (vals may be NULL), fmt [J], y, offs and strata [K]; any output may be NULL. */
 int ccdh_data_get_csc(ccdh_data_t data, int64_t* col_ptr, int32_t* rows, double* vals,
                       int8_t* fmt, double* y, double* offs, int32_t* strata);
+/* Introspection of a data set's derived per-column tables (no reference
+ * counterpart: the reference walks its sorted columns directly; these are this
+ * library's indexes over them, built on the host by ccdh_data_create and on the
+ * device by ccdh_data_create_long).  Conditional logistic: seg_ptr [J+1], seg_off
+ * and seg_key [n_seg] (runs of equal stratum inside each column: offset in the
+ * column, stratum), nz_key [nnz] (stratum of each nonzero), strat_start
+ * [n_strata+1]; every model: shard_row [65] (row shards), col_shard [J*65] (each
+ * column's offset of each shard's first nonzero); not Cox: blk [J] (column
+ * blocks: length at a block start, 0 inside a block, 1 alone).  Any pointer may
+ * be NULL; call with only n_seg and n_strata first to size the arrays. */
+int ccdh_data_get_tables(ccdh_data_t data, int64_t* n_seg, int* n_strata, int64_t* seg_ptr,
+                         int32_t* seg_off, int32_t* seg_key, int32_t* nz_key,
+                         int32_t* strat_start, int32_t* shard_row, int32_t* col_shard,
+                         int8_t* blk);
 
 /* ---- per-coordinate engine (AbstractModelSpecifics virtuals) ---- */
 /* ModelSpecifics ctor + initialize(N,K,J,...) (ModelSpecifics.hpp:1899-1952) */
