@@ -642,18 +642,21 @@ __global__ __launch_bounds__(kThreads) void k_xbeta_rows(int K, const int64_t* p
 
 // MM increments (ModelSpecifics.h:500-517, 571-584) factor into a per-row value
 // r_k = nW_g * oeb_k / denom_g over the row's group g (logistic: nW_k = w_k and
-// denom_k = 1 + oeb_k); a column's gradient is sum r_k x_k and its Hessian
-// sum r_k x_k^2.
+// denom_k = 1 + oeb_k; conditional logistic: the stratum; Cox: the row's pid,
+// whose denominator is the pid's own sum -- a tie group of events or one row --
+// as computeMMGradientAndHessianImpl reads denomPid, not the accumulated risk
+// set; zero-weight rows read slot N, whose event weight is 0); a column's
+// gradient is sum r_k x_k and its Hessian sum r_k x_k^2.
 template <typename R>
 __global__ void k_mm_rows(int model, int K, const R* oeb, const double* kw, const double* nw,
-                          const double* denom, const int32_t* strata, double* r) {
+                          const double* denom, const int32_t* group, double* r) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < K;
        k += (int64_t)gridDim.x * blockDim.x) {
     const double o = (double)oeb[k];
     if (model == M_LOGISTIC) {
       r[k] = kw[k] * o / (1.0 + o);
     } else {
-      const int s = strata[k];
+      const int s = group[k];
       r[k] = nw[s] * o / denom[s];
     }
   }
@@ -1270,6 +1273,10 @@ int setWeights(ccdh_fit* f, const double* w, int useCV) {
     k_nw_cox<<<gridFor(f->N, 1), kThreads, 0, f->stream>>>(f->N, d->y, f->pid, f->pidFirst,
                                                             f->kw, f->nw);
     HIP_TRY(hipGetLastError());
+    // the slot of the zero-weight rows (pid N) holds no events, as setWeights fills
+    // all N + 1 event weights before summing (ModelSpecifics.hpp:447-455)
+    rc = launchFill(f, f->nw + f->N, 1, 0.0);
+    if (rc) return rc;
   } else if (d->model == CCDH_CONDITIONAL_LOGISTIC) {
     k_nw_clr<<<gridFor(d->nStrata, 1), kThreads, 0, f->stream>>>(d->view(), f->kw, f->nw);
     HIP_TRY(hipGetLastError());
@@ -1679,17 +1686,16 @@ int computeXBetaRows(ccdh_fit* f, const double* beta) {
 // per-row factors, one wave per column chunk, then chunk sums per column
 int mmGradHess(ccdh_fit* f, const uint8_t* fixed, double* gh) {
   ccdh_data* d = f->d;
-  if (d->model != CCDH_LOGISTIC && d->model != CCDH_CONDITIONAL_LOGISTIC)
-    return setError(CCDH_E_INVALID,
-                    "MM gradient: not model-specific (logistic and conditional logistic only)");
+  if (d->model == CCDH_POISSON)  // ModelSpecifics.h:1546-1552
+    return setError(CCDH_E_INVALID, "MM gradient: not model-specific (Poisson)");
   if (!fixed || !gh) return setError(CCDH_E_INVALID, "null argument");
   if (!f->weightsSet || !f->statsKnown)
     return setError(CCDH_E_STATE, "MM gradient before weights and statistics");
   int rc = ensureMm(f);
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(f->mmFixed, fixed, d->J, hipMemcpyHostToDevice, f->stream));
-  LAUNCH_R(k_mm_rows, gridFor(d->K), d->model, d->K, OEB(T_), f->kw, f->nw, f->denom, d->strata,
-           f->mmR);
+  LAUNCH_R(k_mm_rows, gridFor(d->K), d->model, d->K, OEB(T_), f->kw, f->nw, f->denom,
+           d->model == CCDH_COX ? (const int32_t*)f->pid : d->strata, f->mmR);
   const int wavesPerBlock = kThreads / kWave;
   if (d->nChunks > 0) {
     HIP_TRY(hipEventRecord(f->mmEv[0], f->stream));

@@ -286,9 +286,8 @@ extern "C" int ccdh_fit_mm(ccdh_data_t data, const ccdh_fit_spec* spec, double* 
   if (data) {
     int model = 0, K = 0, J = 0, hasOffset = 0;
     ccdh_data_shape(data, &model, &K, &J, &hasOffset);
-    if (model != CCDH_LOGISTIC && model != CCDH_CONDITIONAL_LOGISTIC)
-      return setError(CCDH_E_INVALID,
-                      "MM algorithm: not model-specific (logistic and conditional logistic only)");
+    if (model == CCDH_POISSON)  // the reference throws (ModelSpecifics.h:1546-1552)
+      return setError(CCDH_E_INVALID, "MM algorithm: not model-specific (Poisson)");
     // FP32 row state puts ~1e-7 relative noise into the log posterior, which trips
     // the strict non-increasing check long before the fit converges
     if (ccdh_data_precision(data) != CCDH_FP64)

@@ -209,7 +209,9 @@ int ccdh_compute_xbeta(ccdh_fit_t fit, const double* beta);
  * bool) (:70; ModelSpecifics.hpp:776-889): every column's MM gradient and Hessian
  * from the current statistics into gh_out[2j], gh_out[2j+1]; (0, 0) where fixed[j].
  * The increments weigh by nW whatever use_weights says (ModelSpecifics.h:500-517,
- * 571-584).  Logistic and conditional logistic models only (CCDH_E_INVALID otherwise). */
+ * 571-584); Cox divides by each pid's own denominator (a tie group of events, or one
+ * row), as the reference's Survival increments read denomPid.  Poisson is refused
+ * with CCDH_E_INVALID, as the reference throws (ModelSpecifics.h:1546-1552). */
 int ccdh_mm_grad_hess(ccdh_fit_t fit, const uint8_t* fixed, int use_weights, double* gh_out);
 /* computeRemainingStatistics(bool) (:81) */
 int ccdh_remaining_stats(ccdh_fit_t fit, int use_weights);
@@ -237,9 +239,9 @@ int ccdh_fit_plain(ccdh_data_t data, const ccdh_fit_spec* spec, double* beta_out
  * free coordinate from one set of statistics (no bounding box), then recomputes xBeta
  * from the transpose and the statistics.  A decrease of the log posterior stops the fit
  * with CCDH_E_FIT ("Non-increasing!"); beta_out then holds the coefficients reached and
- * result->iterations the iteration.  Logistic and conditional logistic models, FP64 data
- * (on the mixed path the log posterior's FP32 noise trips that check long before the fit
- * converges). */
+ * result->iterations the iteration.  Logistic, conditional logistic and Cox models (not
+ * Poisson), FP64 data (on the mixed path the log posterior's FP32 noise trips that check
+ * long before the fit converges). */
 int ccdh_fit_mm(ccdh_data_t data, const ccdh_fit_spec* spec, double* beta_out,
                 ccdh_fit_result* result);
 

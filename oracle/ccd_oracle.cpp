@@ -491,8 +491,12 @@ struct Engine {
   }
 
   // computeMMGradientAndHessianImpl (ModelSpecifics.hpp:848-889) with the
-  // increments of ModelSpecifics.h:500-517 (conditional models) and 571-584
-  // (logistic): weight nW and denominator of the row's group, no 1-t factor.
+  // increments of ModelSpecifics.h:500-517 (Survival: conditional logistic and
+  // Cox) and 571-584 (logistic): weight nW and denominator of the row's group, no
+  // 1-t factor.  For Cox the group is the row's pid and the denominator the pid's
+  // own sum (denomPid, not the accumulated risk set); a zero-weight row's pid is N,
+  // whose event weight is 0 and whose denominator slot keeps what an earlier
+  // statistics pass left there (0 in a fresh engine: the term is then 0/0).
   void mmGradHess(int j, double* og, double* oh) const {
     const bool ind = isIndicatorFmt(d.fmt[j]);
     double g = 0.0, h = 0.0;
@@ -624,9 +628,8 @@ typedef struct {
 //   heldout   nullable; when non-null, the predictive log likelihood of these weights
 //             is evaluated after the fit (CV scoring)
 // Returns 0; with the MM algorithm -1 for a model without MM increments (Poisson:
-// ModelSpecifics.h:1546-1552 throws; Cox: not offered, its MM increments divide by
-// tie-group sums rather than risk-set sums) and -2 when the log posterior
-// decreased ("Non-increasing!", CyclicCoordinateDescent.cpp:1033-1046).
+// ModelSpecifics.h:1546-1552 throws) and -2 when the log posterior decreased
+// ("Non-increasing!", CyclicCoordinateDescent.cpp:1033-1046).
 int orc_fit(const orc_data* od, const int* prior_type, const double* prior_var,
             const uint8_t* fixed, double* beta, const double* weights, const double* heldout,
             const orc_control* ctl, orc_result* out) {
@@ -637,7 +640,7 @@ int orc_fit(const orc_data* od, const int* prior_type, const double* prior_var,
   const bool useCV = weights != nullptr;
   const bool hasOffset = ctl->has_offset != 0;  // CyclicCoordinateDescent.cpp:256-264
   const bool mm = ctl->algorithm == 1;
-  if (mm && d.model != M_LOGISTIC && d.model != M_CLR) return -1;
+  if (mm && d.model == M_POISSON) return -1;
   std::vector<double> allDelta(J, 0.0);
 
   std::vector<double> b(beta, beta + J), bound(J, ctl->initial_bound);
@@ -836,7 +839,7 @@ int orc_mm_eval(const orc_data* od, const double* beta, const double* weights, d
                 double* h) {
   Data d{od->K, od->J, od->col_ptr, od->rows, od->vals, od->fmt, od->y, od->offs, od->strata,
          od->model};
-  if (d.model != M_LOGISTIC && d.model != M_CLR) return -1;
+  if (d.model == M_POISSON) return -1;
   Engine e(d);
   const bool useCV = weights != nullptr;
   e.setWeights(weights, useCV);

@@ -6,6 +6,7 @@ the same coefficients (chunked sums against sequential ones); xBeta from the
 transpose within 1e-14 relative (the same per-row order); MM fits within 1e-8
 relative on coefficients and log likelihood with the same iteration count and
 return flag, and on overshooting designs the same "Non-increasing!" iteration.
+Cox (Breslow ties) runs the reference's Survival increments; Poisson is refused.
 """
 import numpy as np
 import pytest
@@ -16,7 +17,7 @@ from ccd_amd.data import (CONDITIONAL_LOGISTIC, INDICATOR, LOGISTIC, PRIOR_LAPLA
 from oracle import oracle as orc
 from tests.test_gpu_plain import check_fit
 from tests.test_oracle import _small_problems
-from tests.test_oracle_mm import _mm_problems, _weights
+from tests.test_oracle_mm import _cox_mm_problems, _mm_problems, _weights
 
 pytestmark = pytest.mark.gpu
 
@@ -111,23 +112,82 @@ def test_mm_fit_matches_oracle(name, p, prior, var, weighted):
         d.close()
 
 
-def test_mm_refused_for_poisson_and_cox():
-    probs = dict(_small_problems())
-    for name in ("poisson", "cox"):
-        p = probs[name]
+def test_mm_refused_for_poisson():
+    p = dict(_small_problems())["poisson"]
+    d = api.Data(p)
+    fh = api.FitHandle(d)
+    try:
+        with pytest.raises(api.CcdhError, match="not model-specific"):
+            api.fit_plain(d, default_spec(p), algorithm="mm")
+        fh.set_weights(None)
+        fh.fixed_terms()
+        fh.compute_xbeta(np.zeros(p.J))
+        fh.remaining_stats()
+        with pytest.raises(api.CcdhError, match="not model-specific"):
+            fh.mm_grad_hess()
+    finally:
+        fh.close()
+        d.close()
+
+
+_COX = _cox_mm_problems()
+
+
+@pytest.mark.parametrize("name,p", _COX, ids=_IDS)
+@pytest.mark.parametrize("weights", ["none", "positive", "zeros"])
+def test_cox_mm_grad_hess_matches_oracle(name, p, weights):
+    """Cox MM increments over each pid's own denominator; with zero weights a fresh
+    fit reads the unwritten slot of the zero-weight rows (0/0) in the columns that
+    touch them, as the oracle's fresh engine does."""
+    rng = np.random.default_rng(25)
+    beta = rng.normal(0, 0.4, p.J)
+    w = {"none": None, "positive": rng.choice([0.5, 1.0, 2.0], p.K),
+         "zeros": (rng.random(p.K) < 0.8).astype(float)}[weights]
+    fixed = np.zeros(p.J, np.uint8)
+    d = api.Data(p)
+    fh = api.FitHandle(d)
+    try:
+        fh.set_weights(w, use_cv=w is not None)
+        fh.fixed_terms(use_cv=w is not None)
+        fh.compute_xbeta(beta)
+        fh.remaining_stats(use_w=True)
+        g, h = fh.mm_grad_hess(fixed, use_w=True)
+    finally:
+        fh.close()
+        d.close()
+    g_c, h_c = orc.mm_evaluate(p, beta, weights=w)
+    np.testing.assert_array_equal(np.isnan(g), np.isnan(g_c))
+    np.testing.assert_array_equal(np.isnan(h), np.isnan(h_c))
+    ok = ~np.isnan(g_c)
+    scale = np.abs(g_c[ok]).max() + np.abs(h_c[ok]).max() + 1.0 if ok.any() else 1.0
+    np.testing.assert_allclose(g[ok], g_c[ok], rtol=1e-11, atol=1e-12 * scale)
+    np.testing.assert_allclose(h[ok], h_c[ok], rtol=1e-11, atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize("name,p", _COX, ids=_IDS)
+@pytest.mark.parametrize("prior,var", [(PRIOR_NORMAL, 1.0), (PRIOR_LAPLACE, 0.1)])
+def test_cox_mm_fit_matches_oracle(name, p, prior, var):
+    """From zero coefficients every Cox MM increment cancels, so the fit stays put;
+    from a warm start the tie groups move it.  Either way the same trajectory,
+    iterations and return flag, or the same "Non-increasing!" iteration."""
+    for start in (np.zeros(p.J), np.random.default_rng(26).normal(0, 0.3, p.J)):
+        sp = default_spec(p, prior=prior, variance=var)
+        sp.beta0 = start.copy()
+        bc, ic = orc.fit(p, sp, algorithm="mm")
         d = api.Data(p)
-        fh = api.FitHandle(d)
         try:
-            with pytest.raises(api.CcdhError, match="not model-specific"):
-                api.fit_plain(d, default_spec(p), algorithm="mm")
-            fh.set_weights(None)
-            fh.fixed_terms()
-            fh.compute_xbeta(np.zeros(p.J))
-            fh.remaining_stats()
-            with pytest.raises(api.CcdhError, match="not model-specific"):
-                fh.mm_grad_hess()
+            if ic["status"] == -2:
+                with pytest.raises(api.NonIncreasing) as err:
+                    api.fit_plain(d, sp, algorithm="mm")
+                assert err.value.iterations == ic["iterations"]
+                np.testing.assert_allclose(err.value.beta, bc, rtol=1e-8, atol=1e-10)
+            else:
+                assert ic["status"] == 0
+                bg, ig = api.fit_plain(d, sp, algorithm="mm")
+                check_fit(bg, ig, bc, ic)
+                if not start.any():
+                    np.testing.assert_array_equal(bg, 0.0)
         finally:
-            fh.close()
             d.close()
 
 
