@@ -133,7 +133,7 @@ class BatchOptions(ctypes.Structure):
                 ("use_graph", ctypes.c_int), ("max_gang", ctypes.c_int),
                 ("gang_min_nnz", ctypes.c_int64), ("pass_items", ctypes.c_int),
                 ("autotune", ctypes.c_int), ("quantum_ms", ctypes.c_double),
-                ("bundle", ctypes.c_int)]
+                ("bundle", ctypes.c_int), ("solo", ctypes.c_int)]
 
 
 class ServerOptions(ctypes.Structure):
@@ -408,9 +408,11 @@ def mm_last_stats():
 
 
 def fit_batch(data: Data, specs, heldouts=None, threads=0, sweeps_per_sync=0, use_graph=1,
-              max_gang=0, gang_min_nnz=0, pass_items=0, autotune=0, quantum_ms=0.0, bundle=0):
+              max_gang=0, gang_min_nnz=0, pass_items=0, autotune=0, quantum_ms=0.0, bundle=0,
+              solo=0):
     """Many fits over one resident data set with device-resident sweeps; a fit
-    runs on up to `max_gang` workgroups when fewer fits than CUs remain."""
+    runs on up to `max_gang` workgroups when fewer fits than CUs remain (solo=-1:
+    a gang sweeps every column together, none on its lead alone)."""
     lib = load_library()
     n = len(specs)
     keep = []
@@ -420,7 +422,7 @@ def fit_batch(data: Data, specs, heldouts=None, threads=0, sweeps_per_sync=0, us
     betas = np.zeros((n, data.problem.J))
     res = (FitResult * n)()
     opt = BatchOptions(threads, sweeps_per_sync, use_graph, max_gang, gang_min_nnz, pass_items,
-                       autotune, quantum_ms, bundle)
+                       autotune, quantum_ms, bundle, solo)
     _check(lib.ccdh_batch_fit(data.handle, n, arr, _p(betas), res, ctypes.byref(opt)))
     return betas, [r.as_dict() for r in res]
 

@@ -48,92 +48,184 @@ using namespace ccdamd;
 
 namespace {
 
-constexpr int kItems = 4;  // Cox pids loaded together per lane in a scan pass
+constexpr int kIT = 4;           // nonzeros per lane of a single-pass column
+constexpr int kITLDefault = 8;  // long-column pass width (nonzeros per lane); 4 is the alternative
 
-// ------------------------------------------------------------ Cox pieces
-// Is nonzero p (of a column starting at cb) the first of its pid's run within
-// [lo, p]?  Rows are sorted by pid; rows of pid N (zero weight) are skipped.
-template <typename R>
-__device__ __forceinline__ bool coxHead(const DevX& d, int fmt, int64_t cb, int64_t lo, int64_t p,
-                                        const int32_t* pid, int N, int* myPid) {
-  const int pp = pid[colRow(d, fmt, p, cb)];
-  *myPid = pp;
-  if (pp >= N) return false;
-  for (int64_t q = p - 1; q >= lo; --q) {
-    const int pq = pid[colRow(d, fmt, q, cb)];
-    if (pq < N) return pq != pp;
+// ------------------------------------------------------------ Cox column passes
+// Numerators and updates of a Cox column part, in passes of blockDim * kIT nonzeros.
+// Entry li = it * blockDim + lane of a pass is nonzero c0 + li of the part (so
+// each index load of a wave reads 64 consecutive entries), with its pid (key; N
+// for a zero-weight row, whose values are 0 and which belongs to no run) and its
+// values.  Rows are sorted by pid, so a pid's entries form one run of the part,
+// with zero-weight entries possibly inside.  The run's last entry (no later
+// entry of the pass carries a pid, or the next one carries another) accumulates
+// the run forward in column order, as the reference does row by row --
+// numerators from 0 (MODE 0), a pid's denominator from its current value
+// (MODE 1) -- and writes the result.  A run that may go on past the pass is
+// carried: its running values pass to the next pass, where the run's last entry
+// continues them, or thread 0 writes them out when the pass starts with another
+// pid.  `key` is an LDS array of the pass size; S.cValid[0] = 0 before the
+// first pass of a part.
+template <int MODE, typename R, class LDS>
+__device__ __forceinline__ void coxPassRuns(const Slot<R>& s, LDS& S, int* key, int n, bool final,
+                                            bool two, const int (&k)[kIT], const double (&v1)[kIT],
+                                            const double (&v2)[kIT]) {
+  const int N = s.N;
+  auto write = [&](int pid, double a1, double a2) {
+    if (MODE == 0) {
+      s.numer[pid] = a1;
+      if (two) s.numer2[pid] = a2;
+    } else {
+      s.grp[pid].den = a1;
+    }
+  };
+#pragma unroll
+  for (int it = 0; it < kIT; ++it) {
+    const int li = it * blockDim.x + threadIdx.x;
+    if (li < n) {
+      key[li] = k[it];
+      S.v1[li] = v1[it];
+      if (two) S.v2[li] = v2[it];
+    }
   }
-  return true;
+  __syncthreads();
+  const bool cin = S.cValid[0] != 0;
+  const int ck = S.cKey;
+  if (threadIdx.x == 0) {
+    int q = 0;
+    while (q < n && key[q] >= N) ++q;
+    // the carried run ended with the last pass: the pass starts with another pid
+    // (or the part ends here without one)
+    if (cin && (q < n ? key[q] != ck : final)) write(ck, S.c1[0], S.c2[0]);
+    if (q == n && !final) {  // no pid in this pass: the carried run goes on
+      S.n1[0] = S.c1[0];
+      S.n2[0] = S.c2[0];
+      S.nKey = ck;
+      S.nValid[0] = S.cValid[0];
+    } else if (final) {
+      S.nValid[0] = 0;
+    }  // otherwise the pass's last pid entry sets the carry below
+  }
+#pragma unroll
+  for (int it = 0; it < kIT; ++it) {
+    const int li = it * blockDim.x + threadIdx.x;
+    const int p = k[it];
+    if (li >= n || p >= N) continue;
+    int q = li + 1;
+    while (q < n && key[q] >= N) ++q;
+    if (q < n && key[q] == p) continue;  // not the run's last entry in this pass
+    int s0 = li;
+    bool toStart = true;  // the run reaches back to the pass's first entry
+    for (int r = li - 1; r >= 0; --r) {
+      const int kr = key[r];
+      if (kr < N && kr != p) {
+        toStart = false;
+        break;
+      }
+      s0 = r;
+    }
+    double a1, a2 = 0.0;
+    if (toStart && cin && ck == p) {
+      a1 = S.c1[0];
+      a2 = S.c2[0];
+    } else {
+      a1 = MODE == 0 ? 0.0 : s.grp[p].den;
+    }
+    for (int r = s0; r <= li; ++r) {
+      a1 += S.v1[r];
+      if (two) a2 += S.v2[r];
+    }
+    if (q == n && !final) {  // the run may go on in the next pass
+      S.n1[0] = a1;
+      S.n2[0] = a2;
+      S.nKey = p;
+      S.nValid[0] = 1;
+    } else {
+      write(p, a1, a2);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    S.c1[0] = S.n1[0];
+    S.c2[0] = S.n2[0];
+    S.cKey = S.nKey;
+    S.cValid[0] = S.nValid[0];
+  }
 }
 
-// Numerators of the pids a column touches, from its nonzeros [b, e) (column
-// start cb): the first nonzero of each pid's run sums the run
-// (computeNumeratorForGradient, ModelSpecifics.hpp:1353-1464).  A part [b, e)
-// starts and ends at tie-group boundaries, so no run crosses it.
-template <typename R>
-__device__ __forceinline__ void coxNumerators(const Slot<R>& s, int fmt, int64_t cb, int64_t b,
-                                              int64_t e, int useW) {
+// Numerators of the pids a column part [b, e) touches (computeNumeratorForGradient,
+// ModelSpecifics.hpp:1353-1464), in passes of blockDim * kIT nonzeros.
+template <typename R, class LDS>
+__device__ __forceinline__ void coxNumeratorsLds(const Slot<R>& s, LDS& S, int* key, int fmt,
+                                                 int64_t cb, int64_t b, int64_t e, int useW) {
   const DevX& d = s.d;
-  const double* kw = s.kw;
-  const int32_t* pid = s.pid;
   const int N = s.N;
   const bool ind = isIndicator(fmt);
-  for (int64_t p = b + threadIdx.x; p < e; p += blockDim.x) {
-    int pp;
-    if (!coxHead<R>(d, fmt, cb, b, p, pid, N, &pp)) continue;
-    double n1 = 0.0, n2 = 0.0;
-    for (int64_t q = p; q < e; ++q) {
-      const int k = colRow(d, fmt, q, cb);
-      const int pq = pid[k];
-      if (pq >= N) continue;
-      if (pq != pp) break;
-      const double x = (double)colValue<R>(d, fmt, q);
-      const double o = (double)s.row[k].e;
-      n1 += useW ? kw[k] * (o * x) : o * x;
-      if (!ind) n2 += useW ? kw[k] * (o * x * x) : o * x * x;
+  const int64_t P = (int64_t)blockDim.x * kIT;
+  if (threadIdx.x == 0) S.cValid[0] = 0;
+  for (int64_t c0 = b; c0 < e; c0 += P) {
+    const int n = (int)min(P, e - c0);
+    int k[kIT];
+    double v1[kIT], v2[kIT];
+#pragma unroll
+    for (int it = 0; it < kIT; ++it) {
+      const int li = it * blockDim.x + threadIdx.x;
+      k[it] = N;
+      v1[it] = v2[it] = 0.0;
+      if (li < n) {
+        const int64_t p = c0 + li;
+        const int r = colRow(d, fmt, p, cb);
+        const int pp = s.pid[r];
+        k[it] = pp;
+        if (pp < N) {
+          const double x = (double)colValue<R>(d, fmt, p);
+          const double o = (double)s.row[r].e;
+          v1[it] = useW ? s.kw[r] * (o * x) : o * x;
+          if (!ind) v2[it] = useW ? s.kw[r] * (o * x * x) : o * x * x;
+        }
+      }
     }
-    s.numer[pp] = n1;
-    if (!ind) s.numer2[pp] = n2;
+    coxPassRuns<0, R>(s, S, key, n, c0 + P >= e, !ind, k, v1, v2);
   }
 }
 
-// xBeta, exp(xBeta) of the rows of nonzeros [b, e) and the denominators of
-// their pids (updateXBeta, ModelSpecifics.hpp:1524-1586).
-template <typename R>
-__device__ __forceinline__ void updCox(const Slot<R>& s, int fmt, int64_t cb, int64_t b, int64_t e,
-                                       int useW, double delta) {
+// xBeta, exp(xBeta) of the rows of a column part [b, e) and the denominators of
+// their pids (updateXBeta, ModelSpecifics.hpp:1524-1586), in passes as above.
+template <typename R, class LDS>
+__device__ __forceinline__ void updCoxLds(const Slot<R>& s, LDS& S, int* key, int fmt, int64_t cb,
+                                          int64_t b, int64_t e, int useW, double delta) {
   const DevX& d = s.d;
-  RowSt<R>* rowSt = s.row;
-  const double* kw = s.kw;
-  const int32_t* pid = s.pid;
   const int N = s.N;
   const R rd = (R)delta;
-  for (int64_t p = b + threadIdx.x; p < e; p += blockDim.x) {
-    int pp;
-    if (!coxHead<R>(d, fmt, cb, b, p, pid, N, &pp)) {
-      if (pp >= N) {
-        const int k = colRow(d, fmt, p, cb);
-        const R nx = rowSt[k].x + rd * colValue<R>(d, fmt, p);
-        rowSt[k] = RowSt<R>{nx, rexp(nx)};
+  const int64_t P = (int64_t)blockDim.x * kIT;
+  if (threadIdx.x == 0) S.cValid[0] = 0;
+  for (int64_t c0 = b; c0 < e; c0 += P) {
+    const int n = (int)min(P, e - c0);
+    int k[kIT];
+    double v1[kIT], v2[kIT];
+#pragma unroll
+    for (int it = 0; it < kIT; ++it) {
+      const int li = it * blockDim.x + threadIdx.x;
+      k[it] = N;
+      v1[it] = v2[it] = 0.0;
+      if (li < n) {
+        const int64_t p = c0 + li;
+        const int r = colRow(d, fmt, p, cb);
+        const int pp = s.pid[r];
+        const RowSt<R> old = s.row[r];
+        const R nx = old.x + rd * colValue<R>(d, fmt, p);
+        const R o = rexp(nx);
+        s.row[r] = RowSt<R>{nx, o};
+        k[it] = pp;
+        if (pp < N) {
+          const double w = useW ? s.kw[r] : 1.0;
+          const double oldE = useW ? w * (double)old.e : (double)old.e;
+          const double newE = useW ? w * (double)o : (double)o;
+          v1[it] = newE - oldE;
+        }
       }
-      continue;
     }
-    double den = s.grp[pp].den;
-    for (int64_t q = p; q < e; ++q) {
-      const int k = colRow(d, fmt, q, cb);
-      const int pq = pid[k];
-      if (pq >= N) continue;
-      if (pq != pp) break;
-      const RowSt<R> old = rowSt[k];
-      const R nx = old.x + rd * colValue<R>(d, fmt, q);
-      const double w = useW ? kw[k] : 1.0;
-      const double oldE = useW ? w * (double)old.e : (double)old.e;
-      const R o = rexp(nx);
-      rowSt[k] = RowSt<R>{nx, o};
-      const double newE = useW ? w * (double)o : (double)o;
-      den += newE - oldE;
-    }
-    s.grp[pp].den = den;
+    coxPassRuns<1, R>(s, S, key, n, c0 + P >= e, false, k, v1, v2);
   }
 }
 
@@ -160,8 +252,6 @@ __device__ __forceinline__ bool quantumOver(long long deadline) {
 // xBeta and stratum terms in registers from gather to update; longer columns
 // write one record per nonzero in the gather pass and the update pass reads the
 // records back in one coalesced hop.
-constexpr int kIT = 4;
-constexpr int kITLDefault = 8;  // long-column pass width (nonzeros per lane); 4 is the alternative
 
 template <int NMAX>
 struct PassLds {
@@ -171,6 +261,7 @@ struct PassLds {
   unsigned char start[NMAX];
   double c1[kMaxBundle], c2[kMaxBundle], n1[kMaxBundle], n2[kMaxBundle];
   int cValid[kMaxBundle], nValid[kMaxBundle];
+  int cKey, nKey;  // Cox: pid of the run carried into / out of a pass
 };
 
 // One lane's items of a pass.  In a slot of F fits (a bundle) lane u of a pass
@@ -685,7 +776,7 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
   // rank's row writes before the meeting before every read after it.  A lead
   // whose quantum runs out inside a run stops there and tells the others when
   // they meet it at the run's end.
-  const int64_t soloMax = multi ? CH : 0;
+  const int64_t soloMax = (multi && s.ctl.solo >= 0) ? CH : -1;
   bool inSolo = false, soloStop = false;
   auto soloSync = [&](int jStop) -> bool {
     if (threadIdx.x == 0 && lead) {
@@ -1133,9 +1224,84 @@ __device__ __forceinline__ Seg<NV> waveScanInclS(Seg<NV> x) {
 // clearing the numerators for the next coordinate.  NV = 2 (indicator column:
 // denominator, numerator) or 3 (with the second numerator); SEGS = false when the
 // fit has one stratum (no reset after its first pid).
+//
+// A range of at most blockDim * kScanSmall pids (risk-set pieces of tied event
+// times, or a rank's share of them) is loaded once: thread t holds the pids
+// [t*c, (t+1)*c) of it in registers, folds them, takes its carry from one wave
+// scan of the thread folds, the waves' totals (and the gang carry) and walks
+// them again from registers -- one round of loads instead of two passes of
+// them, and one wave scan for all of them (a wave scan per row of blockDim pids
+// with coalesced loads instead measured 1.5x slower: its shuffles cost more than
+// the strided loads save).
+constexpr int kScanSmall = 8;
+
+template <typename R, int NV, bool SEGS>
+__device__ __forceinline__ bool coxScanSmall(const Slot<R>& s, int pLo, int pHi, Gang& gg,
+                                             double (&v)[2]) {
+  const GrpSt* grp = s.grp;
+  double* numer = s.numer;
+  double* numer2 = s.numer2;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int n = pHi - pLo;
+  const int c = (n + (int)blockDim.x - 1) / (int)blockDim.x;  // <= kScanSmall
+  const int a = pLo + min(n, (int)threadIdx.x * c);
+  const int z = pLo + min(n, ((int)threadIdx.x + 1) * c);
+  Seg<NV> it[kScanSmall];
+  double wn[kScanSmall];
+#pragma unroll
+  for (int u = 0; u < kScanSmall; ++u) {
+    it[u] = segZero<NV>();
+    wn[u] = 0.0;
+    const int i = a + u;
+    if (i < z) {
+      const GrpSt gs = grp[i];
+      it[u].f = SEGS ? s.reset[i] : 0;
+      it[u].v[0] = gs.den;
+      wn[u] = gs.nw;
+      it[u].v[1] = numer[i];
+      if constexpr (NV > 2) it[u].v[2] = numer2[i];
+    }
+  }
+  Seg<NV> l = segZero<NV>();
+#pragma unroll
+  for (int u = 0; u < kScanSmall; ++u) l = scanCombine<NV, SEGS>(l, it[u]);
+  const Seg<NV> inc = waveScanInclS<NV, SEGS>(l);
+  Seg<NV> r = segShflUp(inc, 1);
+  if (lane == 0) r = segZero<NV>();
+  Seg<NV> total;
+  Seg<NV> run = waveCarries<NV>(segFromLane(inc, kWave - 1), total);
+  if (gg.G > 1) {
+    Seg<NV> carry;
+    if (!gangCarry<NV>(gg, total, carry)) return false;
+    run = scanCombine<NV, SEGS>(carry, run);
+  }
+  r = scanCombine<NV, SEGS>(run, r);
+#pragma unroll
+  for (int u = 0; u < kScanSmall; ++u) {
+    const int i = a + u;
+    if (i >= z) break;
+    r = scanCombine<NV, SEGS>(r, it[u]);
+    if (wn[u] != 0.0) {
+      const double accD = r.v[0];
+      const double t = r.v[1] / accD;
+      const double g = wn[u] * t;
+      v[0] += g;
+      if constexpr (NV > 2) v[1] += wn[u] * (r.v[2] / accD - t * t);
+      else v[1] += g * (1.0 - t);
+    }
+    if (it[u].v[1] != 0.0) numer[i] = 0.0;
+    if constexpr (NV > 2) {
+      if (it[u].v[2] != 0.0) numer2[i] = 0.0;
+    }
+  }
+  return true;
+}
+
 template <typename R, int NV, bool SEGS>
 __device__ __forceinline__ bool coxScanT(const Slot<R>& s, int pLo, int pHi, Gang& gg,
                                          double (&v)[2]) {
+  if (pHi - pLo <= (int)blockDim.x * kScanSmall)  // uniform
+    return coxScanSmall<R, NV, SEGS>(s, pLo, pHi, gg, v);
   const GrpSt* grp = s.grp;
   double* numer = s.numer;
   double* numer2 = s.numer2;
@@ -1291,33 +1457,82 @@ __device__ __forceinline__ double coxLogDen(const Slot<R>& s, int pLo, int pHi, 
 // gang), one exchange of (gradient, Hessian), the prior step, the update of its
 // rows and pids.  Rank r owns the rows of its shards and the pids of those rows
 // (shards start at tie groups, so no pid spans two ranks).  Returns as cycleFit.
-template <typename R>
-__device__ __forceinline__ int cycleCox(const Slot<R>& s, FitStatus& cs, int j0, long long deadline,
-                                        Gang& gg) {
-  const DevX& d = s.d;
+//
+// Solo runs (gangs over at most kCoxSoloPids pids, as risk-set pieces of tied
+// event times give): consecutive short columns are run by the lead rank alone --
+// numerators of the whole column, the scan of every pid, the step and the update
+// of every row the column touches -- while the other ranks skip ahead, so such
+// a column costs no exchange and no carry.  Entering and leaving a run, and every
+// kSoloSpan coordinates inside one, the ranks meet (gangSoloSync), which orders
+// every rank's writes before the meeting before every read after it and keeps
+// the waiting ranks' bounded waits short.  The sums are the same sums in the
+// same order as on one workgroup, so a column's step does not depend on G.
+constexpr int kCoxSoloPids = 16384;
+constexpr int kSoloSpan = 2048;
+
+template <typename R, int ITL, class LDS>
+__device__ __forceinline__ int cycleCox(const Slot<R>& s, FitStatus& cs, LDS& S, int j0,
+                                        long long deadline, Gang& gg) {
+  static_assert(ITL >= kIT, "a Cox pass stages blockDim * kIT nonzeros in LDS");
+  __shared__ int sKey[LDS::kItems];
   const int useW = s.ctl.useW;
   const bool multi = gg.G > 1;
   const bool lead = gg.rank == 0;
   const int pLo = multi ? s.pidShard[gg.s0] : 0;
   const int pHi = multi ? s.pidShard[gg.s1] : s.N;
+  Gang one = gg;  // the lead alone: whole columns, every pid, no exchange
+  one.G = 1;
+  const int64_t soloMax =
+      (multi && s.N <= kCoxSoloPids && s.ctl.solo >= 0) ? (int64_t)blockDim.x * kIT : -1;
   long long cG = 0, cU = 0, cP = 0;
   long long tN = 0, tS = 0, tU = 0;  // device ticks: numerators, scan, exchange + update
   int jEnd = s.J;
+  __shared__ double xl[kXchLead];
+  __shared__ ColMeta M;
+  loadColMeta<M_COX, R>(M, s, j0, gg);
+  bool inSolo = false, soloStop = false;
+  auto soloSync = [&](int jStop) -> bool {
+    if (threadIdx.x == 0 && lead) {
+      xl[0] = soloStop ? 1.0 : 0.0;
+      xl[1] = (double)jStop;
+    }
+    return gangSoloSync(gg, xl);
+  };
   for (int j = j0; j < s.J; ++j) {
-    if (!multi && j > j0 && (j & 3) == 0 && quantumOver(deadline)) {
+    if ((!multi || (inSolo && lead)) && j > j0 && (j & 3) == 0 && quantumOver(deadline)) {
+      soloStop = multi;
       jEnd = j;
       break;
     }
-    if (s.fixed[j]) continue;
-    const int64_t cb = d.colPtr[j], ce = d.colPtr[j + 1];
-    int64_t b = cb, e = ce;
+    if (j + 1 > M.w0 + kMetaW) loadColMeta<M_COX, R>(M, s, j, gg);  // uniform
+    const int mj = j - M.w0;
+    if (M.fixed[mj]) continue;
+    const int64_t cb = M.cb[mj], ce = M.cb[mj + 1];
     if (multi) {
-      const int32_t* off = d.colShard + (size_t)j * (d.nShards + 1);
-      b = cb + off[gg.s0];
-      e = cb + off[gg.s1];
+      const bool soloCol = ce - cb <= soloMax;
+      // every rank sees the same columns, so every rank meets at the same j
+      if (soloCol != inSolo || (inSolo && (j % kSoloSpan) == 0)) {
+        if (!soloSync(j)) return -1;
+        const bool stopped = inSolo && xl[0] != 0.0;
+        inSolo = soloCol;
+        if (stopped) {  // the lead's quantum ran out inside the run just left
+          inSolo = false;
+          jEnd = (int)xl[1];
+          break;
+        }
+      }
+      if (inSolo && !lead) continue;
     }
+    const bool part = multi && !inSolo;
+    int64_t b = cb, e = ce;
+    if (part) {
+      b = cb + M.b[mj];
+      e = cb + M.e[mj];
+    }
+    const int qLo = part ? pLo : 0, qHi = part ? pHi : s.N;
+    Gang& gx = part ? gg : one;
     const int64_t njAll = ce - cb;
-    const int fmt = d.fmt[j];
+    const int fmt = M.fmt[mj];
     // in a gang, only the lead rank's reads count (it publishes them)
     double bj = s.beta[j];
     double bnd = s.bound[j];
@@ -1328,15 +1543,15 @@ __device__ __forceinline__ int cycleCox(const Slot<R>& s, FitStatus& cs, int j0,
     const long long t0 = wall_clock64();
     long long t1 = t0, t2 = t0;
     if (njAll > 0) {
-      coxNumerators<R>(s, fmt, cb, b, e, useW);
+      coxNumeratorsLds<R>(s, S, sKey, fmt, cb, b, e, useW);
       __syncthreads();
       t1 = wall_clock64();
-      if (!coxScan<R>(s, pLo, pHi, isIndicator(fmt), gg, v)) return -1;
+      if (!coxScan<R>(s, qLo, qHi, isIndicator(fmt), gx, v)) return -1;
       t2 = wall_clock64();
-      cP += pHi - pLo;
+      cP += qHi - qLo;
     }
     blockSum<2>(v);
-    if (multi) {
+    if (part) {
       double ld[3] = {bj, bnd, (deadline > 0 && wall_clock64() >= deadline) ? 1.0 : 0.0};
       if (!gangExchange<2, 3>(gg, v, ld)) return -1;
       bj = ld[0];
@@ -1355,7 +1570,7 @@ __device__ __forceinline__ int cycleCox(const Slot<R>& s, FitStatus& cs, int j0,
     }
     cG += e - b;
     if (delta != 0.0) {
-      updCox<R>(s, fmt, cb, b, e, useW, delta);
+      updCoxLds<R>(s, S, sKey, fmt, cb, b, e, useW, delta);
       cU += e - b;
     }
     __syncthreads();
@@ -1366,6 +1581,10 @@ __device__ __forceinline__ int cycleCox(const Slot<R>& s, FitStatus& cs, int j0,
       jEnd = j + 1;
       break;
     }
+  }
+  if (multi && inSolo) {  // every rank leaves the last run together
+    if (!soloSync(jEnd)) return -1;
+    if (xl[0] != 0.0) jEnd = (int)xl[1];
   }
   if (threadIdx.x == 0) {
     atomicAdd((unsigned long long*)&cs.gatherNnz, (unsigned long long)cG);
@@ -1661,7 +1880,7 @@ __device__ __forceinline__ void sweepSlot(const Slot<R>& s, FitStatus* st, LDS& 
   int iteration = st[f].iteration;
   double lastObj = st[f].lastObj;
   for (int sw = 0; sw < sweeps; ++sw) {
-    const int jEnd = MODEL == M_COX ? cycleCox<R>(s, st[0], j0, deadline, gg)
+    const int jEnd = MODEL == M_COX ? cycleCox<R, ITL>(s, st[0], S, j0, deadline, gg)
                                     : cycleFit<MODEL, R, ITL>(s, st, S, j0, deadline, gg, act);
     __syncthreads();
     if (jEnd < 0) {
@@ -2046,7 +2265,7 @@ template <typename R, int ONLY>
 static const void* sweepKernelOf(int threads, int itl) {
   const bool t512 = clampThreads(threads) == 512;
   if constexpr (ONLY == M_COX) {
-    return t512 ? (const void*)k_slot_sweep<R, 512, 1, M_COX> : (const void*)k_slot_sweep<R, 256, 1, M_COX>;
+    return t512 ? (const void*)k_slot_sweep<R, 512, 4, M_COX> : (const void*)k_slot_sweep<R, 256, 4, M_COX>;
   } else if constexpr (ONLY >= 0) {
     if (t512 && itl == 4) return (const void*)k_slot_sweep<R, 512, 4, ONLY>;
     if (!t512 && itl == 8) return (const void*)k_slot_sweep<R, 256, 8, ONLY>;
@@ -2668,6 +2887,7 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
                         &st[fit0[b]], s);
     if (rc) return rc;
     slots[b].fit0 = fit0[b];
+    slots[b].ctl.solo = (opt && opt->solo < 0) ? -1 : 0;
     active[b] = b;
   }
   HIP_TRY(hipMemcpyAsync(dslots, slots.data(), nSlot * sizeof(Slot<R>), hipMemcpyHostToDevice, s));

@@ -32,7 +32,7 @@ struct FitCtl {
   int maxIterations;
   int convergenceType;
   int useW;
-  int pad;
+  int solo;  // gangs: short columns on the lead rank alone (0) or never (-1)
 };
 
 // Status of one fit.  A slot's fits have consecutive statuses; the first also

@@ -280,6 +280,8 @@ typedef struct {
   int bundle;            /* fits per slot moving in lock step over interleaved state: 1, 2, 4
                             or 8 (0: 1; Cox: 1).  Consecutive fits with the same fixed
                             coordinates and controls share a slot. */
+  int solo;              /* gangs: runs of short columns are swept by the lead workgroup alone,
+                            without exchanges (0); -1: every column by the whole gang */
 } ccdh_batch_options;
 
 /* Batched engine: n_fits fits over one data handle, device-resident sweeps.

@@ -101,15 +101,17 @@ def test_batch_mixed_precision(name):
     check_fit(bg[0], ig[0], bc, ic, rtol=2e-4, atol=2e-5, ll_rtol=1e-6, same_path=False)
 
 
+@pytest.mark.parametrize("solo", [0, -1])
 @pytest.mark.parametrize("name", ["lr", "poisson", "clr", "cox"])
 @pytest.mark.parametrize("gang", [2, 8, 64])
-def test_batch_gangs_match_oracle(name, gang):
-    """Fits run by gangs of workgroups (row shards, one exchange per coordinate)
-    match the oracle to the FP64 tolerance with the same iterations."""
+def test_batch_gangs_match_oracle(name, gang, solo):
+    """Fits run by gangs of workgroups (row shards, one exchange per coordinate;
+    solo 0: runs of short columns on the lead alone) match the oracle to the FP64
+    tolerance with the same iterations."""
     p, prior, var, maxit = SYNTH[name]()
     specs, held = cv_specs(p, prior, var, maxit, n_fold=2, grid=(1.0,))
     d = api.Data(p)
-    bg, ig = api.fit_batch(d, specs, heldouts=held, max_gang=gang, gang_min_nnz=1)
+    bg, ig = api.fit_batch(d, specs, heldouts=held, max_gang=gang, gang_min_nnz=1, solo=solo)
     for i, sp in enumerate(specs):
         bc, ic = orc.fit(p, sp, heldout=held[i])
         check_fit(bg[i], ig[i], bc, ic)
@@ -126,8 +128,9 @@ def _stratified_cox(n_strata=5):
     return dataclasses.replace(p, strata=strata)
 
 
+@pytest.mark.parametrize("solo", [0, -1])
 @pytest.mark.parametrize("gang", [4, 32])
-def test_batch_cox_gangs_strata_and_weights(gang):
+def test_batch_cox_gangs_strata_and_weights(gang, solo):
     """Cox gangs: the segmented risk-set scan carried across ranks restarts at
     strata inside and at the edges of rank ranges; zero-weight rows (pid N)
     under CV weights; the log-likelihood convergence criterion."""
@@ -140,7 +143,7 @@ def test_batch_cox_gangs_strata_and_weights(gang):
     specs.append(default_spec(p, prior=prior, variance=var, max_iterations=maxit,
                               convergence_type=1))  # log-likelihood criterion: gang LL scan
     d = api.Data(p)
-    bg, ig = api.fit_batch(d, specs, max_gang=gang, gang_min_nnz=1)
+    bg, ig = api.fit_batch(d, specs, max_gang=gang, gang_min_nnz=1, solo=solo)
     st = api.batch_last_stats()
     assert max(t[2] for t in st["timeline"]) == gang
     for i, sp in enumerate(specs):
