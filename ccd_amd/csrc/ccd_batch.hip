@@ -506,13 +506,10 @@ __device__ __forceinline__ void blockSumFits(double (&v)[NV], int F) {
     for (int i = 0; i < NV; ++i) sh[i][wid][lane] = v[i];
   }
   __syncthreads();
-  if ((int)threadIdx.x < F) {
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      double t = 0.0;
-      for (int w = 0; w < nw; ++w) t += sh[i][w][threadIdx.x];
-      tot[i][threadIdx.x] = t;
-    }
+  // thread i * F + f adds value i of fit f over the waves, in wave order
+  if ((int)threadIdx.x < NV * F) {
+    const int i = (int)threadIdx.x / F, ff = (int)threadIdx.x & (F - 1);
+    tot[i][ff] = sumWaves(&sh[i][0][ff], kMaxBundle, nw);
   }
   __syncthreads();
   const int f = threadIdx.x & (F - 1);

@@ -84,6 +84,21 @@ __device__ __forceinline__ double waveSum(double v) {
   return v;
 }
 
+// p[0] + p[stride] + ... + p[(nw - 1) * stride] from LDS, added in that order
+// (nw <= kMaxBlockThreads / kWave); the loads are unrolled so that they are all
+// in flight before the first add.
+__device__ __forceinline__ double sumWaves(const double* p, int stride, int nw) {
+  constexpr int kMaxW = kMaxBlockThreads / kWave;
+  double part[kMaxW];
+#pragma unroll
+  for (int w = 0; w < kMaxW; ++w) part[w] = w < nw ? p[w * stride] : 0.0;
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < kMaxW; ++w)
+    if (w < nw) t += part[w];
+  return t;
+}
+
 // Sum NV values over the block; the totals are returned to every thread.
 template <int NV>
 __device__ __forceinline__ void blockSum(double (&v)[NV]) {
@@ -98,12 +113,9 @@ __device__ __forceinline__ void blockSum(double (&v)[NV]) {
     for (int i = 0; i < NV; ++i) sh[i][wid] = v[i];
   }
   __syncthreads();
+  // the waves' totals in wave order, every load issued before the first add
 #pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    double s = 0.0;
-    for (int w = 0; w < nw; ++w) s += sh[i][w];
-    v[i] = s;
-  }
+  for (int i = 0; i < NV; ++i) v[i] = sumWaves(&sh[i][0], 1, nw);
   __syncthreads();
 }
 
