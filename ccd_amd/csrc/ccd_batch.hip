@@ -757,7 +757,16 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
   const bool multi = gg.G > 1;
   const bool lead = gg.rank == 0;
   __shared__ double xs[kXchSum], xl[kXchLead];
-  long long cG = 0, cGR = 0, cU = 0, cUR = 0, tS = 0, tL = 0, nL = 0;
+  // work counters and phase ticks of the cycle (gathered and updated nonzeros
+  // and runs; device ticks of single-pass and multi-pass columns, multi-pass
+  // visits), kept in LDS by thread 0 rather than in every lane's registers
+  enum { kCG, kCGR, kCU, kCUR, kTS, kTL, kNL, kNCtr };
+  __shared__ long long sCtr[kNCtr];
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kNCtr; ++i) sCtr[i] = 0;
+  auto count = [&](int i, long long v) {
+    if (threadIdx.x == 0) sCtr[i] += v;
+  };
   int jEnd = J;
   // Records are indexed by position in the column, and a rank's positions move
   // from column to column: a rank may already store coordinate k+1's records
@@ -890,19 +899,19 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
           if (delta[c] != 0.0) s.beta[jj] = bj[c] + delta[c];
         }
         moved = moved || delta[c] != 0.0;
-        cG += nMine[c];
-        cGR += MODEL == M_CLR && lead ? runsC[c] : 0;
+        count(kCG, nMine[c]);
+        count(kCGR, MODEL == M_CLR && lead ? runsC[c] : 0);
       }
 #pragma unroll
       for (int c = 0; c < kBlockCols; ++c) {
         if (!live[c] || delta[c] == 0.0 || nMine[c] == 0) continue;
         if (threadIdx.x == 0) S.cValid[0] = 0;
         passUpdate<MODEL, kBlkIT, R>(s, S, (int)nMine[c], stg[c], L[c], delta[c], true);
-        cU += nMine[c];
-        cUR += MODEL == M_CLR && lead ? runsC[c] : 0;
+        count(kCU, nMine[c]);
+        count(kCUR, MODEL == M_CLR && lead ? runsC[c] : 0);
       }
       if (moved) __syncthreads();
-      tS += wall_clock64() - t0;
+      count(kTS, wall_clock64() - t0);
       if (stop) {
         jEnd = j + B;
         break;
@@ -1016,15 +1025,15 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
     }
     const long long t1 = wall_clock64();
     if (nj > CH) {
-      tL += t1 - t0;
-      nL += 1;
+      count(kTL, t1 - t0);
+      count(kNL, 1);
     } else {
-      tS += t1 - t0;
+      count(kTS, t1 - t0);
     }
-    cG += nj * nAct;
-    cGR += MODEL == M_CLR && lead ? runs * nAct : 0;
-    cU += nj * mv;
-    cUR += MODEL == M_CLR && lead ? runs * mv : 0;
+    count(kCG, nj * nAct);
+    count(kCGR, MODEL == M_CLR && lead ? runs * nAct : 0);
+    count(kCU, nj * mv);
+    count(kCUR, MODEL == M_CLR && lead ? runs * mv : 0);
     if (stop) {
       jEnd = j + 1;
       break;
@@ -1036,13 +1045,13 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
   }
   if (threadIdx.x == 0) {
     FitStatus& cs = st[0];
-    atomicAdd((unsigned long long*)&cs.tSmall, (unsigned long long)tS);
-    atomicAdd((unsigned long long*)&cs.tLarge, (unsigned long long)tL);
-    atomicAdd((unsigned long long*)&cs.nLarge, (unsigned long long)nL);
-    atomicAdd((unsigned long long*)&cs.gatherNnz, (unsigned long long)cG);
-    atomicAdd((unsigned long long*)&cs.gatherRuns, (unsigned long long)cGR);
-    atomicAdd((unsigned long long*)&cs.updateNnz, (unsigned long long)cU);
-    atomicAdd((unsigned long long*)&cs.updateRuns, (unsigned long long)cUR);
+    atomicAdd((unsigned long long*)&cs.tSmall, (unsigned long long)sCtr[kTS]);
+    atomicAdd((unsigned long long*)&cs.tLarge, (unsigned long long)sCtr[kTL]);
+    atomicAdd((unsigned long long*)&cs.nLarge, (unsigned long long)sCtr[kNL]);
+    atomicAdd((unsigned long long*)&cs.gatherNnz, (unsigned long long)sCtr[kCG]);
+    atomicAdd((unsigned long long*)&cs.gatherRuns, (unsigned long long)sCtr[kCGR]);
+    atomicAdd((unsigned long long*)&cs.updateNnz, (unsigned long long)sCtr[kCU]);
+    atomicAdd((unsigned long long*)&cs.updateRuns, (unsigned long long)sCtr[kCUR]);
   }
   return jEnd;
 }
