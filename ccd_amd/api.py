@@ -115,13 +115,13 @@ class BatchStats(ctypes.Structure):
         ("launch_gang", ctypes.c_int32 * 128), ("tuned_pass_items", ctypes.c_int32),
         ("tuned_max_gang", ctypes.c_int32), ("tune_ms", ctypes.c_double),
         ("tune_rates", ctypes.c_double * 4), ("bundles", ctypes.c_int32),
-        ("tuned_threads", ctypes.c_int32)]
+        ("tuned_threads", ctypes.c_int32), ("gang_restarts", ctypes.c_int32)]
 
     def as_dict(self):
         out = {n: getattr(self, n) for n, _ in self._fields_[:15]}
         out.update(tuned_pass_items=self.tuned_pass_items, tuned_max_gang=self.tuned_max_gang,
                    tune_ms=self.tune_ms, tune_rates=list(self.tune_rates), bundles=self.bundles,
-                   tuned_threads=self.tuned_threads)
+                   tuned_threads=self.tuned_threads, gang_restarts=self.gang_restarts)
         k = min(128, self.launches)
         out["timeline"] = [(round(self.launch_ms[i], 2), self.launch_fits[i], self.launch_gang[i])
                            for i in range(k)]
@@ -133,13 +133,14 @@ class BatchOptions(ctypes.Structure):
                 ("use_graph", ctypes.c_int), ("max_gang", ctypes.c_int),
                 ("gang_min_nnz", ctypes.c_int64), ("pass_items", ctypes.c_int),
                 ("autotune", ctypes.c_int), ("quantum_ms", ctypes.c_double),
-                ("bundle", ctypes.c_int), ("solo", ctypes.c_int)]
+                ("bundle", ctypes.c_int), ("solo", ctypes.c_int), ("spin_limit", ctypes.c_int64)]
 
 
 class ServerOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("max_slots", ctypes.c_int),
                 ("threads", ctypes.c_int), ("max_gang", ctypes.c_int),
-                ("quantum_ms", ctypes.c_double), ("hbm_budget_bytes", ctypes.c_int64)]
+                ("quantum_ms", ctypes.c_double), ("hbm_budget_bytes", ctypes.c_int64),
+                ("spin_limit", ctypes.c_int64)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -155,7 +156,7 @@ class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "submitted", "completed", "failed", "in_flight", "queued", "launches")] + [
         ("kernel_ms", ctypes.c_double)] + [(n, ctypes.c_int64) for n in (
-            "arena_bytes", "arena_used", "arena_peak")]
+            "arena_bytes", "arena_used", "arena_peak", "gang_restarts")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -409,10 +410,12 @@ def mm_last_stats():
 
 def fit_batch(data: Data, specs, heldouts=None, threads=0, sweeps_per_sync=0, use_graph=1,
               max_gang=0, gang_min_nnz=0, pass_items=0, autotune=0, quantum_ms=0.0, bundle=0,
-              solo=0):
+              solo=0, spin_limit=0):
     """Many fits over one resident data set with device-resident sweeps; a fit
     runs on up to `max_gang` workgroups when fewer fits than CUs remain (solo=-1:
-    a gang sweeps every column together, none on its lead alone)."""
+    a gang sweeps every column together, none on its lead alone; spin_limit: polls a
+    gang rank waits for its peers before it gives up and the fit restarts on one
+    workgroup, 0 for the default)."""
     lib = load_library()
     n = len(specs)
     keep = []
@@ -422,7 +425,7 @@ def fit_batch(data: Data, specs, heldouts=None, threads=0, sweeps_per_sync=0, us
     betas = np.zeros((n, data.problem.J))
     res = (FitResult * n)()
     opt = BatchOptions(threads, sweeps_per_sync, use_graph, max_gang, gang_min_nnz, pass_items,
-                       autotune, quantum_ms, bundle, solo)
+                       autotune, quantum_ms, bundle, solo, spin_limit)
     _check(lib.ccdh_batch_fit(data.handle, n, arr, _p(betas), res, ctypes.byref(opt)))
     return betas, [r.as_dict() for r in res]
 
@@ -722,9 +725,10 @@ class Server:
     request is collected."""
 
     def __init__(self, precision=FP64, max_slots=0, threads=0, quantum_ms=0.0,
-                 hbm_budget_bytes=0, max_gang=0):
+                 hbm_budget_bytes=0, max_gang=0, spin_limit=0):
         self.lib = load_library()
-        opt = ServerOptions(precision, max_slots, threads, max_gang, quantum_ms, hbm_budget_bytes)
+        opt = ServerOptions(precision, max_slots, threads, max_gang, quantum_ms, hbm_budget_bytes,
+                            spin_limit)
         h = ctypes.c_void_p()
         _check(self.lib.ccdh_server_create(ctypes.byref(opt), ctypes.byref(h)))
         self.handle = h

@@ -543,13 +543,15 @@ __global__ void k_xcc_probe(int* out) {
 // by returning atomic swaps and increments the slot's arrival counter only
 // after the swaps have returned (a plain or atomic store may be acknowledged
 // before it is performed, and a waiter could then see the arrival first).
-// Spins are bounded: a rank that waits longer than kSpinLimit polls gives up
-// and the launch reports the failure.
+// Spins are bounded: a rank that waits longer than kSpinLimit polls (or the
+// slot's own limit) gives up and the launch reports the failure; the host then
+// restarts the slot's fits cold on one workgroup.
 struct Gang {
   int rank, G;
   int rowLo, rowHi;  // rows owned by this rank
   int s0, s1;        // shard range
   unsigned seq;      // exchanges done in this launch
+  long long spin;    // polls before a wait gives up
   Xch* x;
   bool ok;
 };
@@ -596,7 +598,7 @@ __device__ __forceinline__ bool gangExchange(Gang& g, double (&sum)[NS], double 
     int ok = 1;
     while (__hip_atomic_load(&g.x->cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       __builtin_amdgcn_s_sleep(1);
-      if (++spins > kSpinLimit) {
+      if (++spins > g.spin) {
         ok = 0;
         break;
       }
@@ -650,7 +652,7 @@ __device__ __forceinline__ bool gangExchangeN(Gang& g, double* xs, int ns, doubl
     int ok = 1;
     while (__hip_atomic_load(&g.x->cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       __builtin_amdgcn_s_sleep(1);
-      if (++spins > kSpinLimit) {
+      if (++spins > g.spin) {
         ok = 0;
         break;
       }
@@ -1092,7 +1094,7 @@ __device__ __forceinline__ bool gangCarry(Gang& g, const Seg<NV>& total, Seg<NV>
     int ok = 1;
     while (__hip_atomic_load(&g.x->cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       __builtin_amdgcn_s_sleep(1);
-      if (++spins > kSpinLimit) {
+      if (++spins > g.spin) {
         ok = 0;
         break;
       }
@@ -2038,6 +2040,7 @@ __global__ void __launch_bounds__(BLOCK)
   gg.rank = ge.rank;
   gg.G = ge.G;
   gg.seq = 0;
+  gg.spin = s.ctl.spinLimit > 0 ? s.ctl.spinLimit : kSpinLimit;
   gg.ok = true;
   gg.x = ge.G > 1 ? xch + ge.slot : nullptr;
   if (ge.G > 1) {
@@ -2894,6 +2897,7 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
     if (rc) return rc;
     slots[b].fit0 = fit0[b];
     slots[b].ctl.solo = (opt && opt->solo < 0) ? -1 : 0;
+    slots[b].ctl.spinLimit = (opt && opt->spin_limit > 0) ? opt->spin_limit : 0;
     active[b] = b;
   }
   HIP_TRY(hipMemcpyAsync(dslots, slots.data(), nSlot * sizeof(Slot<R>), hipMemcpyHostToDevice, s));
@@ -2935,6 +2939,7 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
   std::vector<char> running(nSlot, 1);
   int gangCap = maxGang;
   int curFits = nFit, curGang = 1;
+  std::vector<char> oneGroup(nSlot, 0);  // slots held to one workgroup
   auto buildMap = [&]() -> int {
     std::vector<GangEntry> gangs;
     std::vector<double> prio;
@@ -2948,6 +2953,8 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
     if (!useGraph && gangCap > 1)
       assignGangs(gangs, prio, fw, std::vector<int>(gangs.size(), model), capacity, gangCap,
                   minNnz, true);
+    for (GangEntry& g : gangs)
+      if (oneGroup[g.slot]) g.G = 1;  // restarted after a gang wait gave up
     int G = 1;
     for (const GangEntry& g : gangs) G = std::max(G, g.G);
     if (G > 1 && !layoutGangs(gangs, capacity, xcdCount(), map)) G = 1;
@@ -3120,7 +3127,37 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
     bool misplaced = false;
     for (int b = 0; b < nSlot; ++b) {
       FitStatus& h = o.hst[fit0[b]];
-      if (h.fault == 1) return setError(CCDH_E_DEVICE, "gang exchange timed out");
+      if (h.fault == 1) {
+        // A rank of the slot's gang gave up waiting for its peers (other kernels
+        // holding CUs can keep a rank from being scheduled): the ranks may have
+        // left the fits' state half updated, so every fit of the slot starts over
+        // from its starting coefficients with its current bound, on one
+        // workgroup, where no rank waits for another.
+        oneGroup[b] = 1;
+        g_stats.gang_restarts += 1;
+        h.fault = 0;
+        HIP_TRY(hipMemcpyAsync(&dst[fit0[b]].fault, &h.fault, sizeof(int), hipMemcpyHostToDevice, s));
+        const unsigned all = (1u << fitsOf[b]) - 1u;
+        rc = uploadStart(b, all, [&](int f) { return specs[f].beta0; });
+        if (rc) return rc;
+        active[0] = b;
+        HIP_TRY(hipMemcpyAsync(dactive, active.data(), sizeof(int32_t), hipMemcpyHostToDevice, s));
+        rc = slotLaunchInit(model, mixed, dslots, dst, dactive, 1, threads, s, false, setupRanks,
+                            all);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(o.hst + fit0[b], dst + fit0[b], fitsOf[b] * sizeof(FitStatus),
+                               hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int i = 0; i < fitsOf[b]; ++i) {
+          const int f = fit0[b] + i;
+          pendingRestart[f] = 0;
+          doneAt[f] = 0.0;
+          lastIter[f] = 0;
+        }
+        for (int i = 0; i < nSlot; ++i) active[i] = i;
+        HIP_TRY(hipMemcpyAsync(dactive, active.data(), nSlot * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        continue;
+      }
       if (h.fault == 2) {  // a gang spanned XCDs: it backed off untouched
         misplaced = true;
         h.fault = 0;

@@ -60,6 +60,7 @@ struct Request {
   FitStatus st0{};
   int slot = -1;
   int boundCount = 1;
+  bool oneGroup = false;  // restarted on one workgroup after its gang's wait gave up
   int64_t launches = 0;
   Clock::time_point tSubmit, tAdmit, tReady, tDone;
   bool done = false;
@@ -221,6 +222,7 @@ class Server final : public ccdh_server {
     out->arena_bytes = (int64_t)arena_.capacity();
     out->arena_used = (int64_t)arena_.used();
     out->arena_peak = (int64_t)arena_.peak();
+    out->gang_restarts = gangRestarts_;
   }
 
   void shutdown() override {
@@ -335,6 +337,7 @@ class Server final : public ccdh_server {
     Slot<R> s;
     rc = slotPrepare<R>(r->data, &r->spec, 1, slotBlock, r->layout, &s, &r->st0, admitStream_);
     if (rc) return rc;
+    s.ctl.spinLimit = opt_.spin_limit > 0 ? opt_.spin_limit : 0;
     // set-up kernel for this fit alone, on the admission stream
     char* tail = slotBlock + r->layout.bytes;
     auto* dS = (Slot<R>*)tail;
@@ -417,6 +420,8 @@ class Server final : public ccdh_server {
       models.push_back(r->model);
     }
     if (maxGang_ > 1) assignGangs(want, prio, nnz, models, capacity_, maxGang_, kGangMinNnz);
+    for (GangEntry& g : want)
+      if (slotReq_[g.slot]->oneGroup) g.G = 1;
     bool gangs = false;
     for (const GangEntry& g : want) gangs |= g.G > 1;
     if (gangs && !layoutGangs(want, capacity_, xcdCount(), map_)) gangs = false;
@@ -439,7 +444,31 @@ class Server final : public ccdh_server {
     const hipError_t e = hipStreamSynchronize(schedStream_);
     if (e != hipSuccess) return setError(CCDH_E_DEVICE, std::string("sweep launch: ") + hipGetErrorString(e));
     for (int32_t slot : active) {
-      if (hSt_[slot].fault == 1) return setError(CCDH_E_DEVICE, "gang exchange timed out");
+      if (hSt_[slot].fault == 1) {
+        // A rank of the request's gang gave up waiting for its peers (other work
+        // holding CUs can keep a rank from being scheduled): the fit starts over
+        // from its starting coefficients with its current bound, on one
+        // workgroup, where no rank waits for another; the other requests go on.
+        Request* r = slotReq_[slot];
+        r->oneGroup = true;
+        hSt_[slot].fault = 0;
+        if (hipMemcpyAsync(&dSt_[slot].fault, &hSt_[slot].fault, sizeof(int), hipMemcpyHostToDevice,
+                           schedStream_) != hipSuccess ||
+            hipMemcpyAsync(hSlots_[slot].beta, r->spec.beta0, r->J * sizeof(double),
+                           hipMemcpyHostToDevice, schedStream_) != hipSuccess ||
+            hipMemcpyAsync(dOne_, &slot, sizeof(int32_t), hipMemcpyHostToDevice, schedStream_) != hipSuccess)
+          return setError(CCDH_E_DEVICE, "gang restart upload");
+        int rc2 = slotLaunchInit(M_ANY, kMixed, dSlots_, dSt_, dOne_, 1, threads_, schedStream_, false,
+                                 initRanks(r->data->nnz, r->K));
+        if (rc2) return rc2;
+        if (hipMemcpyAsync(&hSt_[slot], dSt_ + slot, sizeof(FitStatus), hipMemcpyDeviceToHost,
+                           schedStream_) != hipSuccess ||
+            hipStreamSynchronize(schedStream_) != hipSuccess)
+          return setError(CCDH_E_DEVICE, "gang restart");
+        std::lock_guard<std::mutex> g(mu_);
+        ++gangRestarts_;
+        continue;
+      }
       if (hSt_[slot].fault == 2) {  // a gang spanned XCDs and backed off untouched
         maxGang_ = 1;
         hSt_[slot].fault = 0;
@@ -592,6 +621,7 @@ class Server final : public ccdh_server {
 
   static constexpr int64_t kGangMinNnz = 32768;
   ccdh_server_options opt_{};
+  int64_t gangRestarts_ = 0;
   int maxSlots_ = 0, threads_ = 0, itl_ = 8, maxGang_ = 0, capacity_ = 0;
   std::vector<GangEntry> map_;
   GangEntry* dMap_ = nullptr;

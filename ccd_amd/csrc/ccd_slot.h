@@ -33,6 +33,7 @@ struct FitCtl {
   int convergenceType;
   int useW;
   int solo;  // gangs: short columns on the lead rank alone (0) or never (-1)
+  long long spinLimit;  // polls a gang rank waits for its peers before giving up (0: default)
 };
 
 // Status of one fit.  A slot's fits have consecutive statuses; the first also

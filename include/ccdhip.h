@@ -282,6 +282,9 @@ typedef struct {
                             coordinates and controls share a slot. */
   int solo;              /* gangs: runs of short columns are swept by the lead workgroup alone,
                             without exchanges (0); -1: every column by the whole gang */
+  int64_t spin_limit;    /* polls a gang rank waits for its peers before it gives up (0: 2^24).
+                            A fit whose gang gives up restarts cold on one workgroup
+                            (ccdh_batch_stats.gang_restarts); small values force that path. */
 } ccdh_batch_options;
 
 /* Batched engine: n_fits fits over one data handle, device-resident sweeps.
@@ -319,6 +322,7 @@ typedef struct {
   double tune_rates[4];
   int32_t bundles;         /* slots the fits were bundled into */
   int32_t tuned_threads;   /* workgroup size of the sweep launches */
+  int32_t gang_restarts;   /* slots restarted cold on one workgroup after a gang wait gave up */
 } ccdh_batch_stats;
 int ccdh_batch_last_stats(ccdh_batch_stats* out);
 
@@ -397,6 +401,8 @@ typedef struct {
   int max_gang;             /* workgroups one fit may use when the device has room (0: 64; 1: one) */
   double quantum_ms;        /* device time per fit per launch (0: 20 ms) */
   int64_t hbm_budget_bytes; /* arena size (0: half of the free device memory) */
+  int64_t spin_limit;       /* polls a gang rank waits for its peers before it gives up (0: 2^24);
+                               a request whose gang gives up restarts cold on one workgroup */
 } ccdh_server_options;
 
 typedef struct {
@@ -413,6 +419,7 @@ typedef struct {
   int64_t launches;         /* sweep launches so far */
   double kernel_ms;         /* summed HIP-event time of those launches */
   int64_t arena_bytes, arena_used, arena_peak;
+  int64_t gang_restarts;    /* requests restarted cold on one workgroup after a gang wait gave up */
 } ccdh_server_stats;
 
 int ccdh_server_create(const ccdh_server_options* options, ccdh_server_t* out);

@@ -96,3 +96,47 @@ def test_served_failure_path():
         got = [srv.wait(i) for i in ids]
     for (name, _, _, _), (bc, ic), (bg, ig, _) in zip(CASES, refs, got):
         check_flagged(bg, ig, bc, ic, name)
+
+
+# ---- a gang wait that gives up (forced from the host: a one-poll spin limit on
+# the ranks' own bounded waits, so every kernel still exits normally)
+def _gang_problem(name):
+    from tests.test_gpu_plain import SYNTH
+    p, prior, var, maxit = SYNTH[name]()
+    return p, default_spec(p, prior=prior, variance=var, max_iterations=maxit)
+
+
+@pytest.mark.parametrize("name", ["clr", "cox"])
+def test_batch_gang_timeout_restarts_on_one_workgroup(name):
+    """Fits whose gang ranks give up waiting restart from their starting
+    coefficients on one workgroup: the same bits as a batch that ran them on one
+    workgroup from the start, the oracle's fit, and a restart count."""
+    p, spec = _gang_problem(name)
+    specs = [spec, default_spec(p, prior=PRIOR_LAPLACE, variance=0.5,
+                                max_iterations=spec.max_iterations)]
+    d = api.Data(p)
+    print("batch gang timeout", name, flush=True)
+    bg, ig = api.fit_batch(d, specs, max_gang=8, gang_min_nnz=1, spin_limit=1)
+    st = api.batch_last_stats()
+    b1, i1 = api.fit_batch(d, specs, max_gang=1)
+    d.close()
+    assert st["gang_restarts"] >= 1
+    for i, sp in enumerate(specs):
+        assert np.array_equal(bg[i], b1[i]) and ig[i]["iterations"] == i1[i]["iterations"]
+        bc, ic = orc.fit(p, sp)
+        check_flagged(bg[i], ig[i], bc, ic, f"{name} fit {i}")
+
+
+def test_served_gang_timeout_restarts_on_one_workgroup():
+    """A served request whose gang gives up restarts on one workgroup; the other
+    requests go on, and every result matches the oracle."""
+    reqs = [_gang_problem(n) for n in ("clr", "lr", "poisson")]  # clr, poisson: gangs of 2-4
+    refs = [orc.fit(p, s) for p, s in reqs]
+    print("served gang timeout", flush=True)
+    with api.Server(quantum_ms=1.0, max_gang=8, spin_limit=1) as srv:
+        ids = [srv.submit(p, s) for p, s in reqs]
+        got = [srv.wait(i) for i in ids]
+        stats = srv.stats()
+    assert stats["gang_restarts"] >= 1 and stats["failed"] == 0
+    for (bc, ic), (bg, ig, _) in zip(refs, got):
+        check_flagged(bg, ig, bc, ic)
