@@ -51,7 +51,7 @@ struct DevX {
 };
 
 constexpr int kShards = 64;      // row shards per data set (largest gang)
-constexpr int kBlockCols = 4;    // columns of one block at most
+constexpr int kBlockCols = 3;    // columns of one block at most
 constexpr int kBlockNnz = 512;   // nonzeros of a block column at most
 
 template <typename R>

@@ -181,7 +181,7 @@ __global__ __launch_bounds__(kLenWaves * 64) void k_block_len(int J, const int64
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = blockIdx.x * kLenWaves + wave;
   if (j >= J) return;  // whole waves leave; no workgroup barrier below
-  int cnt[kBlockCols - 1] = {0, 0, 0};
+  int cnt[kBlockCols - 1] = {};
   int B = 0;
   for (int c = j; c < J && B < kBlockCols; ++c) {
     const int64_t b = colPtr[c], n = colPtr[c + 1] - b;
