@@ -663,18 +663,23 @@ __device__ __forceinline__ bool gangExchangeN(Gang& g, double* xs, int ns, doubl
   __syncthreads();
   const bool ok = sOk[par] != 0;
   if (threadIdx.x < kWave) {
-    // all loads in flight before the first sum
+    // a chunk's loads all in flight before its first sum; chunks of 8 values
+    // (one chunk unless a bundle exchanges more), so that the exchange holds
+    // few registers beside the lane state live across it
+    constexpr int kChunk = 8;
     const bool mine = ok && (int)threadIdx.x < g.G;
-    double t[kXchSum];
-#pragma unroll
-    for (int i = 0; i < kXchSum; ++i)
-      t[i] = (mine && i < ns) ? loadShared(&g.x->v[par][threadIdx.x][i]) : 0.0;
     const double l = (ok && (int)threadIdx.x < nl) ? loadShared(&g.x->v[par][0][kXchSum + threadIdx.x]) : 0.0;
+    for (int c0 = 0; c0 < ns; c0 += kChunk) {
+      double t[kChunk];
 #pragma unroll
-    for (int i = 0; i < kXchSum; ++i) {
-      if (i >= ns) break;
-      const double r = waveSum(t[i]);
-      if (threadIdx.x == 0) xs[i] = r;
+      for (int i = 0; i < kChunk; ++i)
+        t[i] = (mine && c0 + i < ns) ? loadShared(&g.x->v[par][threadIdx.x][c0 + i]) : 0.0;
+#pragma unroll
+      for (int i = 0; i < kChunk; ++i) {
+        if (c0 + i >= ns) break;
+        const double r = waveSum(t[i]);
+        if (threadIdx.x == 0) xs[c0 + i] = r;
+      }
     }
     if ((int)threadIdx.x < nl) xl[threadIdx.x] = l;
   }
@@ -817,11 +822,15 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
       if (inSolo && !lead) continue;
     }
     const bool part = multi && !inSolo;  // this rank's part of the column, one exchange
-    // A block of short columns with pairwise disjoint rows (strata): each column
-    // is gathered on its own, then one reduction and one exchange serve the
-    // whole block, then each column takes its step and its update.  No column's
+    // A block of short columns with pairwise disjoint rows (strata), swept by
+    // one workgroup (a fit alone, or a gang's lead in a lead-alone run): each
+    // column is gathered on its own, then one block reduction serves the whole
+    // block, then each column takes its step and its update.  No column's
     // update changes another's gradient, so this is the sequential result.
-    const int B = (F == 1 && MODEL != M_COX && d.blk != nullptr) ? M.blk[mj] : 1;
+    // Block columns are short, so in a gang they run on the lead alone; with
+    // lead-alone runs off, a gang sweeps them one at a time with an exchange
+    // each (keeping the exchange out of this path keeps its registers low).
+    const int B = (F == 1 && MODEL != M_COX && d.blk != nullptr && !part) ? M.blk[mj] : 1;
     if (B > 1) {
       // nonzeros per lane of a block column: one pass of the workgroup covers it
       constexpr int kLanes = LDS::kItems / ITL;
@@ -832,7 +841,6 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
       double v[2 * kBlockCols], bj[kBlockCols], bnd[kBlockCols], delta[kBlockCols];
       int nAll[kBlockCols], nMine[kBlockCols], runsC[kBlockCols];  // <= kBlockNnz each
       bool live[kBlockCols], stg[kBlockCols];
-      bool stop = false;
 #pragma unroll
       for (int c = 0; c < kBlockCols; ++c) {
         const int jj = j + c;
@@ -844,11 +852,7 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
         live[c] = c < B && !M.fixed[mj + c];  // uniform
         if (!live[c]) continue;
         const int64_t cb = M.cb[mj + c], ce = M.cb[mj + c + 1];
-        int64_t b = cb, e = ce;
-        if (part) {
-          b = cb + M.b[mj + c];
-          e = cb + M.e[mj + c];
-        }
+        const int64_t b = cb, e = ce;
         nAll[c] = (int)(ce - cb);
         nMine[c] = (int)(e - b);
         const int fmtc = M.fmt[mj + c];
@@ -864,27 +868,6 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
         v[2 * c + 1] = vv[1];
       }
       blockSumFits<2 * kBlockCols>(v, 1);
-      if (part) {
-        if (threadIdx.x == 0) {
-          for (int c = 0; c < B; ++c) {
-            xs[2 * c] = v[2 * c];
-            xs[2 * c + 1] = v[2 * c + 1];
-            xl[2 * c] = bj[c];
-            xl[2 * c + 1] = bnd[c];
-          }
-          xl[2 * B] = (deadline > 0 && wall_clock64() >= deadline) ? 1.0 : 0.0;
-        }
-        if (!gangExchangeN(gg, xs, 2 * B, xl, 2 * B + 1)) return -1;
-#pragma unroll
-        for (int c = 0; c < kBlockCols; ++c) {
-          if (c >= B) continue;
-          v[2 * c] = xs[2 * c];
-          v[2 * c + 1] = xs[2 * c + 1];
-          bj[c] = xl[2 * c];
-          bnd[c] = xl[2 * c + 1];
-        }
-        stop = xl[2 * B] != 0.0;
-      }
       bool moved = false;
 #pragma unroll
       for (int c = 0; c < kBlockCols; ++c) {
@@ -914,11 +897,7 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
       }
       if (moved) __syncthreads();
       count(kTS, wall_clock64() - t0);
-      if (stop) {
-        jEnd = j + B;
-        break;
-      }
-      j += B - 1;
+      j += B - 1;  // the quantum ends at the check on top of the loop
       continue;
     }
     Records<R> rec = s.rec;
