@@ -9,7 +9,7 @@ reorders the per-coordinate sums); graph replay and direct launches agree.
 import numpy as np
 import pytest
 
-from ccd_amd import api
+from ccd_amd import api, synth
 from ccd_amd.data import CONDITIONAL_LOGISTIC, PRIOR_LAPLACE, PRIOR_NORMAL, default_spec
 from oracle import oracle as orc
 from tests import fixtures
@@ -401,3 +401,23 @@ def test_device_cox_groups_match_oracle():
             assert np.array_equal(first, exp_first)
             assert shard[0] == 0 and shard[-1] == N and np.all(np.diff(shard) >= 0)
         d.close()
+
+
+@pytest.mark.parametrize("gang", [4, 16])
+def test_batch_cox_long_lead_alone_runs(gang):
+    """Cox gangs over more than 2,048 consecutive short columns: the lead sweeps
+    them alone and the ranks meet every 2,048 coordinates inside the run; tied
+    days make few risk-set pieces.  Matches the oracle, and a gang that sweeps
+    every column together (solo=-1)."""
+    p, _ = synth.cox_zipf(K=8000, P=5000, per_row=6, n_true=40, seed=31)
+    spec = default_spec(p, prior=PRIOR_LAPLACE, variance=0.05, max_iterations=1000)
+    d = api.Data(p)
+    print("cox lead-alone runs, gang", gang, flush=True)
+    bg, ig = api.fit_batch(d, [spec], max_gang=gang, gang_min_nnz=1)
+    st = api.batch_last_stats()
+    assert max(t[2] for t in st["timeline"]) == gang
+    bt, it = api.fit_batch(d, [spec], max_gang=gang, gang_min_nnz=1, solo=-1)
+    d.close()
+    bc, ic = orc.fit(p, spec)
+    check_fit(bg[0], ig[0], bc, ic)
+    check_fit(bt[0], it[0], bc, ic)
