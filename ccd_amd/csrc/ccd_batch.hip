@@ -229,12 +229,30 @@ __device__ __forceinline__ void updCoxLds(const Slot<R>& s, LDS& S, int* key, in
   }
 }
 
-// True when the launch's time quantum has run out (block-uniform).
-__device__ __forceinline__ bool quantumOver(long long deadline) {
+// Device ticks per nonzero of this workgroup's last column of more than one
+// pass in this launch (0 before the first): thread 0 keeps it, and it prices the
+// next such column when the quantum is checked.
+__device__ __forceinline__ float& longTicksPerNnz() {
+  __shared__ float sTpn;
+  return sTpn;
+}
+
+// Ticks that a column of `nnz` nonzeros (this rank's part) is expected to take,
+// 0 for one that fits a single pass (thread 0).
+__device__ __forceinline__ long long aheadTicks(int64_t nnz, int64_t onePass) {
+  return nnz > onePass ? (long long)(longTicksPerNnz() * (float)nnz) : 0;
+}
+
+// True when the launch's time quantum has run out, or would run out within
+// `nnzAhead` nonzeros of a column of more than one pass (block-uniform).  A
+// launch lasts until its last slot stops, so a workgroup that stops a column
+// early idles alone while one that overruns holds every other workgroup.
+__device__ __forceinline__ bool quantumOver(long long deadline, int64_t nnzAhead = 0,
+                                            int64_t onePass = 0) {
   __shared__ int sOver;
   if (deadline <= 0) return false;
   __syncthreads();
-  if (threadIdx.x == 0) sOver = wall_clock64() >= deadline;
+  if (threadIdx.x == 0) sOver = wall_clock64() + aheadTicks(nnzAhead, onePass) >= deadline;
   __syncthreads();
   return sOver != 0;
 }
@@ -742,6 +760,31 @@ __device__ __forceinline__ void loadColMeta(ColMeta& M, const Slot<R>& s, int w0
   __syncthreads();
 }
 
+// The first coordinate from k that ends a lead-alone run -- a free column of
+// more than `soloMax` nonzeros -- or J: the ranks other than the lead skip the
+// run's columns with one search of each metadata window instead of a visit per
+// column (block-uniform; leaves the window that holds the result in M).
+template <int MODEL, typename R>
+__device__ __forceinline__ int soloRunEnd(ColMeta& M, const Slot<R>& s, int k, int64_t soloMax,
+                                          const Gang& gg) {
+  __shared__ int sEnd;
+  const int J = s.J;
+  while (k < J) {
+    if (k + kBlockCols > M.w0 + kMetaW) loadColMeta<MODEL, R>(M, s, k, gg);
+    if (threadIdx.x == 0) sEnd = J;
+    __syncthreads();
+    const int lim = min(kMetaW, J - M.w0);
+    for (int t = k - M.w0 + (int)threadIdx.x; t < lim; t += blockDim.x)
+      if (!M.fixed[t] && M.cb[t + 1] - M.cb[t] > soloMax) atomicMin(&sEnd, M.w0 + t);
+    __syncthreads();
+    const int e = sEnd;
+    __syncthreads();  // every thread has read sEnd before it is set again
+    if (e < J) return e;
+    k = M.w0 + lim;
+  }
+  return J;
+}
+
 // One coordinate cycle of a slot's fits (independent rows or grouped) over this
 // rank's rows, from coordinate j0; `act` has bit f set for each fit still
 // running (the others' lanes load and store nothing).  Every lane takes its
@@ -799,13 +842,18 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
     return gangSoloSync(gg, xl);
   };
   for (int j = j0; j < J; ++j) {
-    if ((!multi || (inSolo && lead)) && j > j0 && (j & 3) == 0 && quantumOver(deadline)) {
+    if (j + kBlockCols > M.w0 + kMetaW) loadColMeta<MODEL, R>(M, s, j, gg);  // uniform
+    const int mj = j - M.w0;
+    // The quantum is checked every fourth coordinate, and before every column
+    // of more than one pass (milliseconds for a workgroup alone), with that
+    // column's expected time: the launch lasts until its last slot stops.
+    const int64_t nCol = M.fixed[mj] ? 0 : M.cb[mj + 1] - M.cb[mj];
+    if ((!multi || (inSolo && lead)) && j > j0 && ((j & 3) == 0 || nCol > CH) &&
+        quantumOver(deadline, nCol, CH)) {
       soloStop = multi;
       jEnd = j;
       break;
     }
-    if (j + kBlockCols > M.w0 + kMetaW) loadColMeta<MODEL, R>(M, s, j, gg);  // uniform
-    const int mj = j - M.w0;
     if (M.fixed[mj]) continue;
     if (multi) {
       const bool soloCol = M.cb[mj + 1] - M.cb[mj] <= soloMax;
@@ -819,7 +867,10 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
           break;
         }
       }
-      if (inSolo && !lead) continue;
+      if (inSolo && !lead) {  // on to the column that ends the run
+        j = soloRunEnd<MODEL, R>(M, s, j + 1, soloMax, gg) - 1;
+        continue;
+      }
     }
     const bool part = multi && !inSolo;  // this rank's part of the column, one exchange
     // A block of short columns with pairwise disjoint rows (strata), swept by
@@ -942,7 +993,12 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
         xl[2 * threadIdx.x] = bj;
         xl[2 * threadIdx.x + 1] = bnd;
       }
-      if (threadIdx.x == 0) xl[2 * F] = (deadline > 0 && wall_clock64() >= deadline) ? 1.0 : 0.0;
+      if (threadIdx.x == 0) {  // this rank's part of the next column, when the window holds it
+        const int64_t nNext = (mj + 1 < kMetaW && j + 1 < J && !M.fixed[mj + 1])
+                                  ? (int64_t)M.e[mj + 1] - M.b[mj + 1]
+                                  : 0;
+        xl[2 * F] = (deadline > 0 && wall_clock64() + aheadTicks(nNext, CH) >= deadline) ? 1.0 : 0.0;
+      }
       if (!gangExchangeN(gg, xs, 2 * F, xl, 2 * F + 1)) return false;
       v[0] = xs[2 * f];
       v[1] = xs[2 * f + 1];
@@ -1008,6 +1064,10 @@ __device__ __forceinline__ int cycleFit(const Slot<R>& s, FitStatus* st, LDS& S,
     if (nj > CH) {
       count(kTL, t1 - t0);
       count(kNL, 1);
+      if (threadIdx.x == 0) {  // the last column's rate, decayed slowly towards a faster one
+        float& tpn = longTicksPerNnz();
+        tpn = fmaxf((float)(t1 - t0) / (float)nj, 0.75f * tpn);
+      }
     } else {
       count(kTS, t1 - t0);
     }
@@ -1486,15 +1546,17 @@ __device__ __forceinline__ int cycleCox(const Slot<R>& s, FitStatus& cs, LDS& S,
     return gangSoloSync(gg, xl);
   };
   for (int j = j0; j < s.J; ++j) {
-    if ((!multi || (inSolo && lead)) && j > j0 && (j & 3) == 0 && quantumOver(deadline)) {
+    if (j + 1 > M.w0 + kMetaW) loadColMeta<M_COX, R>(M, s, j, gg);  // uniform
+    const int mj = j - M.w0;
+    const int64_t cb = M.cb[mj], ce = M.cb[mj + 1];
+    // every fourth coordinate, and before every column of more than one pass
+    if ((!multi || (inSolo && lead)) && j > j0 &&
+        ((j & 3) == 0 || ce - cb > (int64_t)blockDim.x * kIT) && quantumOver(deadline)) {
       soloStop = multi;
       jEnd = j;
       break;
     }
-    if (j + 1 > M.w0 + kMetaW) loadColMeta<M_COX, R>(M, s, j, gg);  // uniform
-    const int mj = j - M.w0;
     if (M.fixed[mj]) continue;
-    const int64_t cb = M.cb[mj], ce = M.cb[mj + 1];
     if (multi) {
       const bool soloCol = ce - cb <= soloMax;
       // every rank sees the same columns, so every rank meets at the same j
@@ -1700,7 +1762,13 @@ __device__ __forceinline__ bool gangObjective(const Slot<R>& s, Gang& gg, long l
     xs[threadIdx.x] = v;
     xl[threadIdx.x] = lp;
   }
-  if (threadIdx.x == 0) xl[F] = (deadline > 0 && wall_clock64() >= deadline) ? 1.0 : 0.0;
+  if (threadIdx.x == 0) {  // with this rank's part of the next cycle's first column
+    int64_t n0 = 0;
+    if (MODEL != M_COX && s.J > 0 && !s.fixed[0])
+      n0 = (int64_t)s.d.colShard[gg.s1] - s.d.colShard[gg.s0];
+    const int64_t onePass = ((int64_t)blockDim.x * kIT) >> s.lF;
+    xl[F] = (deadline > 0 && wall_clock64() + aheadTicks(n0, onePass) >= deadline) ? 1.0 : 0.0;
+  }
   if (!gangExchangeN(gg, xs, F, xl, F + 1)) return false;
   *stop = xl[F] != 0.0;
   const double fl = s.fixedLL[f];
@@ -1931,7 +1999,10 @@ __device__ __forceinline__ void sweepSlot(const Slot<R>& s, FitStatus* st, LDS& 
     act = sAct;
     __syncthreads();
     if (act == 0) return;
-    if (multi ? stop : quantumOver(deadline)) return;
+    // the next cycle opens with column 0
+    const int64_t n0 =
+        (MODEL != M_COX && s.J > 0 && !s.fixed[0]) ? s.d.colPtr[1] - s.d.colPtr[0] : 0;
+    if (multi ? stop : quantumOver(deadline, n0, ((int64_t)blockDim.x * kIT) >> s.lF)) return;
   }
 }
 
@@ -2012,7 +2083,10 @@ __global__ void __launch_bounds__(BLOCK)
   bool running = false;
   for (int i = 0; i < s.F; ++i) running |= fs[i].done == 0;
   if (!running) return;
-  if (threadIdx.x == 0) sStart = wall_clock64();
+  if (threadIdx.x == 0) {
+    sStart = wall_clock64();
+    longTicksPerNnz() = 0.f;
+  }
   __syncthreads();
   const long long deadline = quantum > 0 ? sStart + quantum : 0;
   Gang gg;
@@ -2697,7 +2771,7 @@ struct TuneChoice {
 };
 std::map<TuneKey, TuneChoice> g_tuned;
 
-constexpr long long kTrialTicks = 5000000;  // 50 ms of device time per candidate
+constexpr long long kTrialTicks = 2500000;  // 25 ms of device time per candidate
 
 // One quantum-long launch from the starting point; rate = fit-coordinates per ms.
 int timeTrial(bool mixed, const void* dslots, FitStatus* dst, const GangEntry* dmap, int mapN,
