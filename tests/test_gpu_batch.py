@@ -253,6 +253,47 @@ def test_batch_gangs_long_columns_many_fits():
         assert [r["iterations"] for r in i2] == [r["iterations"] for r in i1]
 
 
+_ZIPF = {}
+
+
+def _zipf_case(name):
+    """Zipf designs (about 10^6 nonzeros) whose head columns take several passes
+    of a workgroup and whose tail is short columns (blocks, lead-alone runs), CV
+    specs over two folds and two variances, and the oracle's fits (computed once).
+    The Poisson fits stop at 8 sweeps (coordinate descent converges slowly on
+    this design): both engines return MAX_ITERATIONS there."""
+    if name not in _ZIPF:
+        if name == "clr":
+            p, _ = synth.clr_strata(n_strata=6000, P=600, per_row=25, s=1.1, max_controls=10,
+                                    n_true=40, sd=0.5, seed=12)
+            maxit = 200
+        else:
+            p, _ = synth.poisson_zipf(K=40_000, P=600, per_row=25, s=1.1, n_true=40, sd=0.3,
+                                      seed=12)
+            maxit = 8
+        specs, _ = cv_specs(p, PRIOR_LAPLACE, 1.0, maxit, n_fold=2, grid=(0.02, 1.0))
+        _ZIPF[name] = (p, specs, [orc.fit(p, sp) for sp in specs])
+    return _ZIPF[name]
+
+
+@pytest.mark.parametrize("gangs", [False, True])
+@pytest.mark.parametrize("name", ["clr", "poisson"])
+def test_batch_short_quantum_matches_oracle(name, gangs):
+    """A 0.3 ms launch quantum: the fits pause hundreds of times, at the checks
+    before multi-pass columns (priced by the workgroup's time per nonzero), at the
+    cycle's end, inside lead-alone runs and at a gang's exchanges, and resume in
+    the next launch; every fit still matches the oracle with the same iterations.
+    Without gangs every slot is one workgroup (gangs allowed, none grown)."""
+    p, specs, ref = _zipf_case(name)
+    d = api.Data(p)
+    b, info = api.fit_batch(d, specs, max_gang=8, gang_min_nnz=1 if gangs else 10**12,
+                            quantum_ms=0.3)
+    st = api.batch_last_stats()
+    assert st["launches"] > 4 * max(r["iterations"] for r in info)
+    for i, (bc, ic) in enumerate(ref):
+        check_fit(b[i], info[i], bc, ic)
+
+
 def test_batch_pass_width_changes_only_rounding():
     """Long-column passes of 4 or 8 nonzeros per lane give the same fits up to
     the order of the sums."""
