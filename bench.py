@@ -527,7 +527,9 @@ def main():
                     help="largest Cox request rows (pool mode)")
     ap.add_argument("--serve-plain-max-nnz", type=int, default=5_000_000,
                     help="largest request (nonzeros) in the plain sequential sample")
-    ap.add_argument("--serve-quantum-ms", type=float, default=20.0)
+    ap.add_argument("--serve-quantum-ms", type=float, default=100.0,
+                    help="server launch quantum; results return at launch ends, so it is "
+                         "also the floor of a request's latency (library default 20)")
     ap.add_argument("--serve-threads", type=int, default=0,
                     help="server workgroup size: 256 (0) or 512 (x 4 items per lane)")
     ap.add_argument("--serve-plain-seconds", type=float, default=20.0,
