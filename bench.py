@@ -591,12 +591,14 @@ def main():
     log(f"resident: {data.device_bytes / 1e9:.2f} GB")
 
     folds, grid = args.folds, args.grid
+    # one weights array per fold, shared by that fold's fits across the grid (the
+    # batch uploads it once and computes its X'Y once)
+    fold_w = [api.cv_fold_weights(prob.strata, folds, 123, f, by_pid=True) for f in range(folds)]
     specs = []
     for g in range(grid):
         var = api.cv_grid_point(1e-4, 1.0, grid, g)
         for f in range(folds):
-            w = api.cv_fold_weights(prob.strata, folds, 123, f, by_pid=True)
-            specs.append(default_spec(prob, prior=PRIOR_LAPLACE, variance=var, weights=w,
+            specs.append(default_spec(prob, prior=PRIOR_LAPLACE, variance=var, weights=fold_w[f],
                                       max_iterations=args.max_iter))
     if args.limit:
         specs = specs[-args.limit:]

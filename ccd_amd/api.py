@@ -115,13 +115,15 @@ class BatchStats(ctypes.Structure):
         ("launch_gang", ctypes.c_int32 * 128), ("tuned_pass_items", ctypes.c_int32),
         ("tuned_max_gang", ctypes.c_int32), ("tune_ms", ctypes.c_double),
         ("tune_rates", ctypes.c_double * 4), ("bundles", ctypes.c_int32),
-        ("tuned_threads", ctypes.c_int32), ("gang_restarts", ctypes.c_int32)]
+        ("tuned_threads", ctypes.c_int32), ("gang_restarts", ctypes.c_int32),
+        ("shared_setups", ctypes.c_int32)]
 
     def as_dict(self):
         out = {n: getattr(self, n) for n, _ in self._fields_[:15]}
         out.update(tuned_pass_items=self.tuned_pass_items, tuned_max_gang=self.tuned_max_gang,
                    tune_ms=self.tune_ms, tune_rates=list(self.tune_rates), bundles=self.bundles,
-                   tuned_threads=self.tuned_threads, gang_restarts=self.gang_restarts)
+                   tuned_threads=self.tuned_threads, gang_restarts=self.gang_restarts,
+                   shared_setups=self.shared_setups)
         k = min(128, self.launches)
         out["timeline"] = [(round(self.launch_ms[i], 2), self.launch_fits[i], self.launch_gang[i])
                            for i in range(k)]

@@ -2611,7 +2611,8 @@ static T* at(char* block, size_t off) {
 
 template <typename R>
 int slotPrepare(const ccdh_data* d, const ccdh_fit_spec* sps, int F, char* block,
-                const SlotLayout& L, Slot<R>* out, FitStatus* st, hipStream_t s) {
+                const SlotLayout& L, Slot<R>* out, FitStatus* st, hipStream_t s,
+                const double* sharedKw) {
   const int K = d->K, J = d->J;
   if (F < 1 || F > kMaxBundle || (F & (F - 1)) != 0 || (F > 1 && d->model == CCDH_COX))
     return setError(CCDH_E_INVALID, "a slot holds 1, 2, 4 or 8 fits (Cox: 1)");
@@ -2678,7 +2679,9 @@ int slotPrepare(const ccdh_data* d, const ccdh_fit_spec* sps, int F, char* block
   HIP_TRY(hipMemcpyAsync(x.ptype, ht.data(), ht.size(), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(x.fixed, sp.fixed, J, hipMemcpyHostToDevice, s));
   std::vector<double> hw;
-  if (sp.weights) {
+  if (sp.weights && F == 1 && sharedKw) {
+    x.kw = const_cast<double*>(sharedKw);  // read-only on the device
+  } else if (sp.weights) {
     if (F == 1) {
       HIP_TRY(hipMemcpyAsync(x.kw, sp.weights, K * sizeof(double), hipMemcpyHostToDevice, s));
     } else {
@@ -2712,9 +2715,11 @@ int slotPrepare(const ccdh_data* d, const ccdh_fit_spec* sps, int F, char* block
 }
 
 template int slotPrepare<double>(const ccdh_data*, const ccdh_fit_spec*, int, char*,
-                                 const SlotLayout&, Slot<double>*, FitStatus*, hipStream_t);
+                                 const SlotLayout&, Slot<double>*, FitStatus*, hipStream_t,
+                                 const double*);
 template int slotPrepare<float>(const ccdh_data*, const ccdh_fit_spec*, int, char*,
-                                const SlotLayout&, Slot<float>*, FitStatus*, hipStream_t);
+                                const SlotLayout&, Slot<float>*, FitStatus*, hipStream_t,
+                                const double*);
 
 }  // namespace ccdamd
 
@@ -2944,9 +2949,26 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
   std::vector<FitStatus> st(nFit);
   std::vector<int32_t> active(nSlot);
   std::vector<int> boundCount(nFit, 1);
+  // Slots of one fit whose specs pass the same host weights array (a CV grid's
+  // fits of one fold, given one array per fold) read the first such slot's
+  // device copy of it and take over its weight-only set-up (X'Y, group weight
+  // sums) instead of repeating the upload and the pass over every nonzero.
+  std::vector<int> shareOf(nSlot, -1);
+  {
+    std::map<const double*, int> firstOf;
+    for (int b = 0; b < nSlot; ++b) {
+      const double* w = specs[fit0[b]].weights;
+      if (fitsOf[b] != 1 || w == nullptr) continue;
+      const auto it = firstOf.find(w);
+      if (it == firstOf.end())
+        firstOf.emplace(w, b);
+      else
+        shareOf[b] = it->second;
+    }
+  }
   for (int b = 0; b < nSlot; ++b) {
     rc = slotPrepare<R>(d, specs + fit0[b], fitsOf[b], arena + (size_t)b * L.bytes, L, &slots[b],
-                        &st[fit0[b]], s);
+                        &st[fit0[b]], s, shareOf[b] >= 0 ? slots[shareOf[b]].kw : nullptr);
     if (rc) return rc;
     slots[b].fit0 = fit0[b];
     slots[b].ctl.solo = (opt && opt->solo < 0) ? -1 : 0;
@@ -2964,8 +2986,39 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
     for (int b = 0; b < nSlot; ++b) slots[b].N = st[fit0[b]].N;
   }
   const int setupRanks = initRanks(d->nnz, d->K);
-  rc = slotLaunchInit(model, mixed, dslots, dst, dactive, nSlot, threads, s, true, setupRanks);
-  if (rc) return rc;
+  const bool grouped = model == CCDH_CONDITIONAL_LOGISTIC || model == CCDH_COX;
+  std::vector<int32_t> setupOrder;  // slots with their own set-up, then those sharing one
+  for (int b = 0; b < nSlot; ++b) {
+    if (shareOf[b] >= 0 && slots[b].N != slots[shareOf[b]].N) shareOf[b] = -1;
+    if (shareOf[b] < 0) setupOrder.push_back(b);
+  }
+  const int nOwn = (int)setupOrder.size();
+  for (int b = 0; b < nSlot; ++b)
+    if (shareOf[b] >= 0) setupOrder.push_back(b);
+  if (nOwn == nSlot) {
+    rc = slotLaunchInit(model, mixed, dslots, dst, dactive, nSlot, threads, s, true, setupRanks);
+    if (rc) return rc;
+  } else {
+    HIP_TRY(hipMemcpyAsync(dactive, setupOrder.data(), nSlot * sizeof(int32_t),
+                           hipMemcpyHostToDevice, s));
+    rc = slotLaunchInit(model, mixed, dslots, dst, dactive, nOwn, threads, s, true, setupRanks);
+    if (rc) return rc;
+    // X'Y and the group weight sums (the denominators beside them are rebuilt
+    // by the set-up of the slot's own rows), then the rest of the set-up
+    for (int k = nOwn; k < nSlot; ++k) {
+      const int b = setupOrder[k], a = shareOf[b];
+      HIP_TRY(hipMemcpyAsync(slots[b].xjy, slots[a].xjy, (size_t)J * sizeof(double),
+                             hipMemcpyDeviceToDevice, s));
+      if (grouped)
+        HIP_TRY(hipMemcpyAsync(slots[b].grp, slots[a].grp, (size_t)slots[a].N * sizeof(GrpSt),
+                               hipMemcpyDeviceToDevice, s));
+    }
+    rc = slotLaunchInit(model, mixed, dslots, dst, dactive + nOwn, nSlot - nOwn, threads, s, false,
+                        setupRanks);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(dactive, active.data(), nSlot * sizeof(int32_t), hipMemcpyHostToDevice,
+                           s));
+  }
   // work per sweep of each slot (its fits; Cox: its own tie-group count) for the gang sizes
   std::vector<int64_t> work(nSlot);
   for (int b = 0; b < nSlot; ++b) work[b] = slotWork(d, st[fit0[b]].N) * fitsOf[b];
@@ -3050,6 +3103,7 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
     return 0;
   };
   std::memset(&g_stats, 0, sizeof(g_stats));
+  g_stats.shared_setups = nSlot - nOwn;
   g_stats.tuned_pass_items = itl;
   g_stats.tuned_max_gang = gangCap;
   g_stats.tuned_threads = clampThreads(threads);

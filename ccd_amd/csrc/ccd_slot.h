@@ -157,10 +157,13 @@ SlotLayout slotLayout(int model, int K, int J, int nCap, int64_t maxLen, bool he
 // weights), the fixed log-likelihood terms, and the interleaved uploads of
 // coefficients, priors, weights and groupings into the slot's block.  `out`
 // receives the device pointers (fit0 = 0); st[0..F) the initial statuses.
+// With sharedKw (F = 1) the slot reads those device weights, another slot's
+// copy of the same host array, instead of uploading its own.
 // Returns a ccdh status code.
 template <typename R>
 int slotPrepare(const ccdh_data* d, const ccdh_fit_spec* sp, int F, char* block,
-                const SlotLayout& L, Slot<R>* out, FitStatus* st, hipStream_t s);
+                const SlotLayout& L, Slot<R>* out, FitStatus* st, hipStream_t s,
+                const double* sharedKw = nullptr);
 
 // Groups a fit of this data has under weights w (K for independent rows,
 // strata for CLR, tie groups for Cox).

@@ -323,6 +323,9 @@ typedef struct {
   int32_t bundles;         /* slots the fits were bundled into */
   int32_t tuned_threads;   /* workgroup size of the sweep launches */
   int32_t gang_restarts;   /* slots restarted cold on one workgroup after a gang wait gave up */
+  int32_t shared_setups;   /* one-fit slots whose specs passed another slot's weights array:
+                              they read its device copy and took over its X'Y and group
+                              weight sums */
 } ccdh_batch_stats;
 int ccdh_batch_last_stats(ccdh_batch_stats* out);
 

@@ -119,6 +119,35 @@ def test_batch_gangs_match_oracle(name, gang, solo):
                                    rtol=1e-8)
 
 
+@pytest.mark.parametrize("name", ["lr", "poisson", "clr", "cox"])
+def test_batch_shared_fold_weights(name):
+    """A CV grid given one weights array per fold: each array is uploaded once and
+    the fold's other fits take X'Y and the group weight sums of its first fit.
+    The fits equal those of the same batch given a copy of the weights per fit
+    (bit for bit on one workgroup per fit) and match the oracle (also on gangs)."""
+    import dataclasses
+    p, prior, var, maxit = SYNTH[name]()
+    by_pid = p.model == CONDITIONAL_LOGISTIC
+    ids = p.strata if by_pid else np.zeros(p.K, np.int32)
+    fold_w = [orc.cv_weights(ids, 3, 17, f, by_pid=by_pid) for f in range(3)]
+    shared = [default_spec(p, prior=prior, variance=v * var, weights=fold_w[f],
+                           max_iterations=maxit) for v in (0.05, 0.5, 1.0) for f in range(3)]
+    copies = [dataclasses.replace(sp, weights=sp.weights.copy()) for sp in shared]
+    d = api.Data(p)
+    bs, infs = api.fit_batch(d, shared, max_gang=1)
+    assert api.batch_last_stats()["shared_setups"] == 6
+    bc, infc = api.fit_batch(d, copies, max_gang=1)
+    assert api.batch_last_stats()["shared_setups"] == 0
+    np.testing.assert_array_equal(bs, bc)
+    assert [r["iterations"] for r in infs] == [r["iterations"] for r in infc]
+    bg, infg = api.fit_batch(d, shared, max_gang=8, gang_min_nnz=1)
+    assert api.batch_last_stats()["shared_setups"] == 6
+    for i, sp in enumerate(shared):
+        br, ir = orc.fit(p, sp)
+        check_fit(bs[i], infs[i], br, ir)
+        check_fit(bg[i], infg[i], br, ir)
+
+
 def _stratified_cox(n_strata=5):
     """The synthetic Cox design split into contiguous row blocks as strata (rows
     stay sorted by stratum, -time, y), so risk sets restart inside rank ranges."""
