@@ -230,8 +230,8 @@ __device__ __forceinline__ void updCoxLds(const Slot<R>& s, LDS& S, int* key, in
 }
 
 // Device ticks per nonzero of this workgroup's last column of more than one
-// pass in this launch (0 before the first): thread 0 keeps it, and it prices the
-// next such column when the quantum is checked.
+// pass (the slot's lead carries it from launch to launch): thread 0 keeps it,
+// and it prices the next such column when the quantum is checked.
 __device__ __forceinline__ float& longTicksPerNnz() {
   __shared__ float sTpn;
   return sTpn;
@@ -2085,7 +2085,7 @@ __global__ void __launch_bounds__(BLOCK)
   if (!running) return;
   if (threadIdx.x == 0) {
     sStart = wall_clock64();
-    longTicksPerNnz() = 0.f;
+    longTicksPerNnz() = fs[0].longTpn;
   }
   __syncthreads();
   const long long deadline = quantum > 0 ? sStart + quantum : 0;
@@ -2123,6 +2123,7 @@ __global__ void __launch_bounds__(BLOCK)
   }
   if constexpr (ONLY >= 0) {
     sweepSlot<ONLY, R, ITL>(s, fs, S, sweeps, deadline, gg);
+    if (threadIdx.x == 0 && gg.rank == 0) fs[0].longTpn = longTicksPerNnz();
   } else {
     switch (s.model) {
       case M_LOGISTIC: sweepSlot<M_LOGISTIC, R, ITL>(s, fs, S, sweeps, deadline, gg); break;
@@ -2130,6 +2131,7 @@ __global__ void __launch_bounds__(BLOCK)
       case M_CLR: sweepSlot<M_CLR, R, ITL>(s, fs, S, sweeps, deadline, gg); break;
       case M_COX: sweepSlot<M_COX, R, ITL>(s, fs, S, sweeps, deadline, gg); break;
     }
+    if (threadIdx.x == 0 && gg.rank == 0) fs[0].longTpn = longTicksPerNnz();
   }
 }
 

@@ -52,6 +52,9 @@ struct FitStatus {
   long long gatherNnz, gatherRuns, updateNnz, updateRuns, scanPids, objRows;
   // wall-clock ticks (thread 0) spent per phase: single-pass columns, long columns, objective
   long long tSmall, tLarge, tObj, nLarge;
+  // ticks per nonzero of the lead rank's last column of more than one pass,
+  // carried from launch to launch (0 before the first)
+  float longTpn;
 };
 
 // gather-pass records of long columns, read back by the update pass
