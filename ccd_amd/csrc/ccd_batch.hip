@@ -508,7 +508,7 @@ __device__ __forceinline__ void loadRecords(const DevX& d, int fmt, int64_t e, i
 }
 
 // Per-fit totals of NV values when lane u carries fit u & (F-1): every lane gets
-// the totals of its own fit.  F = 1 sums in blockSum's order.
+// the totals of its own fit.  F = 1 sums in blockSum's order (waveSum).
 template <int NV>
 __device__ __forceinline__ void blockSumFits(double (&v)[NV], int F) {
   __shared__ double sh[NV][kMaxBlockThreads / kWave][kMaxBundle];
@@ -516,9 +516,14 @@ __device__ __forceinline__ void blockSumFits(double (&v)[NV], int F) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = threadIdx.x / kWave;
   const int nw = (blockDim.x + kWave - 1) / kWave;
+  if (F == 1) {  // uniform
 #pragma unroll
-  for (int i = 0; i < NV; ++i)
-    for (int off = kWave / 2; off >= F; off >>= 1) v[i] += __shfl_down(v[i], off, kWave);
+    for (int i = 0; i < NV; ++i) v[i] = waveSum(v[i]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+      for (int off = kWave / 2; off >= F; off >>= 1) v[i] += __shfl_down(v[i], off, kWave);
+  }
   if (lane < F) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) sh[i][wid][lane] = v[i];

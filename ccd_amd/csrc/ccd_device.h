@@ -78,10 +78,24 @@ __device__ __forceinline__ double rexp(double x) { return exp(x); }
 __device__ __forceinline__ float rexp(float x) { return expf(x); }
 
 // ---------------------------------------------------------------- reductions
+// Sum over the 64 lanes of a wave, returned to every lane; every lane must be
+// active.  DPP row shifts by 1, 2, 4 and 8 leave each 16-lane row's inclusive
+// sums in the row (lane 15 holds the row's total), row_bcast:15 adds row 0's
+// total to row 1 and row 2's to row 3, row_bcast:31 adds lane 31 (rows 0-1) to
+// rows 2-3, so lane 63 ends with the total; it is read back as two 32-bit
+// halves.  A fixed order of additions, and register moves instead of LDS
+// permutes.
 __device__ __forceinline__ double waveSum(double v) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
-  return v;
+  v += __builtin_amdgcn_update_dpp(0.0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0.0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0.0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0.0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0.0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0.0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, kWave - 1);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), kWave - 1);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
 // p[0] + p[stride] + ... + p[(nw - 1) * stride] from LDS, added in that order
