@@ -1199,26 +1199,6 @@ __device__ __forceinline__ void waveRange(int pLo, int pHi, int* a, int* z) {
   *z = pLo + min(n, (wid + 1) * per);
 }
 
-template <int NV>
-__device__ __forceinline__ Seg<NV> waveScanIncl(Seg<NV> x) {
-  const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    const Seg<NV> o = segShflUp(x, off);
-    if (lane >= off) x = segCombine(o, x);
-  }
-  return x;
-}
-
-template <int NV>
-__device__ __forceinline__ Seg<NV> segFromLane(const Seg<NV>& a, int src) {
-  Seg<NV> r;
-#pragma unroll
-  for (int i = 0; i < NV; ++i) r.v[i] = __shfl(a.v[i], src, kWave);
-  r.f = __shfl(a.f, src, kWave);
-  return r;
-}
-
 // Each wave's total -> the combination of the earlier waves' totals (this wave's
 // carry) and of all of them (`total`), in wave order.
 template <int NV>
@@ -1252,15 +1232,65 @@ __device__ __forceinline__ Seg<NV> scanCombine(const Seg<NV>& a, const Seg<NV>& 
   }
 }
 
+// DPP moves between lanes: a lane with no source lane, or in a row the mask
+// leaves out, receives 0 (a Seg of zeros with no reset is the identity of the
+// combines above).  Every lane of the wave must be active.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dppD(double v) {
+  return __builtin_amdgcn_update_dpp(0.0, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dppI(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+}
+
+template <int NV, bool SEGS, int CTRL, int ROWS>
+__device__ __forceinline__ void dppScanStep(Seg<NV>& x) {
+  Seg<NV> o;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) o.v[i] = dppD<CTRL, ROWS>(x.v[i]);
+  o.f = SEGS ? dppI<CTRL, ROWS>(x.f) : 0;
+  x = scanCombine<NV, SEGS>(o, x);
+}
+
+// Inclusive (segmented) scan over the wave's lanes in lane order: combining
+// with the lanes 1, 2, 4 and 8 back inside each 16-lane row gives every lane its
+// row prefix; row_bcast:15 brings row 0's total into row 1 and row 2's into row
+// 3, row_bcast:31 brings lane 31's (rows 0-1) into rows 2 and 3.
 template <int NV, bool SEGS>
 __device__ __forceinline__ Seg<NV> waveScanInclS(Seg<NV> x) {
-  const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    const Seg<NV> o = segShflUp(x, off);
-    if (lane >= off) x = scanCombine<NV, SEGS>(o, x);
-  }
+  dppScanStep<NV, SEGS, 0x111, 0xf>(x);  // row_shr:1
+  dppScanStep<NV, SEGS, 0x112, 0xf>(x);  // row_shr:2
+  dppScanStep<NV, SEGS, 0x114, 0xf>(x);  // row_shr:4
+  dppScanStep<NV, SEGS, 0x118, 0xf>(x);  // row_shr:8
+  dppScanStep<NV, SEGS, 0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
+  dppScanStep<NV, SEGS, 0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
   return x;
+}
+
+template <int NV>
+__device__ __forceinline__ Seg<NV> waveScanIncl(Seg<NV> x) {
+  return waveScanInclS<NV, true>(x);
+}
+
+// Lane i - 1's value in lane i (wave_shr:1), the identity in lane 0.
+template <int NV>
+__device__ __forceinline__ Seg<NV> segPrevLane(const Seg<NV>& a) {
+  Seg<NV> r;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) r.v[i] = dppD<0x138, 0xf>(a.v[i]);
+  r.f = dppI<0x138, 0xf>(a.f);
+  return r;
+}
+
+// Lane 63's value, to every lane.
+template <int NV>
+__device__ __forceinline__ Seg<NV> segLastLane(const Seg<NV>& a) {
+  Seg<NV> r;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) r.v[i] = lastLane(a.v[i]);
+  r.f = __builtin_amdgcn_readlane(a.f, kWave - 1);
+  return r;
 }
 
 // Gradient and Hessian terms of one coordinate over this rank's pids [pLo, pHi)
@@ -1318,10 +1348,10 @@ __device__ __forceinline__ bool coxScanSmall(const Slot<R>& s, int pLo, int pHi,
 #pragma unroll
   for (int u = 0; u < kScanSmall; ++u) l = scanCombine<NV, SEGS>(l, it[u]);
   const Seg<NV> inc = waveScanInclS<NV, SEGS>(l);
-  Seg<NV> r = segShflUp(inc, 1);
+  Seg<NV> r = segPrevLane(inc);
   if (lane == 0) r = segZero<NV>();
   Seg<NV> total;
-  Seg<NV> run = waveCarries<NV>(segFromLane(inc, kWave - 1), total);
+  Seg<NV> run = waveCarries<NV>(segLastLane(inc), total);
   if (gg.G > 1) {
     Seg<NV> carry;
     if (!gangCarry<NV>(gg, total, carry)) return false;
@@ -1380,7 +1410,7 @@ __device__ __forceinline__ bool coxScanT(const Slot<R>& s, int pLo, int pHi, Gan
         l = scanCombine<NV, SEGS>(l, x);
       }
     }
-    wagg = scanCombine<NV, SEGS>(wagg, segFromLane(waveScanInclS<NV, SEGS>(l), kWave - 1));
+    wagg = scanCombine<NV, SEGS>(wagg, segLastLane(waveScanInclS<NV, SEGS>(l)));
   }
   Seg<NV> total;
   Seg<NV> run = waveCarries<NV>(wagg, total);
@@ -1400,7 +1430,7 @@ __device__ __forceinline__ bool coxScanT(const Slot<R>& s, int pLo, int pHi, Gan
       l = scanCombine<NV, SEGS>(l, it[u]);
     }
     const Seg<NV> inc = waveScanInclS<NV, SEGS>(l);
-    Seg<NV> r = segShflUp(inc, 1);
+    Seg<NV> r = segPrevLane(inc);
     if (lane == 0) r = segZero<NV>();
     r = scanCombine<NV, SEGS>(run, r);
 #pragma unroll
@@ -1422,7 +1452,7 @@ __device__ __forceinline__ bool coxScanT(const Slot<R>& s, int pLo, int pHi, Gan
         if (it[u].v[2] != 0.0) numer2[i] = 0.0;
       }
     }
-    run = scanCombine<NV, SEGS>(run, segFromLane(inc, kWave - 1));
+    run = scanCombine<NV, SEGS>(run, segLastLane(inc));
   }
   return true;
 }
@@ -1461,7 +1491,7 @@ __device__ __forceinline__ double coxLogDen(const Slot<R>& s, int pLo, int pHi, 
         l = segCombine(l, x);
       }
     }
-    wagg = segCombine(wagg, segFromLane(waveScanIncl(l), kWave - 1));
+    wagg = segCombine(wagg, segLastLane(waveScanIncl(l)));
   }
   Seg<1> total;
   Seg<1> run = waveCarries<1>(wagg, total);
@@ -1490,7 +1520,7 @@ __device__ __forceinline__ double coxLogDen(const Slot<R>& s, int pLo, int pHi, 
       l = segCombine(l, it[u]);
     }
     const Seg<1> inc = waveScanIncl(l);
-    Seg<1> r = segShflUp(inc, 1);
+    Seg<1> r = segPrevLane(inc);
     if (lane == 0) r = segZero<1>();
     r = segCombine(run, r);
 #pragma unroll
@@ -1499,7 +1529,7 @@ __device__ __forceinline__ double coxLogDen(const Slot<R>& s, int pLo, int pHi, 
       r = segCombine(r, it[u]);
       if (wn[u] != 0.0) acc += wn[u] * log(r.v[0]);
     }
-    run = segCombine(run, segFromLane(inc, kWave - 1));
+    run = segCombine(run, segLastLane(inc));
   }
   return acc;
 }

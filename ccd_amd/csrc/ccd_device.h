@@ -78,6 +78,14 @@ __device__ __forceinline__ double rexp(double x) { return exp(x); }
 __device__ __forceinline__ float rexp(float x) { return expf(x); }
 
 // ---------------------------------------------------------------- reductions
+// Lane 63's value, to every lane (two 32-bit reads of the lane).
+__device__ __forceinline__ double lastLane(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, kWave - 1);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), kWave - 1);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // Sum over the 64 lanes of a wave, returned to every lane; every lane must be
 // active.  DPP row shifts by 1, 2, 4 and 8 leave each 16-lane row's inclusive
 // sums in the row (lane 15 holds the row's total), row_bcast:15 adds row 0's
@@ -92,10 +100,7 @@ __device__ __forceinline__ double waveSum(double v) {
   v += __builtin_amdgcn_update_dpp(0.0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
   v += __builtin_amdgcn_update_dpp(0.0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
   v += __builtin_amdgcn_update_dpp(0.0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
-  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, kWave - 1);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), kWave - 1);
-  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+  return lastLane(v);
 }
 
 // p[0] + p[stride] + ... + p[(nw - 1) * stride] from LDS, added in that order
