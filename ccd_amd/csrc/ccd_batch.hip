@@ -3153,8 +3153,21 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
     // the starting point (every slot's state and every fit's status after
     // set-up), copied aside before the first trial and copied back after each;
     // without room for the copy, the coefficients are uploaded again and the
-    // set-up kernel re-run instead
+    // set-up kernel re-run instead.  A sweep writes three ranges of a slot's
+    // block: the row state, the group state (Cox: and its numerators), the
+    // coefficients and bounds; the rest is read-only or scratch (records).
     const size_t arenaBytes = (size_t)nSlot * L.bytes;
+    struct Range {
+      size_t at, end;
+    };
+    std::vector<Range> mutableRanges = {{L.row, L.kw}, {L.beta, L.pvar}};
+    if (L.grp != kNone) mutableRanges.push_back({L.grp, model == CCDH_COX ? L.pid : L.xjy});
+    auto copySlots = [&](char* dstBase, const char* srcBase) -> int {
+      for (const Range& r : mutableRanges)
+        HIP_TRY(hipMemcpy2DAsync(dstBase + r.at, L.bytes, srcBase + r.at, L.bytes, r.end - r.at,
+                                 nSlot, hipMemcpyDeviceToDevice, s));
+      return 0;
+    };
     char* snapArena = nullptr;
     FitStatus* snapSt = nullptr;
     bool snapTried = false;
@@ -3170,7 +3183,8 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
                 hipMalloc((void**)&snapSt, nFit * sizeof(FitStatus)) == hipSuccess) {
               o.ptrs.push_back(snapArena);
               o.ptrs.push_back(snapSt);
-              HIP_TRY(hipMemcpyAsync(snapArena, arena, arenaBytes, hipMemcpyDeviceToDevice, s));
+              const int cr = copySlots(snapArena, arena);
+              if (cr) return cr;
               HIP_TRY(hipMemcpyAsync(snapSt, dst, nFit * sizeof(FitStatus),
                                      hipMemcpyDeviceToDevice, s));
             } else {
@@ -3187,7 +3201,8 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
         },
         [&]() -> int {
           if (snapSt) {
-            HIP_TRY(hipMemcpyAsync(arena, snapArena, arenaBytes, hipMemcpyDeviceToDevice, s));
+            const int cr = copySlots(arena, snapArena);
+            if (cr) return cr;
             HIP_TRY(hipMemcpyAsync(dst, snapSt, nFit * sizeof(FitStatus), hipMemcpyDeviceToDevice,
                                    s));
             HIP_TRY(hipMemcpyAsync(dactive, active.data(), nSlot * sizeof(int32_t),
