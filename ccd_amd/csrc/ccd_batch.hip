@@ -2819,7 +2819,7 @@ constexpr long long kTrialTicks = 2500000;  // 25 ms of device time per candidat
 int timeTrial(bool mixed, const void* dslots, FitStatus* dst, const GangEntry* dmap, int mapN,
               Xch* dxch, bool gangs, int threads, int itl, int nFit, int J,
               const std::vector<int>& fit0, const std::vector<int>& fitsOf, Owned& o,
-              hipStream_t s, double* rate, int model) {
+              hipStream_t s, double* rate, int model, float* trialMs) {
   HIP_TRY(hipEventRecord(o.ev0, s));
   int rc = slotLaunchSweeps(mixed, dslots, dst, dmap, mapN, dxch, gangs, threads, itl, 1 << 30,
                             kTrialTicks, s, model);
@@ -2829,6 +2829,7 @@ int timeTrial(bool mixed, const void* dslots, FitStatus* dst, const GangEntry* d
   HIP_TRY(hipStreamSynchronize(s));
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, o.ev0, o.ev1));
+  *trialMs = ms;
   double coords = 0.0;
   for (size_t b = 0; b < fit0.size(); ++b) {
     const FitStatus& h = o.hst[fit0[b]];
@@ -2843,10 +2844,12 @@ int timeTrial(bool mixed, const void* dslots, FitStatus* dst, const GangEntry* d
 // workload (one quantum per candidate, back to the starting point after each);
 // the choice is cached per workload.  With the workgroup size left open the
 // shapes are 256 x 8 and 512 x 4 (two waves per SIMD, the same pass width);
-// a caller's own size keeps it and varies the pass width.
+// a caller's own size keeps it and varies the pass width.  When the last
+// candidate timed wins, its launch is kept as the batch's first (*kept).
 template <class Trial, class Reset>
 int tuneBatch(const TuneKey& key, int threads, bool autoThreads, int maxGang, Trial trial,
-              Reset reset, TuneChoice* out) {
+              Reset reset, TuneChoice* out, bool* kept) {
+  *kept = false;
   const auto t0 = std::chrono::steady_clock::now();
   {
     std::lock_guard<std::mutex> g(g_tuneMu);
@@ -2857,9 +2860,10 @@ int tuneBatch(const TuneKey& key, int threads, bool autoThreads, int maxGang, Tr
     }
   }
   std::vector<TuneChoice> cands;
-  if (autoThreads) {
-    cands = {{256, 8, maxGang}, {512, 4, maxGang}};
+  if (autoThreads) {  // the shape that usually wins last, so that its launch is kept
+    cands = {{256, 8, maxGang}};
     if (maxGang > 1) cands.push_back({256, 8, 1});
+    cands.push_back({512, 4, maxGang});
   } else {
     const int T = clampThreads(threads);
     cands = {{T, 8, maxGang}};
@@ -2876,6 +2880,10 @@ int tuneBatch(const TuneKey& key, int threads, bool autoThreads, int maxGang, Tr
     if (rate > best) {
       best = rate;
       pick = cands[c];
+      if (c + 1 == cands.size()) {  // the fits go on from this launch
+        *kept = true;
+        break;
+      }
     }
     rc = reset();
     if (rc) return rc;
@@ -3171,6 +3179,8 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
     char* snapArena = nullptr;
     FitStatus* snapSt = nullptr;
     bool snapTried = false;
+    float lastTrialMs = 0.f;
+    bool keptTrial = false;
     rc = tuneBatch(
         key, threads, autoThreads, maxGang,
         [&](const TuneChoice& cand, double* rate) -> int {
@@ -3197,7 +3207,7 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
           int r = buildMap();
           if (r) return r;
           return timeTrial(mixed, dslots, dst, dmap, mapN, dxch, mapGangs, threads, itl, nFit, J,
-                           fit0, fitsOf, o, s, rate, model);
+                           fit0, fitsOf, o, s, rate, model, &lastTrialMs);
         },
         [&]() -> int {
           if (snapSt) {
@@ -3225,11 +3235,18 @@ int runBatch(ccdh_data* d, int nFit, const ccdh_fit_spec* specs, double* betasOu
           HIP_TRY(hipStreamSynchronize(s));
           return 0;
         },
-        &choice);
+        &choice, &keptTrial);
     if (rc) return rc;
     threads = choice.threads;
     itl = choice.itl;
     gangCap = choice.gang;
+    if (keptTrial) {  // the winner's trial launch is the batch's first
+      g_stats.kernel_ms += lastTrialMs;
+      g_stats.launch_ms[0] = lastTrialMs;
+      g_stats.launch_fits[0] = curFits;
+      g_stats.launch_gang[0] = curGang;
+      g_stats.launches = 1;
+    }
     g_stats.tuned_pass_items = itl;
     g_stats.tuned_max_gang = gangCap;
     g_stats.tuned_threads = clampThreads(threads);

@@ -316,7 +316,10 @@ typedef struct {
   float launch_ms[128];
   int32_t launch_fits[128];
   int32_t launch_gang[128];
-  /* what the batch ran with; with autotune, the candidates' rates (coordinates per ms) */
+  /* what the batch ran with; with autotune, the candidates' rates (coordinates per ms) in the
+     order timed: 256 x 8 on gangs, 256 x 8 on one workgroup, 512 x 4 on gangs (a caller's own
+     workgroup size T: T x 8 on gangs, T x 8 on one workgroup, then T x 4 on gangs when T is
+     256); when the last one timed wins, its launch is the batch's first (launch_ms[0]) */
   int32_t tuned_pass_items, tuned_max_gang;
   double tune_ms;
   double tune_rates[4];
